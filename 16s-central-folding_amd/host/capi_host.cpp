@@ -1,0 +1,50 @@
+// C entry points of the host side: system construction and error reporting.
+#include <cstdlib>
+#include <new>
+#include <string>
+
+#include "../../include/tis_gpu.h"
+#include "host_util.h"
+
+namespace tis_host {
+tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws);
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+}  // namespace tis_host
+
+extern "C" const char *tis_last_error(void) { return tis_host::g_error.c_str(); }
+
+extern "C" int tis_build_16scd(const char *data_dir, const tis_build_args *args, tis_system **out,
+                               long long *rand_draws_out) {
+    if (!args || !out) { tis_host::set_error("tis_build_16scd: null argument"); return TIS_EINVAL; }
+    *out = nullptr;
+    if (!(args->T > 0) || args->c_mg < 0 || args->c_k < 0 || !(args->box >= 0)) {
+        tis_host::set_error("tis_build_16scd: T must be > 0 and concentrations >= 0");
+        return TIS_EINVAL;
+    }
+    try {
+        *out = tis_host::build_16scd(data_dir ? data_dir : ".", *args, rand_draws_out);
+        return TIS_OK;
+    } catch (const tis_host::IoError &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    } catch (const std::bad_alloc &) {
+        tis_host::set_error("tis_build_16scd: out of host memory");
+        return TIS_ENOMEM;
+    } catch (const std::exception &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    }
+}
+
+extern "C" void tis_system_free(tis_system *s) {
+    if (!s) return;
+    void *ptrs[] = {s->type, s->cls, s->pos, s->pos_old, s->vel, s->unit_bead0, s->unit_bead1,
+                    s->unit_kind, s->bond_ij, s->bond_kr, s->angle_ijk, s->angle_kt, s->st3_beads,
+                    s->st3_geo, s->st3_E, s->st3_face, s->st3_disable, s->st2_beads, s->st2_r0,
+                    s->st2_E, s->st_excess, s->hb_beads, s->hb_geo, s->hb_E, s->hb_dode,
+                    s->hb_site_off, s->hb_site_idx, s->site_hb_off, s->site_hb_idx, s->site_valence};
+    for (void *p : ptrs) std::free(p);
+    std::free(s);
+}

@@ -1,0 +1,206 @@
+"""ctypes binding of the C ABI in include/tis_gpu.h.
+
+Thin plumbing for tests, smoke() and bench.py: it loads the in-tree
+libtis_mi355x.so, mirrors the tis_system / tis_options / tis_energies structs
+and converts systems to and from numpy dictionaries.  No physics lives here.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtis_mi355x.so")
+DATA_16SCD = os.path.join(HERE, "data", "16SCD")
+
+NTYPES = 9
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+class TisSystem(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int), ("n_beads", C.c_int), ("n_units", C.c_int),
+        ("n_bonds", C.c_int), ("n_angles", C.c_int),
+        ("n_st3", C.c_int), ("n_st2", C.c_int),
+        ("n_hb", C.c_int), ("n_hb_sites", C.c_int), ("n_hb_incid", C.c_int),
+        ("n_mg", C.c_int), ("n_cl", C.c_int), ("n_k", C.c_int),
+        ("box", C.c_double * 3), ("T", C.c_double), ("lB", C.c_double), ("h", C.c_double),
+        ("recentre", C.c_int),
+        ("type_mass", C.c_double * NTYPES), ("type_R", C.c_double * NTYPES),
+        ("type_eps", C.c_double * NTYPES), ("type_q", C.c_double * NTYPES),
+        ("type", _ip), ("cls", _ip),
+        ("pos", _dp), ("pos_old", _dp), ("vel", _dp),
+        ("unit_bead0", _ip), ("unit_bead1", _ip), ("unit_kind", C.POINTER(C.c_char)),
+        ("bond_ij", _ip), ("bond_kr", _dp), ("angle_ijk", _ip), ("angle_kt", _dp),
+        ("st3_beads", _ip), ("st3_geo", _dp), ("st3_E", _dp), ("st3_face", _ip), ("st3_disable", _ip),
+        ("st2_beads", _ip), ("st2_r0", _dp), ("st2_E", _dp), ("st_excess", _ip),
+        ("hb_beads", _ip), ("hb_geo", _dp), ("hb_E", _dp), ("hb_dode", _ip),
+        ("hb_site_off", _ip), ("hb_site_idx", _ip),
+        ("site_hb_off", _ip), ("site_hb_idx", _ip), ("site_valence", _ip),
+    ]
+
+
+class BuildArgs(C.Structure):
+    _fields_ = [("ss_D", C.c_double), ("hs_D", C.c_double), ("st_D", C.c_double),
+                ("T", C.c_double), ("c_mg", C.c_double), ("c_k", C.c_double),
+                ("seed", C.c_int), ("box", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
+                ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
+                ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
+                ("graph_steps", C.c_int)]
+
+
+class Energies(C.Structure):
+    _fields_ = [("e2_hb", C.c_double), ("e3_hb", C.c_double), ("e2_stack", C.c_double),
+                ("e3_stack", C.c_double), ("e_wca", C.c_double), ("e_coul_sr", C.c_double),
+                ("e_coul_lr", C.c_double), ("e_bond", C.c_double), ("e_angle", C.c_double),
+                ("rg", C.c_double), ("klok", C.c_longlong), ("n_rebuilds", C.c_longlong),
+                ("progress", C.c_double), ("n_sr_pairs", C.c_longlong),
+                ("n_lr_pairs", C.c_longlong), ("n_hb_active", C.c_longlong)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+PREC_FP64, PREC_MIXED = 0, 1
+RNG_PHILOX, RNG_STREAM = 0, 1
+TERM_STACK, TERM_HB, TERM_BACKBONE, TERM_SR, TERM_LR, TERM_ALL = 1, 2, 4, 8, 16, 31
+
+# (field, dtype, length-expression) for every array of tis_system.
+_ARRAYS = [
+    ("type", np.int32, "N"), ("cls", np.int32, "N"),
+    ("pos", np.float64, "3N"), ("pos_old", np.float64, "3N"), ("vel", np.float64, "3N"),
+    ("unit_bead0", np.int32, "U"), ("unit_bead1", np.int32, "U"), ("unit_kind", np.uint8, "U"),
+    ("bond_ij", np.int32, "2B"), ("bond_kr", np.float64, "2B"),
+    ("angle_ijk", np.int32, "3A"), ("angle_kt", np.float64, "2A"),
+    ("st3_beads", np.int32, "6S3"), ("st3_geo", np.float64, "6S3"), ("st3_E", np.float64, "S3"),
+    ("st3_face", np.int32, "2S3"), ("st3_disable", np.int32, "2S3"),
+    ("st2_beads", np.int32, "7S2"), ("st2_r0", np.float64, "S2"), ("st2_E", np.float64, "S2"),
+    ("st_excess", np.int32, "U"),
+    ("hb_beads", np.int32, "6H"), ("hb_geo", np.float64, "6H"), ("hb_E", np.float64, "H"),
+    ("hb_dode", np.int32, "H"), ("hb_site_off", np.int32, "H+1"), ("hb_site_idx", np.int32, "I"),
+    ("site_hb_off", np.int32, "S+1"), ("site_hb_idx", np.int32, "I"), ("site_valence", np.int32, "S"),
+]
+_SCALARS = ["n_particles", "n_beads", "n_units", "n_bonds", "n_angles", "n_st3", "n_st2",
+            "n_hb", "n_hb_sites", "n_hb_incid", "n_mg", "n_cl", "n_k", "T", "lB", "h", "recentre"]
+
+
+def _length(expr, d):
+    env = {"N": d["n_particles"], "U": d["n_units"], "B": d["n_bonds"], "A": d["n_angles"],
+           "S3": d["n_st3"], "S2": d["n_st2"], "H": d["n_hb"], "I": d["n_hb_incid"],
+           "S": d["n_hb_sites"]}
+    for key in ("S3", "S2"):
+        if expr.endswith(key) and expr != key:
+            return int(expr[:-len(key)]) * env[key]
+    if expr in env:
+        return env[expr]
+    if expr.endswith("+1"):
+        return env[expr[:-2]] + 1
+    return int(expr[:-1]) * env[expr[-1]]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    lib.tis_last_error.restype = C.c_char_p
+    lib.tis_build_16scd.argtypes = [C.c_char_p, C.POINTER(BuildArgs), C.POINTER(C.POINTER(TisSystem)),
+                                    C.POINTER(C.c_longlong)]
+    lib.tis_system_free.argtypes = [C.POINTER(TisSystem)]
+    lib.tis_system_free.restype = None
+    if hasattr(lib, "tis_create"):
+        lib.tis_default_options.argtypes = [C.POINTER(Options)]
+        lib.tis_default_options.restype = None
+        lib.tis_create.argtypes = [C.POINTER(TisSystem), C.c_int, C.POINTER(Options), C.POINTER(C.c_void_p)]
+        lib.tis_destroy.argtypes = [C.c_void_p]
+        lib.tis_destroy.restype = None
+        lib.tis_set_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_longlong]
+        lib.tis_get_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        lib.tis_step.argtypes = [C.c_void_p, C.c_longlong]
+        lib.tis_get_energies.argtypes = [C.c_void_p, C.c_int, C.POINTER(Energies)]
+        lib.tis_get_stack_energies.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
+        lib.tis_eval_forces.argtypes = [C.c_void_p, C.c_int, C.c_uint, _dp, C.POINTER(Energies)]
+        lib.tis_get_pairs.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_longlong,
+                                      C.POINTER(C.c_longlong)]
+        lib.tis_get_hb_status.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        lib.tis_set_uniform_stream.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_longlong]
+        lib.tis_get_stream_cursor.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
+        lib.tis_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    return lib
+
+
+class TisError(RuntimeError):
+    def __init__(self, lib, code, what):
+        msg = lib.tis_last_error().decode(errors="replace")
+        super().__init__(f"{what} failed with {code}: {msg}")
+        self.code = code
+
+
+def check(lib, code, what):
+    if code != 0:
+        raise TisError(lib, code, what)
+
+
+def system_to_dict(sp):
+    """Deep-copies a tis_system (pointer or struct) into a dict of numpy arrays."""
+    s = sp.contents if hasattr(sp, "contents") else sp
+    d = {k: getattr(s, k) for k in _SCALARS}
+    d["box"] = np.array(list(s.box))
+    for k in ("type_mass", "type_R", "type_eps", "type_q"):
+        d[k] = np.array(list(getattr(s, k)))
+    for name, dt, expr in _ARRAYS:
+        n = _length(expr, d)
+        ptr = getattr(s, name)
+        if n == 0:
+            d[name] = np.zeros(0, dtype=dt)
+            continue
+        raw = C.cast(ptr, C.POINTER(C.c_ubyte * (n * np.dtype(dt).itemsize))).contents
+        d[name] = np.frombuffer(bytes(raw), dtype=dt).copy()
+    return d
+
+
+class SystemHandle:
+    """A tis_system struct whose arrays are kept alive by numpy buffers."""
+
+    def __init__(self, d):
+        self.d = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+        s = TisSystem()
+        for k in _SCALARS:
+            setattr(s, k, self.d[k])
+        s.box = (C.c_double * 3)(*[float(v) for v in self.d["box"]])
+        for k in ("type_mass", "type_R", "type_eps", "type_q"):
+            setattr(s, k, (C.c_double * NTYPES)(*[float(v) for v in self.d[k]]))
+        for name, dt, expr in _ARRAYS:
+            arr = np.ascontiguousarray(self.d[name], dtype=dt)
+            self.d[name] = arr
+            ctype = C.c_char if name == "unit_kind" else (C.c_int if dt == np.int32 else C.c_double)
+            setattr(s, name, arr.ctypes.data_as(C.POINTER(ctype)))
+        self.struct = s
+
+    def ptr(self):
+        return C.pointer(self.struct)
+
+
+def build_16scd(lib, T=0.615, c_mg=0.005, c_k=0.050, seed=1, box=350.0,
+                ss_D=0.850, hs_D=3.975, st_D=6.5, data_dir=DATA_16SCD):
+    """Runs the host set-up (tis_build_16scd) and returns (dict, rand draws used)."""
+    args = BuildArgs(ss_D, hs_D, st_D, T, c_mg, c_k, seed, box)
+    out = C.POINTER(TisSystem)()
+    draws = C.c_longlong(0)
+    check(lib, lib.tis_build_16scd(data_dir.encode(), C.byref(args), C.byref(out), C.byref(draws)),
+          "tis_build_16scd")
+    try:
+        d = system_to_dict(out)
+    finally:
+        lib.tis_system_free(out)
+    return d, draws.value
+
+
+def default_options(lib):
+    o = Options()
+    lib.tis_default_options(C.byref(o))
+    return o

@@ -1,0 +1,212 @@
+/*
+ * tis_gpu.h - C ABI of the MI355X TIS-RNA + explicit-ion Langevin engine.
+ *
+ * The reference (src/16SCD_Mg_Cl_K_COPY_fast.cpp, "fast.cpp" below) has no
+ * plugin or FFI interface: its hot path is four void() calls on process-global
+ * arrays, made once per Langevin step from main():
+ *     deterministic_forces_MPI(); full_forces(); move_rigid_units(); check_shifts();
+ * (fast.cpp:6380-6386), plus populate_lists() at start-up (fast.cpp:6237).
+ * Each entry point below names the reference routine(s) it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative TIS_E* code; none aborts.
+ *     tis_last_error() gives a human-readable message for the calling thread.
+ *   - Host buffers passed in are copied; host buffers passed out are filled.
+ *   - Particle order is the reference's, shifted to 0-based: reference particle i
+ *     (1..NT_atom) is index i-1 here.  Beads first (S(n)=3n-3, B(n)=3n-2 and
+ *     P(n)=3n-4 in 0-based terms for 16SCD, which has no 5' phosphate), then Mg,
+ *     then Cl, then K.  Vectors are xyz-interleaved double[3*N].
+ *   - One tis_ctx per host thread; all device work runs on the context's stream.
+ */
+#ifndef TIS_GPU_H
+#define TIS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum {
+    TIS_OK = 0,
+    TIS_EINVAL = -1,    /* bad argument */
+    TIS_ENOMEM = -2,    /* host or device allocation failed */
+    TIS_EIO = -3,       /* input file missing or malformed */
+    TIS_EDEVICE = -4,   /* HIP runtime error */
+    TIS_EOVERFLOW = -5, /* a device list or random stream ran out of room */
+    TIS_ESTATE = -6     /* call out of order (e.g. step before set_state) */
+};
+const char *tis_last_error(void);
+
+/* ------------------------------------------------------------ the system */
+/* Number of parameter types (maxi_key 1..9: P S A G C U Mg Cl K, fast.cpp:2883-2945). */
+#define TIS_NTYPES 9
+
+/*
+ * One replica: topology + force-field tables + state, all 0-based.
+ * Filled by tis_build_16scd() (the reference's set-up, fast.cpp:2883-6031) or
+ * by the caller.  Arrays are owned by whoever allocated the struct.
+ */
+typedef struct tis_system {
+    /* sizes */
+    int n_particles;   /* NT_atom */
+    int n_beads;       /* NTP_atom: RNA beads come first */
+    int n_units;       /* NBP_atom: units 1..n alternate nucleoside (odd) / phosphate (even) */
+    int n_bonds, n_angles;
+    int n_st3, n_st2;  /* tertiary (s3_N) and secondary (st_N - s3_N) stacks */
+    int n_hb;          /* hb_N candidates */
+    int n_hb_sites;    /* HB_NT_atom */
+    int n_hb_incid;    /* total site incidences (sum over HBs of sites listed) */
+    int n_mg, n_cl, n_k;
+    /* box and thermodynamics */
+    double box[3];     /* side_X, side_Y, side_Z */
+    double T;          /* kcal/mol */
+    double lB;         /* Bjerrum length, fast.cpp:6109 */
+    double h;          /* time step in tau, fast.cpp:28 */
+    int recentre;      /* 0-based index of the bead kept at the box centre (ref. bead 500) */
+    /* per type t = maxi_key-1 */
+    double type_mass[TIS_NTYPES], type_R[TIS_NTYPES], type_eps[TIS_NTYPES], type_q[TIS_NTYPES];
+    /* per particle */
+    int *type;         /* maxi_key - 1, 0..8 */
+    int *cls;          /* part_key: 0 P, 1 S, 2 B, 3 Mg, 4 Cl, 5 K */
+    double *pos, *pos_old, *vel;   /* 3*N each */
+    /* units: unit u (0-based) is reference unit u+1 */
+    int *unit_bead0;   /* atom_key[u][0]: S for nucleosides, P for phosphates */
+    int *unit_bead1;   /* atom_key[u][1]: B for nucleosides, -1 for phosphates */
+    char *unit_kind;   /* amino_key: 'A','C','G','U' or 'P' */
+    /* backbone (polymer_constraints, fast.cpp:2354-2520), in the reference's term order */
+    int *bond_ij;      /* 2 per bond */
+    double *bond_kr;   /* k, r0 per bond */
+    int *angle_ijk;    /* 3 per angle, vertex in the middle */
+    double *angle_kt;  /* k, theta0 per angle */
+    /* tertiary stacks: beads S_i B_i P_i+1 S_j B_j P_j+1, geometry r th1 th2 psi psi1 psi2, E */
+    int *st3_beads;    /* 6 per stack */
+    double *st3_geo;   /* 6 per stack */
+    double *st3_E;
+    int *st3_face;     /* ATOM_ST[k][0..1]: 1-based unit index of each face */
+    int *st3_disable;  /* 2 per stack: secondary stack switched off when formed, or -1 */
+    /* secondary stacks: beads P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2, r0, E */
+    int *st2_beads;    /* 7 per stack */
+    double *st2_r0, *st2_E;
+    int *st_excess;    /* ST_EXCESS per unit face (n_units) */
+    /* hydrogen-bond candidates: beads k10 k11 k12 k20 k21 k22, geometry r th1 th2 psi psi1 psi2 */
+    int *hb_beads;     /* 6 per HB */
+    double *hb_geo;    /* 6 per HB */
+    double *hb_E;
+    int *hb_dode;      /* 1 = tertiary class (E3_HB), 0 = secondary (E2_HB) */
+    int *hb_site_off;  /* n_hb+1: ATOM_HB as CSR, sites per HB with multiplicity */
+    int *hb_site_idx;
+    int *site_hb_off;  /* n_hb_sites+1: HB_ATOM as CSR, HBs per site in insertion order */
+    int *site_hb_idx;
+    int *site_valence; /* VALENCE */
+} tis_system;
+
+/* Arguments of the reference command line (src/run_sample.sh, fast.cpp:6080-6103). */
+typedef struct tis_build_args {
+    double ss_D;   /* -D */
+    double hs_D;   /* -E */
+    double st_D;   /* -d */
+    double T;      /* -T, kcal/mol */
+    double c_mg;   /* -c, mol/L MgCl2 */
+    double c_k;    /* -f, mol/L KCl */
+    int seed;      /* -s */
+    double box;    /* -L (cubic side, A); 350 is the reference's hard-coded value */
+} tis_build_args;
+
+/*
+ * Builds the 16SCD system from the reference's input files in data_dir
+ * (16SCD.pdb, 16SCD_16SCD_unprocessed_bonds.dat, ..._stacks.dat,
+ * MAXIKEY_Mg_Cl_K.dat).  Replaces read_maxi_key, list_crowder_types,
+ * read_PDB, initialize_HB_atoms, initialize_unprocessed_bonds,
+ * initialize_unprocessed_stacks, set_box and add_crowder, called in the
+ * order of fast.cpp:6105-6177, including the glibc rand() stream seeded with
+ * srand((unsigned)(T*1000*seed)).  *rand_state_out (may be NULL) receives the
+ * number of rand() draws consumed, so a caller can continue the same stream.
+ */
+int tis_build_16scd(const char *data_dir, const tis_build_args *args,
+                    tis_system **out, long long *rand_draws_out);
+void tis_system_free(tis_system *sys);
+
+/* ------------------------------------------------------------ the engine */
+typedef struct tis_ctx tis_ctx;
+
+enum { TIS_PREC_FP64 = 0,   /* all force arithmetic in FP64 ("plain") */
+       TIS_PREC_MIXED = 1 };/* pair / multi-body forces in FP32, state and sums in FP64 */
+
+enum { TIS_RNG_PHILOX = 0,  /* counter-based Philox4x32-10 per particle and step */
+       TIS_RNG_STREAM = 1 };/* caller-fed uniform stream in the reference's draw order */
+
+typedef struct tis_options {
+    int precision;        /* TIS_PREC_* */
+    int rng;              /* TIS_RNG_* */
+    uint64_t seed;        /* Philox key */
+    double noise_scale;   /* multiplies SIGMA_FORCE; 1 = reference, 0 = noise-free */
+    double skin;          /* list skin dr1 (fast.cpp:6227), 3 A in the reference */
+    double r_sr;          /* short-range list radius, 12 + skin in the reference */
+    double r_lr;          /* long-range list radius, 50 + skin in the reference */
+    int lr_every;         /* long-range Coulomb refresh cadence, 10 in the reference */
+    int use_graph;        /* capture each tis_step chunk in a hipGraph */
+    int graph_steps;      /* steps per captured graph */
+} tis_options;
+
+/* Fills *opt with the reference's settings (FP64, stream RNG off => Philox, skin 3, ...). */
+void tis_default_options(tis_options *opt);
+
+/* Copies n_rep replica descriptions to the device. Replaces the set-up of
+ * fast.cpp:6219-6239 (MPI_Init, list buffers, populate_lists). */
+int tis_create(const tis_system *reps, int n_rep, const tis_options *opt, tis_ctx **out);
+void tis_destroy(tis_ctx *ctx);
+
+/* State in/out (xyz-interleaved, 3*N doubles each; any pointer may be NULL on get).
+ * set_state also rebuilds the pair lists (populate_lists, fast.cpp:1546-1737) and
+ * sets PROGRESS = 0 and the step counter klok. */
+int tis_set_state(tis_ctx *ctx, int rep, const double *pos, const double *pos_old,
+                  const double *vel, long long klok);
+int tis_get_state(tis_ctx *ctx, int rep, double *pos, double *pos_old, double *vel);
+
+/* Advances every replica n_steps Langevin steps: the body of fast.cpp:6274-6387
+ * without its output block (deterministic_forces_MPI, full_forces,
+ * move_rigid_units, check_shifts). */
+int tis_step(tis_ctx *ctx, long long n_steps);
+
+/* Energies of the last force evaluation of replica rep. */
+typedef struct tis_energies {
+    double e2_hb, e3_hb, e2_stack, e3_stack;   /* as in the Energies file, fast.cpp:6312 */
+    double e_wca, e_coul_sr, e_coul_lr;        /* pair terms (only after tis_eval_forces) */
+    double e_bond, e_angle;                    /* backbone (only after tis_eval_forces) */
+    double rg;                                 /* radius of gyration of the beads */
+    long long klok;                            /* step counter */
+    long long n_rebuilds;                      /* list rebuilds so far */
+    double progress;                           /* PROGRESS */
+    long long n_sr_pairs, n_lr_pairs, n_hb_active;
+} tis_energies;
+int tis_get_energies(tis_ctx *ctx, int rep, tis_energies *e);
+/* Per-stack energies st_energy[1..s3_N] (the Contacts file, fast.cpp:6290). */
+int tis_get_stack_energies(tis_ctx *ctx, int rep, double *out, int n);
+
+/* Parity hook: forces of the selected terms at the current positions, without
+ * moving anything.  HB uses the replica's uniform source (stream or Philox). */
+enum { TIS_TERM_STACK = 1, TIS_TERM_HB = 2, TIS_TERM_BACKBONE = 4,
+       TIS_TERM_SR = 8, TIS_TERM_LR = 16, TIS_TERM_ALL = 31 };
+int tis_eval_forces(tis_ctx *ctx, int rep, unsigned term_mask, double *f_out, tis_energies *e);
+
+/* Parity hook: current pair lists as (i,j) pairs with i<j, sorted; which 0 = SR, 1 = LR. */
+int tis_get_pairs(tis_ctx *ctx, int rep, int which, int *ij_out, long long cap, long long *n_out);
+/* Parity hook: hb_status (-1 / 0) of every candidate after the last list build. */
+int tis_get_hb_status(tis_ctx *ctx, int rep, int *status_out, int n);
+
+/* TIS_RNG_STREAM: raw glibc rand() values (0..RAND_MAX) consumed in the reference's
+ * order (HB valence resolution, then 1500 Box-Muller pairs per step).  Resets the
+ * replica's cursor to 0.  *consumed gives the number used so far. */
+int tis_set_uniform_stream(tis_ctx *ctx, int rep, const int32_t *raw, long long n);
+int tis_get_stream_cursor(tis_ctx *ctx, int rep, long long *consumed);
+
+/* Average device time per launch of the named kernel over the last tis_step call
+ * (HIP events on the context's stream); name NULL lists kernels in *buf. */
+int tis_kernel_times(tis_ctx *ctx, char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIS_GPU_H */
