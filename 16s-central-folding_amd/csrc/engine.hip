@@ -1,0 +1,823 @@
+// Host side of the MI355X engine: device buffers, launches and the C ABI of
+// include/tis_gpu.h.  One context drives a batch of replicas on one GPU; all
+// work goes to the context's streams.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tis_gpu.h"
+#include "kernels.h"
+
+namespace tis_host {
+void set_error(const std::string &msg);
+}
+
+using namespace tis;
+
+namespace {
+
+struct DeviceError {
+    std::string msg;
+};
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw DeviceError{std::string(#expr) + ": " + hipGetErrorString(e_)};               \
+    } while (0)
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+template <class T>
+T *dalloc(std::vector<void *> &owned, size_t n) {
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    HIP_TRY(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    owned.push_back(p);
+    return (T *)p;
+}
+
+template <class T>
+void upload(T *dst, const std::vector<T> &src) {
+    if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+struct tis_ctx {
+    int n_rep = 0;
+    tis_options opt{};
+    hipStream_t s_main = nullptr, s_hb = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<void *> owned;
+    Dev D{};
+    std::vector<RepParams> rp;
+    std::vector<int> n_st3;
+    int nb_bonded = 0;
+    int n_bonded_terms = 0;
+    int dev_id = 0;
+    std::vector<long long> stream_cap;
+    long long stream_total = 0;
+    bool graph_ok = false;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    // kernel timing: when profiling is on, every launch of tis_step is bracketed
+    // by HIP events on the stream it runs on (no graph), and the durations are
+    // accumulated per kernel name
+    bool profiling = false;
+    struct Timed { const char *name; hipStream_t s; hipEvent_t a, b; };
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> event_pool;
+    size_t event_next = 0;
+    std::vector<std::string> knames;
+    std::vector<double> ksum_ms;
+    std::vector<long long> kcount;
+
+    ~tis_ctx() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+        for (void *p : owned) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (s_main) (void)hipStreamDestroy(s_main);
+        if (s_hb) (void)hipStreamDestroy(s_hb);
+    }
+
+    int grid(int n) const { return (n + kBlock - 1) / kBlock; }
+
+    hipEvent_t next_event() {
+        if (event_next == event_pool.size()) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            event_pool.push_back(e);
+        }
+        return event_pool[event_next++];
+    }
+    // Brackets one launch with events when profiling.
+    template <class L>
+    void timed(const char *name, hipStream_t s, L &&launch) {
+        if (!profiling) { launch(); return; }
+        hipEvent_t a = next_event(), b = next_event();
+        HIP_TRY(hipEventRecord(a, s));
+        launch();
+        HIP_TRY(hipEventRecord(b, s));
+        pending.push_back({name, s, a, b});
+    }
+    void collect_times() {
+        for (const Timed &t : pending) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+            size_t k = 0;
+            while (k < knames.size() && knames[k] != t.name) ++k;
+            if (k == knames.size()) { knames.push_back(t.name); ksum_ms.push_back(0); kcount.push_back(0); }
+            ksum_ms[k] += ms;
+            kcount[k] += 1;
+        }
+        pending.clear();
+        event_next = 0;
+    }
+
+    void launch_rebuild() {
+        timed("k_bin", s_main, [&] { k_bin<<<n_rep, 1024, 0, s_main>>>(D); });
+        timed("k_list", s_main, [&] { k_list<<<grid(D.n_tot), kBlock, 0, s_main>>>(D, 1); });
+        timed("k_hbrebuild", s_main, [&] { k_hbrebuild<<<n_rep, 1024, max_site_lds(), s_main>>>(D); });
+    }
+    size_t max_site_lds() const {
+        int m = 1;
+        for (const RepParams &p : rp) m = std::max(m, p.nsite);
+        return (size_t)m * sizeof(int);
+    }
+
+    // One Langevin step.  HB resolution runs on its own stream, concurrently
+    // with stacking, backbone and the pair gather; the integrator joins both.
+    void launch_step() {
+        HIP_TRY(hipEventRecord(ev_fork, s_main));
+        HIP_TRY(hipStreamWaitEvent(s_hb, ev_fork, 0));
+        timed("k_hb", s_hb, [&] {
+            if (D.precision == TIS_PREC_FP64) k_hb<double><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
+            else k_hb<float><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
+        });
+        HIP_TRY(hipEventRecord(ev_join, s_hb));
+        const int nblk = n_rep + nb_bonded + grid(D.n_tot);
+        timed("k_force", s_main, [&] {
+            if (D.precision == TIS_PREC_FP64) k_force<double><<<nblk, kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+            else k_force<float><<<nblk, kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+        });
+        HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
+        timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
+        timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
+        launch_rebuild();
+    }
+
+    RepState get_rs(int r) {
+        RepState s;
+        HIP_TRY(hipMemcpy(&s, D.rs + r, sizeof s, hipMemcpyDeviceToHost));
+        return s;
+    }
+    void put_rs(int r, const RepState &s) {
+        HIP_TRY(hipMemcpy(D.rs + r, &s, sizeof s, hipMemcpyHostToDevice));
+    }
+    void check_flags() {
+        std::vector<RepState> s(n_rep);
+        HIP_TRY(hipMemcpy(s.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
+        for (int r = 0; r < n_rep; ++r) {
+            if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour pool overflow in replica " + std::to_string(r)};
+            if (s[r].overflow & 2) throw Fail{TIS_EOVERFLOW, "HB working set above the LDS caps in replica " + std::to_string(r)};
+            if (s[r].overflow & 4) throw Fail{TIS_EOVERFLOW, "uniform stream exhausted in replica " + std::to_string(r)};
+            if (s[r].overflow & 16) throw Fail{TIS_EOVERFLOW, "HB site with more than 64 candidates in replica " + std::to_string(r)};
+        }
+    }
+};
+
+namespace {
+
+constexpr double kSigLJh = 1.5852;
+
+// Validates one replica description; throws Fail on the first problem found.
+void validate(const tis_system &s, int r) {
+    auto bad = [&](const std::string &m) { throw Fail{TIS_EINVAL, "replica " + std::to_string(r) + ": " + m}; };
+    if (s.n_particles <= 0 || s.n_beads <= 0 || s.n_beads > s.n_particles) bad("particle counts");
+    if (!s.type || !s.cls || !s.pos || !s.pos_old || !s.vel) bad("missing particle arrays");
+    if (s.recentre < 0 || s.recentre >= s.n_beads) bad("recentring bead out of range");
+    for (int c = 0; c < 3; ++c)
+        if (!(s.box[c] > 0)) bad("box");
+    if (!(s.T > 0) || !(s.h > 0)) bad("T and h must be positive");
+    for (int i = 0; i < s.n_particles; ++i) {
+        if (s.type[i] < 0 || s.type[i] >= TIS_NTYPES) bad("type out of range");
+        if (s.cls[i] < 0 || s.cls[i] > 5) bad("class out of range");
+    }
+    auto chk = [&](const int *a, long n, int hi, const char *what) {
+        if (n > 0 && !a) bad(std::string("missing ") + what);
+        for (long q = 0; q < n; ++q)
+            if (a[q] < 0 || a[q] >= hi) bad(std::string(what) + " index out of range");
+    };
+    chk(s.bond_ij, 2L * s.n_bonds, s.n_beads, "bond");
+    chk(s.angle_ijk, 3L * s.n_angles, s.n_beads, "angle");
+    chk(s.st3_beads, 6L * s.n_st3, s.n_beads, "tertiary stack");
+    chk(s.st2_beads, 7L * s.n_st2, s.n_beads, "secondary stack");
+    chk(s.hb_beads, 6L * s.n_hb, s.n_beads, "HB");
+    if (s.n_hb > 0) {
+        if (!s.hb_site_off || !s.site_hb_off || !s.site_valence || !s.hb_geo || !s.hb_E || !s.hb_dode) bad("HB tables");
+        if (s.hb_site_off[0] != 0 || s.hb_site_off[s.n_hb] != s.n_hb_incid) bad("HB incidence offsets");
+        if (s.site_hb_off[0] != 0 || s.site_hb_off[s.n_hb_sites] != s.n_hb_incid) bad("site offsets");
+        chk(s.hb_site_idx, s.n_hb_incid, s.n_hb_sites, "HB site");
+        chk(s.site_hb_idx, s.n_hb_incid, s.n_hb, "site HB");
+    }
+    if (s.n_st2 > 4096) bad("more than 4096 secondary stacks");
+    for (int k = 0; k < 2 * s.n_st3; ++k)
+        if (s.st3_disable && (s.st3_disable[k] < -1 || s.st3_disable[k] >= s.n_st2)) bad("st3_disable");
+}
+
+}  // namespace
+
+extern "C" void tis_default_options(tis_options *o) {
+    if (!o) return;
+    o->precision = TIS_PREC_FP64;
+    o->rng = TIS_RNG_PHILOX;
+    o->seed = 1;
+    o->noise_scale = 1.0;
+    o->skin = 3.0;
+    o->r_sr = 15.0;
+    o->r_lr = 53.0;
+    o->lr_every = 10;
+    o->use_graph = 1;
+    o->graph_steps = 50;
+}
+
+static int fail(const Fail &f) {
+    tis_host::set_error(f.msg);
+    return f.code;
+}
+static int fail_dev(const DeviceError &e) {
+    tis_host::set_error(e.msg);
+    return TIS_EDEVICE;
+}
+
+// Runs an API body, mapping exceptions to the C error codes.
+template <class F>
+static int guarded(F &&body) {
+    try {
+        return body();
+    } catch (const Fail &f) {
+        return fail(f);
+    } catch (const DeviceError &e) {
+        return fail_dev(e);
+    } catch (const std::bad_alloc &) {
+        tis_host::set_error("out of host memory");
+        return TIS_ENOMEM;
+    }
+}
+
+extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *opt, tis_ctx **out) {
+    if (!reps || n_rep <= 0 || !out) return fail(Fail{TIS_EINVAL, "tis_create: bad arguments"});
+    *out = nullptr;
+    tis_ctx *c = new (std::nothrow) tis_ctx();
+    if (!c) return fail(Fail{TIS_ENOMEM, "tis_create: out of memory"});
+    try {
+        c->n_rep = n_rep;
+        if (opt) c->opt = *opt;
+        else tis_default_options(&c->opt);
+        const tis_options &o = c->opt;
+        if (o.precision != TIS_PREC_FP64 && o.precision != TIS_PREC_MIXED) throw Fail{TIS_EINVAL, "precision"};
+        if (o.rng != TIS_RNG_PHILOX && o.rng != TIS_RNG_STREAM) throw Fail{TIS_EINVAL, "rng"};
+        if (!(o.r_sr > 0) || !(o.r_lr >= o.r_sr) || !(o.skin >= 0) || o.lr_every <= 0) throw Fail{TIS_EINVAL, "list settings"};
+        for (int r = 0; r < n_rep; ++r) validate(reps[r], r);
+        int dev_count = 0;
+        HIP_TRY(hipGetDeviceCount(&dev_count));
+        if (dev_count <= 0) throw Fail{TIS_EDEVICE, "no HIP device"};
+        HIP_TRY(hipGetDevice(&c->dev_id));
+        HIP_TRY(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&c->s_hb, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hb<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hb<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+        // ---- offsets and per-replica parameters
+        int p0 = 0, b0 = 0, a0 = 0, s30 = 0, s20 = 0, h0 = 0, si0 = 0, i0 = 0, cell0 = 0;
+        c->rp.resize(n_rep);
+        for (int r = 0; r < n_rep; ++r) {
+            const tis_system &s = reps[r];
+            RepParams &P = c->rp[r];
+            std::memset(&P, 0, sizeof P);
+            P.p0 = p0; P.n = s.n_particles; P.nbeads = s.n_beads; P.kh = p0 + s.recentre;
+            for (int a = 0; a < 3; ++a) {
+                P.L[a] = s.box[a];
+                P.hL[a] = 0.5 * s.box[a];
+                P.M[a] = std::max(1, (int)(s.box[a] / o.r_lr));
+                P.lc[a] = s.box[a] / P.M[a];
+            }
+            if ((long)P.M[0] * P.M[1] * P.M[2] > kMaxCells) throw Fail{TIS_EINVAL, "box too large for the cell grid"};
+            P.T = s.T; P.lB = s.lB; P.h = s.h; P.vcap = 300.0 * s.T;
+            P.rsr2 = o.r_sr * o.r_sr; P.rlr2 = o.r_lr * o.r_lr; P.skin = o.skin;
+            P.lr_every = o.lr_every;
+            P.cell0 = cell0;
+            P.bond0 = b0; P.nbond = s.n_bonds; P.ang0 = a0; P.nang = s.n_angles;
+            P.st30 = s30; P.nst3 = s.n_st3; P.st20 = s20; P.nst2 = s.n_st2;
+            P.hb0 = h0; P.nhb = s.n_hb; P.site0 = si0; P.nsite = s.n_hb_sites; P.inc0 = i0; P.ninc = s.n_hb_incid;
+            // read_maxi_key derived constants (fast.cpp:2895-2944)
+            for (int t = 0; t < kNT; ++t) {
+                const double visc = 0.555 * s.type_R[t];
+                P.mass[t] = s.type_mass[t];
+                P.K1[t] = std::exp(-visc / s.type_mass[t] * s.h);
+                P.K2[t] = (1.0 - P.K1[t]) / visc;
+                P.sigma[t] = std::sqrt(2.0 * visc * s.T / s.h);
+            }
+            for (int a = 0; a < kNT; ++a)
+                for (int b = 0; b < kNT; ++b) {
+                    const int lo = std::min(a, b), hi = std::max(a, b), ab = a * kNT + b;
+                    const double d = (lo <= 5 && hi <= 5) ? 3.2 : s.type_R[lo] + s.type_R[hi];
+                    P.dlj[ab] = d;
+                    P.d2lj[ab] = d * d;
+                    const double dm = d - 0.4406141861 * kSigLJh;
+                    P.d2min[ab] = dm * dm;
+                    P.elj[ab] = std::sqrt(s.type_eps[lo] * s.type_eps[hi]);
+                    P.flj[ab] = 12.0 * P.elj[ab] / (kSigLJh * kSigLJh);
+                    P.qlb[ab] = s.type_q[lo] * s.type_q[hi] * s.lB;
+                }
+            p0 += s.n_particles; b0 += s.n_bonds; a0 += s.n_angles; s30 += s.n_st3; s20 += s.n_st2;
+            h0 += s.n_hb; si0 += s.n_hb_sites; i0 += s.n_hb_incid;
+            cell0 += P.M[0] * P.M[1] * P.M[2] + 1;
+        }
+        const int N = p0;
+        Dev &D = c->D;
+        D.n_rep = n_rep; D.n_tot = N; D.precision = o.precision; D.rng = o.rng; D.seed = o.seed;
+        D.noise_scale = o.noise_scale;
+        auto &own = c->owned;
+        D.rp = dalloc<RepParams>(own, n_rep);
+        D.rs = dalloc<RepState>(own, n_rep);
+        D.pos = dalloc<double4>(own, 2 * (size_t)N);
+        D.old = dalloc<double4>(own, 2 * (size_t)N);
+        D.vel = dalloc<double4>(own, 2 * (size_t)N);
+        D.f_sr = dalloc<double4>(own, N);
+        D.f_lr = dalloc<double4>(own, N);
+        D.f_mb = dalloc<double4>(own, N);
+        D.ptype = dalloc<int>(own, N);
+        D.prep = dalloc<int>(own, N);
+        D.cell_of = dalloc<int>(own, N);
+        D.cell_rank = dalloc<int>(own, N);
+        D.cell_start = dalloc<int>(own, cell0);
+        D.sorted = dalloc<int>(own, N);
+        D.nb_start = dalloc<int>(own, N);
+        D.nb_nsr = dalloc<int>(own, N);
+        D.nb_nlr = dalloc<int>(own, N);
+        // ---- topology, converted to global indices
+        std::vector<int> ptype(N), prep(N);
+        std::vector<double4> pos(N), old(N), vel(N);
+        std::vector<int2> bij;
+        std::vector<double2> bkr, akt, st2rE;
+        std::vector<int4> aijk;
+        std::vector<int> st3b, st2b, hbb, hbd, hio, hi, so, sh, sv;
+        std::vector<double> st3g, st3E, hbg, hbE;
+        std::vector<int2> st3d;
+        hio.push_back(0);
+        so.push_back(0);
+        int inc_base = 0;
+        for (int r = 0; r < n_rep; ++r) {
+            const tis_system &s = reps[r];
+            const RepParams &P = c->rp[r];
+            const int o0 = P.p0;
+            for (int i = 0; i < s.n_particles; ++i) {
+                const int charged = (s.cls[i] != 1 && s.cls[i] != 2) ? 1 : 0;
+                ptype[o0 + i] = s.type[i] | (s.cls[i] << 4) | (charged << 8);
+                prep[o0 + i] = r;
+                pos[o0 + i] = make_double4(s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2], 0);
+                old[o0 + i] = make_double4(s.pos_old[3 * i], s.pos_old[3 * i + 1], s.pos_old[3 * i + 2], 0);
+                vel[o0 + i] = make_double4(s.vel[3 * i], s.vel[3 * i + 1], s.vel[3 * i + 2], 0);
+            }
+            for (int k = 0; k < s.n_bonds; ++k) {
+                bij.push_back(make_int2(o0 + s.bond_ij[2 * k], o0 + s.bond_ij[2 * k + 1]));
+                bkr.push_back(make_double2(s.bond_kr[2 * k], s.bond_kr[2 * k + 1]));
+            }
+            for (int k = 0; k < s.n_angles; ++k) {
+                aijk.push_back(make_int4(o0 + s.angle_ijk[3 * k], o0 + s.angle_ijk[3 * k + 1],
+                                         o0 + s.angle_ijk[3 * k + 2], 0));
+                akt.push_back(make_double2(s.angle_kt[2 * k], s.angle_kt[2 * k + 1]));
+            }
+            for (int k = 0; k < s.n_st3; ++k) {
+                for (int q = 0; q < 6; ++q) {
+                    st3b.push_back(o0 + s.st3_beads[6 * k + q]);
+                    st3g.push_back(s.st3_geo[6 * k + q]);
+                }
+                st3E.push_back(s.st3_E[k]);
+                int d0 = s.st3_disable ? s.st3_disable[2 * k] : -1, d1 = s.st3_disable ? s.st3_disable[2 * k + 1] : -1;
+                st3d.push_back(make_int2(d0, d1));
+            }
+            for (int k = 0; k < s.n_st2; ++k) {
+                for (int q = 0; q < 7; ++q) st2b.push_back(o0 + s.st2_beads[7 * k + q]);
+                st2b.push_back(0);
+                st2rE.push_back(make_double2(s.st2_r0[k], s.st2_E[k]));
+            }
+            for (int k = 0; k < s.n_hb; ++k) {
+                for (int q = 0; q < 6; ++q) {
+                    hbb.push_back(o0 + s.hb_beads[6 * k + q]);
+                    hbg.push_back(s.hb_geo[6 * k + q]);
+                }
+                hbE.push_back(s.hb_E[k]);
+                hbd.push_back(s.hb_dode[k]);
+                for (int q = s.hb_site_off[k]; q < s.hb_site_off[k + 1]; ++q) hi.push_back(P.site0 + s.hb_site_idx[q]);
+                hio.push_back((int)hi.size());
+            }
+            for (int q = 0; q < s.n_hb_sites; ++q) {
+                for (int e = s.site_hb_off[q]; e < s.site_hb_off[q + 1]; ++e) sh.push_back(P.hb0 + s.site_hb_idx[e]);
+                so.push_back((int)sh.size());
+                sv.push_back(s.site_valence[q]);
+            }
+            inc_base += s.n_hb_incid;
+        }
+        c->n_bonded_terms = (int)bij.size() + (int)aijk.size();
+        c->nb_bonded = c->grid(c->n_bonded_terms);
+        D.bond_ij = dalloc<int2>(own, bij.size()); upload(D.bond_ij, bij);
+        D.bond_kr = dalloc<double2>(own, bkr.size()); upload(D.bond_kr, bkr);
+        D.ang_ijk = dalloc<int4>(own, aijk.size()); upload(D.ang_ijk, aijk);
+        D.ang_kt = dalloc<double2>(own, akt.size()); upload(D.ang_kt, akt);
+        D.st3_b = dalloc<int>(own, st3b.size()); upload(D.st3_b, st3b);
+        D.st3_g = dalloc<double>(own, st3g.size()); upload(D.st3_g, st3g);
+        D.st3_E = dalloc<double>(own, st3E.size()); upload(D.st3_E, st3E);
+        D.st3_dis = dalloc<int2>(own, st3d.size()); upload(D.st3_dis, st3d);
+        D.st2_b = dalloc<int>(own, st2b.size()); upload(D.st2_b, st2b);
+        D.st2_rE = dalloc<double2>(own, st2rE.size()); upload(D.st2_rE, st2rE);
+        D.hb_b = dalloc<int>(own, hbb.size()); upload(D.hb_b, hbb);
+        D.hb_g = dalloc<double>(own, hbg.size()); upload(D.hb_g, hbg);
+        D.hb_E = dalloc<double>(own, hbE.size()); upload(D.hb_E, hbE);
+        D.hb_dode = dalloc<int>(own, hbd.size()); upload(D.hb_dode, hbd);
+        D.hb_inc_off = dalloc<int>(own, hio.size()); upload(D.hb_inc_off, hio);
+        D.hb_inc = dalloc<int>(own, hi.size()); upload(D.hb_inc, hi);
+        D.site_off = dalloc<int>(own, so.size()); upload(D.site_off, so);
+        D.site_hb = dalloc<int>(own, sh.size()); upload(D.site_hb, sh);
+        D.site_val = dalloc<int>(own, sv.size()); upload(D.site_val, sv);
+        const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
+        D.hb_status = dalloc<signed char>(own, nhb_tot);
+        D.act_hb = dalloc<int>(own, nhb_tot);
+        D.act_idx = dalloc<int>(own, nhb_tot);
+        D.asite_id = dalloc<int>(own, nsite_tot);
+        D.asite_off = dalloc<int>(own, nsite_tot + n_rep);
+        D.asite_list = dalloc<int>(own, inc_base);
+        D.ahb_inc_off = dalloc<int>(own, nhb_tot + n_rep);
+        D.ahb_inc = dalloc<int>(own, inc_base);
+        D.st_energy = dalloc<double>(own, st3E.size());
+        D.maxwell = dalloc<double>(own, 3000 * (size_t)n_rep);
+        D.stream_base = dalloc<long long>(own, n_rep);
+        D.stream_len = dalloc<long long>(own, n_rep);
+        D.done_counter = dalloc<unsigned int>(own, 1);
+        c->stream_cap.assign(n_rep, 0);
+        upload(D.ptype, ptype);
+        upload(D.prep, prep);
+        upload(D.pos, pos);
+        upload(D.old, old);
+        upload(D.vel, vel);
+        // ---- size the neighbour pool from a counting pass, then build
+        std::vector<RepState> rs(n_rep);
+        for (int r = 0; r < n_rep; ++r) {
+            std::memset(&rs[r], 0, sizeof(RepState));
+            rs[r].klok = 1;
+            rs[r].need_rebuild = 1;
+            rs[r].lr_due = 1;
+        }
+        upload(D.rs, rs);
+        HIP_TRY(hipMemcpy(D.rp, c->rp.data(), n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
+        k_bin<<<n_rep, 1024, 0, c->s_main>>>(D);
+        k_list<<<c->grid(N), kBlock, 0, c->s_main>>>(D, 0);
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        long long pool_total = 0;
+        for (int r = 0; r < n_rep; ++r) {
+            RepState s = c->get_rs(r);
+            RepParams &P = c->rp[r];
+            P.pool0 = pool_total;
+            P.pool_cap = 3 * s.pool_cursor + 64LL * P.n;
+            pool_total += P.pool_cap;
+        }
+        D.pool = dalloc<int>(own, (size_t)pool_total);
+        HIP_TRY(hipMemcpy(D.rp, c->rp.data(), n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
+        c->launch_rebuild();
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        for (int r = 0; r < n_rep; ++r) {
+            RepState s = c->get_rs(r);
+            s.need_rebuild = 0;
+            c->put_rs(r, s);
+        }
+        c->check_flags();
+        HIP_TRY(hipGetLastError());
+        *out = c;
+        return TIS_OK;
+    } catch (const Fail &f) {
+        delete c;
+        return fail(f);
+    } catch (const DeviceError &e) {
+        delete c;
+        return fail_dev(e);
+    } catch (const std::bad_alloc &) {
+        delete c;
+        tis_host::set_error("out of host memory");
+        return TIS_ENOMEM;
+    }
+}
+
+extern "C" void tis_destroy(tis_ctx *c) { delete c; }
+
+extern "C" int tis_set_state(tis_ctx *c, int rep, const double *pos, const double *pos_old, const double *vel,
+                             long long klok) {
+    if (!c || rep < 0 || rep >= (c ? c->n_rep : 0) || !pos || !pos_old || !vel)
+        return fail(Fail{TIS_EINVAL, "tis_set_state: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        std::vector<double4> a(P.n), b(P.n), v(P.n);
+        for (int i = 0; i < P.n; ++i) {
+            a[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0);
+            b[i] = make_double4(pos_old[3 * i], pos_old[3 * i + 1], pos_old[3 * i + 2], 0);
+            v[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0);
+        }
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        HIP_TRY(hipMemcpy(c->D.pos + P.p0, a.data(), P.n * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->D.old + P.p0, b.data(), P.n * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->D.vel + P.p0, v.data(), P.n * sizeof(double4), hipMemcpyHostToDevice));
+        std::vector<RepState> all(c->n_rep);
+        HIP_TRY(hipMemcpy(all.data(), c->D.rs, c->n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
+        for (int r = 0; r < c->n_rep; ++r) all[r].need_rebuild = (r == rep);
+        all[rep].klok = klok;
+        all[rep].progress = 0;
+        all[rep].lr_due = 1;
+        all[rep].maxd2 = 0;
+        all[rep].overflow = 0;
+        HIP_TRY(hipMemcpy(c->D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+        c->launch_rebuild();
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        HIP_TRY(hipMemcpy(all.data(), c->D.rs, c->n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
+        all[rep].need_rebuild = 0;
+        HIP_TRY(hipMemcpy(c->D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+        c->check_flags();
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_state(tis_ctx *c, int rep, double *pos, double *pos_old, double *vel) {
+    if (!c || rep < 0 || rep >= c->n_rep) return fail(Fail{TIS_EINVAL, "tis_get_state: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        std::vector<double4> t(P.n);
+        double4 *src[3] = {c->D.pos, c->D.old, c->D.vel};
+        double *dst[3] = {pos, pos_old, vel};
+        for (int a = 0; a < 3; ++a) {
+            if (!dst[a]) continue;
+            HIP_TRY(hipMemcpy(t.data(), src[a] + P.p0, P.n * sizeof(double4), hipMemcpyDeviceToHost));
+            for (int i = 0; i < P.n; ++i) {
+                dst[a][3 * i] = t[i].x;
+                dst[a][3 * i + 1] = t[i].y;
+                dst[a][3 * i + 2] = t[i].z;
+            }
+        }
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_step(tis_ctx *c, long long n_steps) {
+    if (!c || n_steps < 0) return fail(Fail{TIS_EINVAL, "tis_step: bad arguments"});
+    return guarded([&]() -> int {
+        long long done = 0;
+        if (c->profiling) {
+            // bounded batches so that the event pool stays small
+            while (done < n_steps) {
+                const long long m = std::min<long long>(64, n_steps - done);
+                for (long long q = 0; q < m; ++q) c->launch_step();
+                HIP_TRY(hipStreamSynchronize(c->s_main));
+                HIP_TRY(hipStreamSynchronize(c->s_hb));
+                c->collect_times();
+                done += m;
+            }
+        }
+        if (c->opt.use_graph && c->opt.graph_steps > 0 && n_steps - done >= c->opt.graph_steps) {
+            if (!c->graph) {
+                hipGraph_t g;
+                HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
+                for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
+                HIP_TRY(hipStreamEndCapture(c->s_main, &g));
+                HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+                HIP_TRY(hipGraphDestroy(g));
+                c->graph_steps = c->opt.graph_steps;
+            }
+            while (n_steps - done >= c->graph_steps) {
+                HIP_TRY(hipGraphLaunch(c->graph, c->s_main));
+                done += c->graph_steps;
+            }
+        }
+        for (; done < n_steps; ++done) c->launch_step();
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->check_flags();
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_energies(tis_ctx *c, int rep, tis_energies *e) {
+    if (!c || rep < 0 || rep >= c->n_rep || !e) return fail(Fail{TIS_EINVAL, "tis_get_energies: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        RepState s = c->get_rs(rep);
+        const RepParams &P = c->rp[rep];
+        std::memset(e, 0, sizeof *e);
+        e->e2_hb = s.e[EI_E2_HB]; e->e3_hb = s.e[EI_E3_HB];
+        e->e2_stack = s.e[EI_E2_ST]; e->e3_stack = s.e[EI_E3_ST];
+        e->e_wca = s.e[EI_WCA]; e->e_coul_sr = s.e[EI_COUL_SR]; e->e_coul_lr = s.e[EI_COUL_LR];
+        e->e_bond = s.e[EI_BOND]; e->e_angle = s.e[EI_ANGLE];
+        e->klok = s.klok; e->n_rebuilds = s.n_rebuilds; e->progress = s.progress;
+        e->n_hb_active = s.n_act;
+        // Rg of the beads (radius_of_gyration, fast.cpp:2844-2876)
+        std::vector<double4> x(P.nbeads);
+        HIP_TRY(hipMemcpy(x.data(), c->D.pos + P.p0, P.nbeads * sizeof(double4), hipMemcpyDeviceToHost));
+        double cm[3] = {0, 0, 0};
+        for (const double4 &q : x) { cm[0] += q.x; cm[1] += q.y; cm[2] += q.z; }
+        for (double &v : cm) v /= P.nbeads;
+        double acc = 0;
+        for (const double4 &q : x) {
+            const double a = q.x - cm[0], b = q.y - cm[1], d = q.z - cm[2];
+            acc += a * a + b * b + d * d;
+        }
+        e->rg = std::sqrt(acc / P.nbeads);
+        // pair counts from the list lengths
+        std::vector<int> nsr(P.n), nlr(P.n);
+        HIP_TRY(hipMemcpy(nsr.data(), c->D.nb_nsr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(nlr.data(), c->D.nb_nlr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
+        long long a = 0, b = 0;
+        for (int i = 0; i < P.n; ++i) { a += nsr[i]; b += nlr[i]; }
+        e->n_sr_pairs = a / 2;
+        e->n_lr_pairs = b / 2;
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_stack_energies(tis_ctx *c, int rep, double *out, int n) {
+    if (!c || rep < 0 || rep >= c->n_rep || !out) return fail(Fail{TIS_EINVAL, "tis_get_stack_energies: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        if (n < P.nst3) throw Fail{TIS_EINVAL, "tis_get_stack_energies: buffer too small"};
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        if (P.nst3) HIP_TRY(hipMemcpy(out, c->D.st_energy + P.st30, P.nst3 * sizeof(double), hipMemcpyDeviceToHost));
+        return TIS_OK;
+    });
+}
+
+// Forces of the selected terms at the current positions of replica rep.  The
+// other replicas are left untouched (their terms run but write nothing that
+// the step keeps: multi-body forces are cleared afterwards, and the SR/LR
+// buffers are recomputed by the next step).
+extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out, tis_energies *e) {
+    if (!c || rep < 0 || rep >= c->n_rep) return fail(Fail{TIS_EINVAL, "tis_eval_forces: bad arguments"});
+    return guarded([&]() -> int {
+        Dev &D = c->D;
+        const RepParams &P = c->rp[rep];
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        // clear energies and forces
+        std::vector<RepState> all(c->n_rep);
+        HIP_TRY(hipMemcpy(all.data(), D.rs, c->n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
+        for (auto &s : all) std::memset(s.e, 0, sizeof s.e);
+        HIP_TRY(hipMemcpy(D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(D.f_mb, 0, D.n_tot * sizeof(double4), c->s_main));
+        if (mask & TIS_TERM_HB) {
+            if (D.precision == TIS_PREC_FP64) k_hb<double><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
+            else k_hb<float><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
+        }
+        const int nblk = c->n_rep + c->nb_bonded + c->grid(D.n_tot);
+        const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
+        if (D.precision == TIS_PREC_FP64) k_force<double><<<nblk, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+        else k_force<float><<<nblk, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        std::vector<double4> a(P.n), b(P.n), d(P.n);
+        HIP_TRY(hipMemcpy(a.data(), D.f_sr + P.p0, P.n * sizeof(double4), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b.data(), D.f_lr + P.p0, P.n * sizeof(double4), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(d.data(), D.f_mb + P.p0, P.n * sizeof(double4), hipMemcpyDeviceToHost));
+        const double ws = (mask & TIS_TERM_SR) ? 1.0 : 0.0, wl = (mask & TIS_TERM_LR) ? 1.0 : 0.0;
+        if (f_out)
+            for (int i = 0; i < P.n; ++i) {
+                f_out[3 * i] = ws * a[i].x + wl * b[i].x + d[i].x;
+                f_out[3 * i + 1] = ws * a[i].y + wl * b[i].y + d[i].y;
+                f_out[3 * i + 2] = ws * a[i].z + wl * b[i].z + d[i].z;
+            }
+        HIP_TRY(hipMemsetAsync(D.f_mb, 0, D.n_tot * sizeof(double4), c->s_main));
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->check_flags();
+        if (e) {
+            int rc = tis_get_energies(c, rep, e);
+            if (rc) return rc;
+        }
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_pairs(tis_ctx *c, int rep, int which, int *ij_out, long long cap, long long *n_out) {
+    if (!c || rep < 0 || rep >= c->n_rep || (which != 0 && which != 1) || !n_out)
+        return fail(Fail{TIS_EINVAL, "tis_get_pairs: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        std::vector<int> st(P.n), ns(P.n), nl(P.n);
+        HIP_TRY(hipMemcpy(st.data(), c->D.nb_start + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ns.data(), c->D.nb_nsr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(nl.data(), c->D.nb_nlr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
+        RepState s = c->get_rs(rep);
+        std::vector<int> pool((size_t)s.pool_cursor);
+        if (s.pool_cursor)
+            HIP_TRY(hipMemcpy(pool.data(), c->D.pool + P.pool0, pool.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<std::pair<int, int>> pr;
+        for (int i = 0; i < P.n; ++i) {
+            const int b = st[i] + (which ? ns[i] : 0), n = which ? nl[i] : ns[i];
+            for (int q = 0; q < n; ++q) {
+                const int j = pool[b + q] - P.p0;
+                if (i < j) pr.push_back({i, j});
+            }
+        }
+        std::sort(pr.begin(), pr.end());
+        *n_out = (long long)pr.size();
+        if (ij_out)
+            for (long long q = 0; q < std::min<long long>(cap, (long long)pr.size()); ++q) {
+                ij_out[2 * q] = pr[q].first;
+                ij_out[2 * q + 1] = pr[q].second;
+            }
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_hb_status(tis_ctx *c, int rep, int *out, int n) {
+    if (!c || rep < 0 || rep >= c->n_rep || !out) return fail(Fail{TIS_EINVAL, "tis_get_hb_status: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        if (n < P.nhb) throw Fail{TIS_EINVAL, "tis_get_hb_status: buffer too small"};
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        std::vector<signed char> s(P.nhb);
+        if (P.nhb) HIP_TRY(hipMemcpy(s.data(), c->D.hb_status + P.hb0, P.nhb, hipMemcpyDeviceToHost));
+        for (int k = 0; k < P.nhb; ++k) out[k] = s[k];
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_set_uniform_stream(tis_ctx *c, int rep, const int32_t *raw, long long n) {
+    if (!c || rep < 0 || rep >= c->n_rep || n < 0 || (n && !raw))
+        return fail(Fail{TIS_EINVAL, "tis_set_uniform_stream: bad arguments"});
+    return guarded([&]() -> int {
+        if (c->opt.rng != TIS_RNG_STREAM) throw Fail{TIS_ESTATE, "context was created without TIS_RNG_STREAM"};
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        // regions are re-laid out: replica rep gets a fresh allocation of n values
+        if (n > c->stream_cap[rep]) {
+            std::vector<long long> base(c->n_rep), len(c->n_rep);
+            HIP_TRY(hipMemcpy(base.data(), c->D.stream_base, c->n_rep * sizeof(long long), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(len.data(), c->D.stream_len, c->n_rep * sizeof(long long), hipMemcpyDeviceToHost));
+            long long total = c->stream_total + n;
+            int32_t *ns = nullptr;
+            HIP_TRY(hipMalloc(&ns, std::max<long long>(total, 1) * sizeof(int32_t)));
+            if (c->D.stream && c->stream_total)
+                HIP_TRY(hipMemcpy(ns, c->D.stream, c->stream_total * sizeof(int32_t), hipMemcpyDeviceToDevice));
+            if (c->D.stream) {
+                auto it = std::find(c->owned.begin(), c->owned.end(), (void *)c->D.stream);
+                if (it != c->owned.end()) c->owned.erase(it);
+                HIP_TRY(hipFree(c->D.stream));
+            }
+            c->owned.push_back(ns);
+            c->D.stream = ns;
+            base[rep] = c->stream_total;
+            c->stream_total = total;
+            c->stream_cap[rep] = n;
+            HIP_TRY(hipMemcpy(c->D.stream_base, base.data(), c->n_rep * sizeof(long long), hipMemcpyHostToDevice));
+            if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+        }
+        long long base_r;
+        HIP_TRY(hipMemcpy(&base_r, c->D.stream_base + rep, sizeof base_r, hipMemcpyDeviceToHost));
+        if (n) HIP_TRY(hipMemcpy(c->D.stream + base_r, raw, n * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->D.stream_len + rep, &n, sizeof n, hipMemcpyHostToDevice));
+        RepState s = c->get_rs(rep);
+        s.stream_cursor = 0;
+        s.overflow &= ~4;
+        c->put_rs(rep, s);
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_stream_cursor(tis_ctx *c, int rep, long long *consumed) {
+    if (!c || rep < 0 || rep >= c->n_rep || !consumed) return fail(Fail{TIS_EINVAL, "tis_get_stream_cursor: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        *consumed = c->get_rs(rep).stream_cursor;
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_kernel_times(tis_ctx *c, char *buf, int buflen) {
+    if (!c || !buf || buflen <= 0) return fail(Fail{TIS_EINVAL, "tis_kernel_times: bad arguments"});
+    std::string s;
+    for (size_t k = 0; k < c->knames.size(); ++k) {
+        char line[160];
+        std::snprintf(line, sizeof line, "%s %lld %.6f\n", c->knames[k].c_str(), c->kcount[k],
+                      c->kcount[k] ? c->ksum_ms[k] / c->kcount[k] : 0.0);
+        s += line;
+    }
+    std::snprintf(buf, buflen, "%s", s.c_str());
+    return TIS_OK;
+}
+
+extern "C" int tis_set_profiling(tis_ctx *c, int on) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_set_profiling: null context"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->profiling = on != 0;
+        c->knames.clear();
+        c->ksum_ms.clear();
+        c->kcount.clear();
+        c->pending.clear();
+        c->event_next = 0;
+        return TIS_OK;
+    });
+}
+
+// Philox4x32-10 on the host, for known-answer tests of the device generator.
+extern "C" void tis_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    Philox4 o = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = o.v[i];
+}

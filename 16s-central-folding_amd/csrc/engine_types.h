@@ -1,0 +1,116 @@
+// Device-side layout of the engine: per-replica parameters, per-replica
+// dynamic state and the flat device buffers shared by all replicas.
+//
+// Replicas are concatenated: replica r owns particles [p0, p0 + n), bonds
+// [bond0, bond0 + nbond) and so on; every bead index stored on the device is
+// already global, so kernels never add offsets to topology entries.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tis {
+
+constexpr int kNT = 9;            // parameter types
+constexpr int kMaxCells = 8192;   // cells per replica (LDS counting sort)
+constexpr int kBlock = 256;
+
+// Static parameters of one replica (read-only on the device).
+struct RepParams {
+    int p0, n, nbeads, kh;        // particle range, RNA beads, recentring bead (global)
+    double L[3], hL[3];
+    double T, lB, h, vcap;        // vcap = 300 T (speed cap, fast.cpp:604-620)
+    double rsr2, rlr2, skin;
+    int M[3];
+    double lc[3];
+    int cell0;                    // offset into the cell-start array (ncell + 1 entries)
+    long long pool0, pool_cap;    // neighbour-pool region
+    int bond0, nbond, ang0, nang;
+    int st30, nst3, st20, nst2;
+    int hb0, nhb, site0, nsite, inc0, ninc;  // HB candidates, sites, site incidences
+    int lr_every;
+    double mass[kNT], K1[kNT], K2[kNT], sigma[kNT];
+    // pair tables, index ta * 9 + tb
+    double d2lj[kNT * kNT], dlj[kNT * kNT], flj[kNT * kNT], elj[kNT * kNT], d2min[kNT * kNT],
+        qlb[kNT * kNT];
+};
+
+// Dynamic state of one replica (read-write on the device).
+struct RepState {
+    long long klok;               // step about to be executed
+    double progress;              // PROGRESS, fast.cpp:1768-1775
+    unsigned long long maxd2;     // bits of max |x - x_old|^2 of the current step
+    int need_rebuild;
+    int lr_due;
+    long long n_rebuilds;
+    long long pool_cursor;
+    int n_act;                    // live HB candidates after the last rebuild
+    int n_asite;                  // sites with at least one live candidate
+    int n_ainc;                   // entries of the per-site live lists
+    int overflow;                 // bit 1 pool, 2 HB caps, 4 stream, 8 cell grid
+    long long stream_cursor;
+    long long hb_draws;           // Philox draws of the current step's HB resolution
+    int hb_iters;
+    int pad;
+    double e[12];                 // energies of the last force evaluation
+};
+enum { EI_E2_HB, EI_E3_HB, EI_E2_ST, EI_E3_ST, EI_WCA, EI_COUL_SR, EI_COUL_LR, EI_BOND, EI_ANGLE };
+
+// Caps of the HB resolution working set held in LDS (per replica).
+constexpr int kActMax = 4096;     // live candidates
+constexpr int kASiteMax = 4096;   // sites with live candidates
+constexpr int kAIncMax = 12288;   // incidences of live candidates
+
+struct Dev {
+    int n_rep;
+    int n_tot;
+    int precision;
+    int rng;
+    uint64_t seed;
+    double noise_scale;
+    RepParams *rp;
+    RepState *rs;
+    // particles (global index)
+    double4 *pos, *old, *vel;
+    double4 *f_sr, *f_lr, *f_mb;
+    int *ptype;                   // type | cls << 4 | charged << 8
+    int *prep;                    // replica of each particle
+    // cells and lists
+    int *cell_of, *cell_rank, *cell_start, *sorted;
+    int *nb_start, *nb_nsr, *nb_nlr;
+    int *pool;
+    // topology
+    int2 *bond_ij;
+    double2 *bond_kr;
+    int4 *ang_ijk;
+    double2 *ang_kt;
+    int *st3_b;                   // 6 per stack
+    double *st3_g;                // 6 per stack
+    double *st3_E;
+    int2 *st3_dis;                // secondary stacks switched off (replica-local index, -1 none)
+    int *st2_b;                   // 8 per stack (7 used)
+    double2 *st2_rE;
+    int *hb_b;                    // 6 per HB
+    double *hb_g;                 // 6 per HB
+    double *hb_E;
+    int *hb_dode;
+    int *hb_inc_off, *hb_inc;     // sites of each HB (global site ids), CSR over global HB ids
+    int *site_off, *site_hb;      // HBs of each site (global HB ids), CSR over global site ids
+    int *site_val;
+    // HB dynamic (rebuild products)
+    signed char *hb_status;       // -1 / 0 per candidate
+    int *act_hb;                  // per replica region [hb0, hb0 + nhb): live candidate ids
+    int *act_idx;                 // candidate id -> live index or -1
+    int *asite_id;                // region [site0, ...): site of each active site
+    int *asite_off;               // region [site0 + rep, ...]: CSR offsets (n_asite + 1)
+    int *asite_list;              // region [inc0, ...): live indices per active site, static order
+    int *ahb_inc_off;             // region [hb0 + rep, ...]: CSR offsets per live candidate
+    int *ahb_inc;                 // region [inc0, ...): active-site index of each incidence
+    // output / scratch
+    double *st_energy;            // per tertiary stack
+    double *maxwell;              // 3000 per replica (stream mode)
+    int32_t *stream;              // raw glibc values, region per replica
+    long long *stream_base, *stream_len;
+    unsigned int *done_counter;
+};
+
+}  // namespace tis

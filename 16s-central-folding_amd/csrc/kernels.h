@@ -1,0 +1,904 @@
+// Kernels of one Langevin step (fast.cpp:6380-6386) for a batch of replicas.
+//
+//   k_hb          one workgroup per replica: HB energies, valence resolution,
+//                 HB forces; reference-stream noise table (hydrogen_bonds_interactions
+//                 fast.cpp:2145-2350, full_forces draws :2598-2604)
+//   k_force       stacking (one workgroup per replica), backbone (thread per term),
+//                 short/long-range pair gather (thread per particle)
+//                 (stacking_interactions :1911-2141, polymer_constraints :2354-2520,
+//                 SR_/LR_interactions_lists :1783-1907)
+//   k_integrate   noise, leap-frog with speed cap, recentring, ion wrap, displacement
+//                 reduction and rebuild decision (full_forces :2590-2625,
+//                 move_rigid_units :2660-2732, check_shifts :1741-1776)
+//   k_bin         LDS counting sort of particles into cells (separate_in_cells :1461-1542)
+//   k_list        full neighbour lists, SR and LR, from the 27 neighbouring cells
+//                 (populate_lists :1546-1700)
+//   k_hbrebuild   live HB candidates and the per-site tables of the resolution
+//                 (populate_lists :1702-1736)
+#pragma once
+#include "device_math.h"
+#include "engine_types.h"
+
+namespace tis {
+
+// ------------------------------------------------------------ block helpers
+TIS_DEV int lane_id() { return threadIdx.x & 63; }
+TIS_DEV int wave_id() { return threadIdx.x >> 6; }
+
+// Ordered exclusive prefix of a per-thread count across the block; *total gets
+// the block sum.  scratch must hold blockDim/64 + 1 ints.  All threads call.
+TIS_DEV int block_excl_sum(int v, int *scratch, int *total) {
+    int lane = lane_id(), w = wave_id(), nw = (blockDim.x + 63) >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) scratch[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < nw; ++i) { int t = scratch[i]; scratch[i] = s; s += t; }
+        scratch[nw] = s;
+    }
+    __syncthreads();
+    int res = scratch[w] + x - v;
+    *total = scratch[nw];
+    __syncthreads();
+    return res;
+}
+
+TIS_DEV double4 d4add(double4 a, double4 b) { return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, 0.0); }
+
+template <class R>
+TIS_DEV void atomic_add_force(double4 *f, int i, V3<R> v) {
+    unsafeAtomicAdd(&f[i].x, (double)v.x);
+    unsafeAtomicAdd(&f[i].y, (double)v.y);
+    unsafeAtomicAdd(&f[i].z, (double)v.z);
+}
+
+TIS_DEV double block_sum_double(double v, double *scratch) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    int w = wave_id(), nw = (blockDim.x + 63) >> 6;
+    if (lane_id() == 0) scratch[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nw; ++i) s += scratch[i];
+    __syncthreads();
+    return s;  // valid in thread 0
+}
+
+// ------------------------------------------------------------ pair terms
+// WCA force factor (LJ_FORCE, fast.cpp:801-821) times the pair vector.
+template <class R>
+TIS_DEV R wca_factor(double r2, int ab, const RepParams &P) {
+    if (r2 > P.d2lj[ab]) return (R)0;
+    R r = sqrt((R)r2);
+    R rho = r + (R)kSigLJ - (R)P.dlj[ab];
+    R u = (R)kSigLJ / rho;
+    R u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u14 = u8 * u4 * u2;
+    return (R)P.flj[ab] * (u14 - u8) * rho / r;
+}
+// WCA_ENERGY (fast.cpp:749-771).
+TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
+    if (r2 > P.d2lj[ab]) return 0.0;
+    if (r2 < P.d2min[ab]) return P.elj[ab] * 1001.0;
+    double rho = sqrt(r2) + kSigLJ - P.dlj[ab];
+    double u = kSigLJ / rho;
+    double u2 = u * u, u4 = u2 * u2, u6 = u4 * u2, u12 = u6 * u6;
+    return P.elj[ab] * (u12 - 2.0 * u6 + 1.0);
+}
+
+// ------------------------------------------------------------ k_bin
+__global__ void __launch_bounds__(1024) k_bin(Dev D) {
+    const int r = blockIdx.x;
+    RepState &S = D.rs[r];
+    if (!S.need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    __shared__ int cnt[kMaxCells];
+    __shared__ int scratch[20];
+    const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2], nc = M0 * M1 * M2;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) cnt[c] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
+        const int p = P.p0 + i;
+        const double4 x = D.pos[p];
+        const double xs[3] = {x.x, x.y, x.z};
+        int ci[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double t = xs[a];
+            if (t < 0) t += P.L[a];
+            else if (t > P.L[a]) t -= P.L[a];
+            int c = (int)ceil(t / P.lc[a]) - 1;
+            ci[a] = c < 0 ? 0 : (c >= P.M[a] ? P.M[a] - 1 : c);
+        }
+        const int cell = (ci[0] * M1 + ci[1]) * M2 + ci[2];
+        D.cell_of[p] = cell;
+        D.cell_rank[p] = atomicAdd(&cnt[cell], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the cell counts: contiguous chunk per thread, then block scan
+    const int per = (nc + blockDim.x - 1) / blockDim.x;
+    const int c0 = threadIdx.x * per;
+    int local = 0;
+    for (int c = c0; c < c0 + per && c < nc; ++c) local += cnt[c];
+    int total;
+    int run = block_excl_sum(local, scratch, &total);
+    for (int c = c0; c < c0 + per && c < nc; ++c) {
+        int v = cnt[c];
+        cnt[c] = run;
+        D.cell_start[P.cell0 + c] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) D.cell_start[P.cell0 + nc] = total;
+    __syncthreads();
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
+        const int p = P.p0 + i;
+        D.sorted[P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p]] = p;
+    }
+    if (threadIdx.x == 0) S.pool_cursor = 0;
+}
+
+// ------------------------------------------------------------ k_list
+// Visits every particle of the 27 (fewer when a dimension has < 3 cells)
+// cells around particle p's cell.
+template <class F>
+TIS_DEV void for_neighbour_cells(const Dev &D, const RepParams &P, int cell, F &&visit) {
+    const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
+    const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
+    const int lo0 = M0 >= 3 ? -1 : 0, hi0 = M0 >= 2 ? 1 : 0;
+    const int lo1 = M1 >= 3 ? -1 : 0, hi1 = M1 >= 2 ? 1 : 0;
+    const int lo2 = M2 >= 3 ? -1 : 0, hi2 = M2 >= 2 ? 1 : 0;
+    for (int a = lo0; a <= hi0; ++a) {
+        int n0 = c0 + a; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0;
+        for (int b = lo1; b <= hi1; ++b) {
+            int n1 = c1 + b; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0;
+            for (int c = lo2; c <= hi2; ++c) {
+                int n2 = c2 + c; n2 += n2 < 0 ? M2 : 0; n2 -= n2 >= M2 ? M2 : 0;
+                const int nc = (n0 * M1 + n1) * M2 + n2;
+                const int s = D.cell_start[P.cell0 + nc], e = D.cell_start[P.cell0 + nc + 1];
+                for (int q = s; q < e; ++q) visit(D.sorted[P.p0 + q]);
+            }
+        }
+    }
+}
+
+// mode 0: count only (sizes the pool), mode 1: build.
+__global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= D.n_tot) return;
+    const int r = D.prep[t];   // sorted slot t lies in replica r's particle range
+    RepState &S = D.rs[r];
+    if (!S.need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    const int p = D.sorted[t];
+    const double4 xi = D.pos[p];
+    const bool chg_i = (D.ptype[p] >> 8) & 1;
+    const int cell = D.cell_of[p];
+    int nsr = 0, nlr = 0;
+    for_neighbour_cells(D, P, cell, [&](int j) {
+        if (j == p) return;
+        double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
+        if (r2 <= P.rsr2) ++nsr;
+        else if (chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1)) ++nlr;
+    });
+    const long long base = atomicAdd((unsigned long long *)&S.pool_cursor, (unsigned long long)(nsr + nlr));
+    if (mode == 0) return;
+    if (base + nsr + nlr > P.pool_cap) {
+        atomicOr(&S.overflow, 1);
+        D.nb_start[p] = 0; D.nb_nsr[p] = 0; D.nb_nlr[p] = 0;
+        return;
+    }
+    int *out = D.pool + P.pool0 + base;
+    int a = 0, b = nsr;
+    for_neighbour_cells(D, P, cell, [&](int j) {
+        if (j == p) return;
+        double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
+        if (r2 <= P.rsr2) out[a++] = j;
+        else if (chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1)) out[b++] = j;
+    });
+    D.nb_start[p] = (int)base;
+    D.nb_nsr[p] = nsr;
+    D.nb_nlr[p] = nlr;
+}
+
+// ------------------------------------------------------------ k_hbrebuild
+// Live candidates (k11, k21 within the SR radius), the active sites in site
+// order with their live candidates in the static HB_ATOM order, and each live
+// candidate's incidences as active-site indices.  Dynamic LDS: nsite ints.
+__global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
+    const int r = blockIdx.x;
+    RepState &S = D.rs[r];
+    if (!S.need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    extern __shared__ int site_map[];
+    __shared__ int scratch[20];
+    int n_act = 0;
+    for (int k0 = 0; k0 < P.nhb; k0 += blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        int live = 0;
+        int g = P.hb0 + k;
+        if (k < P.nhb) {
+            const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
+            live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
+            D.hb_status[g] = live ? 0 : -1;
+        }
+        int tot;
+        int rank = block_excl_sum(live, scratch, &tot);
+        if (k < P.nhb) {
+            if (live) { D.act_hb[P.hb0 + n_act + rank] = g; D.act_idx[g] = n_act + rank; }
+            else D.act_idx[g] = -1;
+        }
+        n_act += tot;
+    }
+    __syncthreads();
+    int n_as = 0, n_inc = 0;
+    for (int s0 = 0; s0 < P.nsite; s0 += blockDim.x) {
+        const int s = s0 + threadIdx.x;
+        const int gs = P.site0 + s;
+        int cnt = 0;
+        if (s < P.nsite)
+            for (int q = D.site_off[gs]; q < D.site_off[gs + 1]; ++q) cnt += D.hb_status[D.site_hb[q]] == 0;
+        int tot_s, tot_i;
+        const int rank = block_excl_sum(cnt > 0, scratch, &tot_s);
+        const int off = block_excl_sum(cnt, scratch, &tot_i);
+        if (s < P.nsite) {
+            if (cnt > 0) {
+                const int ai = n_as + rank;
+                site_map[s] = ai;
+                D.asite_id[P.site0 + ai] = gs;
+                D.asite_off[P.site0 + r + ai] = n_inc + off;
+                int w = P.inc0 + n_inc + off;
+                for (int q = D.site_off[gs]; q < D.site_off[gs + 1]; ++q) {
+                    const int k = D.site_hb[q];
+                    if (D.hb_status[k] == 0) D.asite_list[w++] = D.act_idx[k];
+                }
+            } else {
+                site_map[s] = -1;
+            }
+        }
+        n_as += tot_s;
+        n_inc += tot_i;
+    }
+    if (threadIdx.x == 0) D.asite_off[P.site0 + r + n_as] = n_inc;
+    __syncthreads();
+    int n_ai = 0;
+    for (int a0 = 0; a0 < n_act; a0 += blockDim.x) {
+        const int a = a0 + threadIdx.x;
+        int c = 0, g = 0;
+        if (a < n_act) {
+            g = D.act_hb[P.hb0 + a];
+            c = D.hb_inc_off[g + 1] - D.hb_inc_off[g];
+        }
+        int tot;
+        const int off = block_excl_sum(c, scratch, &tot);
+        if (a < n_act) {
+            D.ahb_inc_off[P.hb0 + r + a] = n_ai + off;
+            for (int q = 0; q < c; ++q)
+                D.ahb_inc[P.inc0 + n_ai + off + q] = site_map[D.hb_inc[D.hb_inc_off[g] + q] - P.site0];
+        }
+        n_ai += tot;
+    }
+    if (threadIdx.x == 0) {
+        D.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
+        S.n_act = n_act;
+        S.n_asite = n_as;
+        S.n_ainc = n_inc;
+        if (n_act > kActMax || n_as > kASiteMax || n_inc > kAIncMax || n_ai > kAIncMax) atomicOr(&S.overflow, 2);
+    }
+}
+
+// ------------------------------------------------------------ uniform draws
+// Draw number idx (0-based within the current step) of replica r:
+// Philox keyed by (seed, replica) with the step and index in the counter, or
+// the caller-fed glibc stream at the replica's cursor.
+TIS_DEV double hb_uniform(const Dev &D, int r, const RepState &S, long long idx, bool *overflow) {
+    if (D.rng == 0) {
+        Philox4 o = philox4x32_10((uint32_t)S.klok, (uint32_t)(S.klok >> 32), (uint32_t)idx, kTagHb,
+                                  (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), (uint32_t)(D.seed >> 32));
+        return u01_closed_open(o.v[0]);
+    }
+    long long c = S.stream_cursor + idx;
+    if (c >= D.stream_len[r]) { *overflow = true; return 0.5; }
+    return (double)D.stream[D.stream_base[r] + c] / kRandMax;
+}
+
+TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fast.cpp:2257)
+    int j = (int)ceil(u * n);
+    if (j == n + 1) j = n;
+    if (j == 0) j = 1;
+    return j;
+}
+
+// ------------------------------------------------------------ k_hb
+// LDS layout (bytes): E[kActMax] double | flag[kActMax] u8 | excess[kASiteMax] int |
+// A0, A1 [kASiteMax] short | asite_off[kASiteMax+1] short | asite_list[kAIncMax] short |
+// ahb_off[kActMax+1] short | ahb_inc[kAIncMax] short
+constexpr size_t kHbLds = kActMax * 8 + kActMax + kASiteMax * 4 + 2 * kASiteMax * 2 + (kASiteMax + 1) * 2 +
+                          kAIncMax * 2 + (kActMax + 1) * 2 + kAIncMax * 2 + 64;
+
+enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
+
+template <class R>
+TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused) {
+    (void)d_unused;
+    const int *b = D.hb_b + 6 * g;
+    const double *geo = D.hb_g + 6 * g;
+    const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
+    const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
+    V3<R> v14 = diff<R>(p1, p4);
+    R rr = sqrt(dot(v14, v14));
+    R c1, c2;
+    V3<R> ga, gb;
+    angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
+    angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
+    R th1 = acos(c1), th2 = acos(c2);
+    R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
+    R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
+    R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
+    psi0[0] = wrap_target<R>(ps, (R)geo[3]);
+    psi0[1] = wrap_target<R>(ps1, (R)geo[4]);
+    psi0[2] = wrap_target<R>(ps2, (R)geo[5]);
+    R dr = rr - (R)geo[0];
+    R q = (R)5 * dr * dr;
+    q += (R)1.5 * (th1 - (R)geo[1]) * (th1 - (R)geo[1]);
+    q += (R)1.5 * (th2 - (R)geo[2]) * (th2 - (R)geo[2]);
+    q += (R)0.15 * (ps - psi0[0]) * (ps - psi0[0]);
+    q += (R)0.15 * (ps1 - psi0[1]) * (ps1 - psi0[1]);
+    q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
+    *q_out = q;
+}
+
+// Forces of the six-bead Gaussian/Lorentzian-type term: with w the coefficient
+// (-E for HB, -E/(1+Q) for stacks), each harmonic piece k (y - y0)^2 is applied
+// with k scaled by w (stacking_*_force calls, fast.cpp:2043-2055, 2333-2345).
+template <class R>
+TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double *geo, R w, const R psi0[3],
+                           R kr, R kth, R kpsi) {
+    const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
+    const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
+    // bond (b1, b4)
+    {
+        V3<R> d = diff<R>(p1, p4);
+        R t = bond_coef<R>(d, kr * w, (R)geo[0], nullptr);
+        atomic_add_force(F, b[1], d * (-t));
+        atomic_add_force(F, b[4], d * t);
+    }
+    // angles (b0, b1, b4) and (b3, b4, b1) with the near-pi guard of stacking_valence_force
+    const int tri[2][3] = {{b[0], b[1], b[4]}, {b[3], b[4], b[1]}};
+    const double4 tp[2][3] = {{p0, p1, p4}, {p3, p4, p1}};
+    for (int a = 0; a < 2; ++a) {
+        V3<R> ba = diff<R>(tp[a][1], tp[a][0]), bc = diff<R>(tp[a][1], tp[a][2]);
+        R c;
+        V3<R> g0, g2;
+        angle_grad<R>(ba, bc, &c, &g0, &g2);
+        if (c < (R)-0.99995) continue;
+        R s = sqrt((R)1 - c * c);
+        R th = acos(c);
+        R coef = (R)-2 * (kth * w) * ((R)geo[1 + a] - th) / s;
+        V3<R> f0 = g0 * coef, f2 = g2 * coef;
+        atomic_add_force(F, tri[a][0], f0);
+        atomic_add_force(F, tri[a][2], f2);
+        atomic_add_force(F, tri[a][1], (f0 + f2) * (R)-1);
+    }
+    // dihedrals (b0,b1,b4,b3), (b4,b1,b0,b2), (b1,b4,b3,b5)
+    const int quad[3][4] = {{b[0], b[1], b[4], b[3]}, {b[4], b[1], b[0], b[2]}, {b[1], b[4], b[3], b[5]}};
+    const double4 qp[3][4] = {{p0, p1, p4, p3}, {p4, p1, p0, p2}, {p1, p4, p3, p5}};
+    for (int a = 0; a < 3; ++a) {
+        V3<R> g[4];
+        R psi = dihedral_grad<R>(diff<R>(qp[a][0], qp[a][1]), diff<R>(qp[a][1], qp[a][2]),
+                                 diff<R>(qp[a][2], qp[a][3]), g);
+        R coef = (R)2 * (kpsi * w) * (psi0[a] - psi);
+        for (int c = 0; c < 4; ++c) atomic_add_force(F, quad[a][c], g[c] * coef);
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
+    // mode bit 0: step (advance the stream cursor, build the noise table); bit 1: forces on
+    const int r = blockIdx.x;
+    RepState &S = D.rs[r];
+    const RepParams &P = D.rp[r];
+    double4 *F = D.f_mb;
+    extern __shared__ unsigned char lds[];
+    double *E = (double *)lds;
+    unsigned char *flag = (unsigned char *)(E + kActMax);
+    int *excess = (int *)(flag + kActMax);
+    short *A0 = (short *)(excess + kASiteMax);
+    short *A1 = A0 + kASiteMax;
+    short *aoff = A1 + kASiteMax;
+    short *alist = aoff + kASiteMax + 1;
+    short *hoff = alist + kAIncMax;
+    short *hinc = hoff + kActMax + 1;
+    __shared__ double red[8];
+    __shared__ int sh_iters;
+    __shared__ long long sh_draws;
+    __shared__ int sh_err;
+    const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
+    if (S.overflow & 2) return;
+    if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; }
+    // ---- preload the rebuild tables
+    for (int i = threadIdx.x; i <= n_as; i += blockDim.x) aoff[i] = (short)D.asite_off[P.site0 + r + i];
+    for (int i = threadIdx.x; i < n_ainc; i += blockDim.x) alist[i] = (short)D.asite_list[P.inc0 + i];
+    for (int i = threadIdx.x; i <= n_act; i += blockDim.x) hoff[i] = (short)D.ahb_inc_off[P.hb0 + r + i];
+    const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
+    for (int i = threadIdx.x; i < n_hinc; i += blockDim.x) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
+    // ---- energies and the distance gate
+    for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+        const int g = D.act_hb[P.hb0 + a];
+        const int *b = D.hb_b + 6 * g;
+        V3<R> v = diff<R>(D.pos[b[1]], D.pos[b[4]]);
+        R rr = sqrt(dot(v, v));
+        if (fabs(rr - (R)D.hb_g[6 * g]) > (R)2) {
+            flag[a] = FLAG_GATED;
+            E[a] = 0.0;
+            continue;
+        }
+        R q, psi0[3];
+        hb_geometry<R>(D, g, &q, psi0, nullptr);
+        E[a] = (double)((R)D.hb_E[g] * exp(-q));
+        flag[a] = FLAG_LIVE;
+    }
+    __syncthreads();
+    // excess = live incidences - valence, per active site
+    for (int s = threadIdx.x; s < n_as; s += blockDim.x) {
+        int c = 0;
+        for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
+        excess[s] = c - D.site_val[D.asite_id[P.site0 + s]];
+    }
+    __syncthreads();
+    // ---- valence resolution (fast.cpp:2239-2315), one wavefront
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int nA = 0;
+        for (int s0 = 0; s0 < n_as; s0 += 64) {
+            const int s = s0 + lane;
+            const bool keep = s < n_as && excess[s] > 0;
+            const unsigned long long m = __ballot(keep);
+            if (keep) A0[nA + __popcll(m & ((1ull << lane) - 1ull))] = (short)s;
+            nA += __popcll(m);
+        }
+        short *A = A0, *B = A1;
+        long long base = 0;
+        int iters = 0;
+        bool ovf = false;
+        const double T = P.T;
+        while (nA > 0) {
+            __builtin_amdgcn_wave_barrier();
+            if (++iters > 1000000) { ovf = true; break; }   // each pass removes >= 1 excess unit
+            // draws for this iteration: lane l holds draws base+l and base+64+l
+            const double u_lo = hb_uniform(D, r, S, base + lane, &ovf);
+            const double u_hi = hb_uniform(D, r, S, base + 64 + lane, &ovf);
+            const double u0 = __shfl(u_lo, 0, 64);
+            const int site = A[ceil_index(u0, nA) - 1];
+            const int o0 = aoff[site], len = aoff[site + 1] - o0;
+            if (len > 64) { ovf = true; break; }
+            int bond = -1;
+            bool keep = false;
+            if (lane < len) { bond = alist[o0 + lane]; keep = flag[bond] == FLAG_LIVE; }
+            const unsigned long long km = __ballot(keep);
+            const int nK = __popcll(km);
+            if (nK == 0) { ovf = true; break; }
+            // bonds[i] held by lane i (order of the static site list)
+            const int pos = __popcll(km & ((1ull << lane) - 1ull));
+            // forward permute: kept lane sends its bond to lane pos; the others to
+            // lane 63, which is never read (nK <= len <= 64, and nK == 64 only if
+            // every lane keeps)
+            int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
+            // shuffle: swap i with ceil(u nK) - 1 for i = 0..nK-1 (draws 1..nK)
+            for (int i = 0; i < nK; ++i) {
+                const int di = 1 + i;
+                const double ui = di < 64 ? __shfl(u_lo, di, 64) : __shfl(u_hi, di - 64, 64);
+                const int jj = ceil_index(ui, nK) - 1;
+                const int bi = __shfl(bv, i, 64), bj = __shfl(bv, jj, 64);
+                if (lane == i) bv = bj;
+                else if (lane == jj) bv = bi;
+            }
+            const double Ev = lane < nK ? E[bv] : 0.0;
+            int kk = 0;
+            double Ek = __shfl(Ev, 0, 64);
+            for (int i = 1; i < nK; ++i) {
+                const int di = nK + i;   // selection draws follow the shuffle draws
+                const double ui = di < 64 ? __shfl(u_lo, di, 64) : __shfl(u_hi, di - 64, 64);
+                const double Ei = __shfl(Ev, i, 64);
+                const double pr = exp((Ei - Ek) / T);
+                if (ui < pr) { kk = i; Ek = Ei; }
+            }
+            const int off_bond = __shfl(bv, kk, 64);
+            base += 2 * nK;
+            if (lane == 0) flag[off_bond] = FLAG_OFF;
+            const int h0 = hoff[off_bond], hn = hoff[off_bond + 1] - h0;
+            for (int q = lane; q < hn; q += 64) atomicSub(&excess[hinc[h0 + q]], 1);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            int nB = 0;
+            for (int s0 = 0; s0 < nA; s0 += 64) {
+                const int e = s0 + lane;
+                const int sv = e < nA ? A[e] : 0;
+                const bool kp = e < nA && excess[sv] > 0;
+                const unsigned long long m = __ballot(kp);
+                if (kp) B[nB + __popcll(m & ((1ull << lane) - 1ull))] = (short)sv;
+                nB += __popcll(m);
+            }
+            short *t = A; A = B; B = t;
+            nA = nB;
+        }
+        if (lane == 0) {
+            sh_iters = iters;
+            sh_draws = base;
+            if (ovf) sh_err = 1;
+        }
+    }
+    __syncthreads();
+    // ---- forces of the surviving HBs and the E2/E3 sums
+    double e2 = 0, e3 = 0;
+    if (mode & 2) {
+        for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+            if (flag[a] != FLAG_LIVE) continue;
+            const int g = D.act_hb[P.hb0 + a];
+            R q, psi0[3];
+            hb_geometry<R>(D, g, &q, psi0, nullptr);
+            const double e = E[a];
+            if (D.hb_dode[g]) e3 += e; else e2 += e;
+            sextet_forces<R>(D, F, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+        }
+    }
+    double s2 = block_sum_double(e2, red);
+    double s3 = block_sum_double(e3, red);
+    if (threadIdx.x == 0) {
+        S.e[EI_E2_HB] = s2;
+        S.e[EI_E3_HB] = s3;
+        S.hb_iters = sh_iters;
+        S.hb_draws = sh_draws;
+        if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);
+    }
+    // ---- reference-stream noise table: 1500 Box-Muller pairs after the HB draws
+    if ((mode & 1) && D.rng == 1) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long c = S.stream_cursor + sh_draws;
+            const long long lim = D.stream_len[r];
+            const int32_t *st = D.stream + D.stream_base[r];
+            double *mw = D.maxwell + 3000 * (size_t)r;
+            bool bad = false;
+            for (int i = 0; i < 3000; i += 2) {
+                double r1;
+                do {
+                    if (c >= lim) { bad = true; r1 = 0.5; break; }
+                    r1 = (double)st[c++] / kRandMax;
+                } while (r1 == 0);
+                double r2 = 0.5;
+                if (c < lim) r2 = (double)st[c++] / kRandMax; else bad = true;
+                const double a = sqrt(-2.0 * log(r1)), b = 2.0 * kPi * r2;
+                mw[i] = a * cos(b);
+                mw[i + 1] = a * sin(b);
+            }
+            S.stream_cursor = c;
+            if (bad) atomicOr(&S.overflow, 4);
+        }
+    } else if (threadIdx.x == 0 && D.rng == 1) {
+        S.stream_cursor += sh_draws;   // evaluation consumes the HB draws only
+    }
+}
+
+// ------------------------------------------------------------ k_force
+// Blocks [0, n_rep): stacking of replica b.  Then backbone terms, then the pair gather.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned mask, int eval) {
+    double4 *F = D.f_mb;
+    if ((int)blockIdx.x < D.n_rep) {
+        if (!(mask & 1)) return;
+        const int r = blockIdx.x;
+        const RepParams &P = D.rp[r];
+        RepState &S = D.rs[r];
+        __shared__ unsigned char off[4096];
+        __shared__ double red[8];
+        for (int k = threadIdx.x; k < P.nst2; k += blockDim.x) off[k] = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < P.nst3; k += blockDim.x) {
+            const int g = P.st30 + k;
+            const int *b = D.st3_b + 6 * g;
+            V3<R> v = diff<R>(D.pos[b[1]], D.pos[b[4]]);
+            R dr = sqrt(dot(v, v)) - (R)D.st3_g[6 * g];
+            if (dr < (R)10) {
+                const int2 dd = D.st3_dis[g];
+                if (dd.x >= 0) off[dd.x] = 1;
+                if (dd.y >= 0) off[dd.y] = 1;
+            }
+        }
+        __syncthreads();
+        double e2 = 0, e3 = 0;
+        for (int k = threadIdx.x; k < P.nst3 + P.nst2; k += blockDim.x) {
+            if (k < P.nst3) {
+                const int g = P.st30 + k;
+                const int *b = D.st3_b + 6 * g;
+                const double *geo = D.st3_g + 6 * g;
+                const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
+                const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
+                V3<R> v14 = diff<R>(p1, p4);
+                R dr = sqrt(dot(v14, v14)) - (R)geo[0];
+                R c1, c2;
+                V3<R> ga, gb;
+                angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
+                angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
+                R th1 = acos(c1), th2 = acos(c2);
+                R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
+                R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
+                R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
+                R psi0[3] = {wrap_target<R>(ps, (R)geo[3]), wrap_target<R>(ps1, (R)geo[4]),
+                             wrap_target<R>(ps2, (R)geo[5])};
+                R q = (R)1 + (R)5 * dr * dr;
+                q += (R)1.5 * (th1 - (R)geo[1]) * (th1 - (R)geo[1]);
+                q += (R)1.5 * (th2 - (R)geo[2]) * (th2 - (R)geo[2]);
+                q += (R)0.15 * (ps - psi0[0]) * (ps - psi0[0]);
+                q += (R)0.15 * (ps1 - psi0[1]) * (ps1 - psi0[1]);
+                q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
+                const R e = (R)D.st3_E[g] / q;
+                D.st_energy[g] = (double)e;
+                e3 += (double)e;
+                sextet_forces<R>(D, F, b, geo, -e / q, psi0, (R)5, (R)1.5, (R)0.15);
+            } else {
+                const int m = k - P.nst3;
+                if (off[m]) continue;
+                const int g = P.st20 + m;
+                const int *b = D.st2_b + 8 * g;   // P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2
+                const double2 rE = D.st2_rE[g];
+                const double4 q0 = D.pos[b[0]], q1 = D.pos[b[1]], q2 = D.pos[b[2]], q3 = D.pos[b[3]];
+                const double4 q4 = D.pos[b[4]], q5 = D.pos[b[5]], q6 = D.pos[b[6]];
+                V3<R> v25 = diff<R>(q2, q5);
+                R rr = sqrt(dot(v25, v25));
+                V3<R> g1[4], g2[4];
+                R ps1 = dihedral_grad<R>(diff<R>(q0, q1), diff<R>(q1, q3), diff<R>(q3, q4), g1);
+                R ps2 = dihedral_grad<R>(diff<R>(q6, q4), diff<R>(q4, q3), diff<R>(q3, q1), g2);
+                R t1 = wrap_target<R>(ps1, (R)-2.58684), t2 = wrap_target<R>(ps2, (R)3.07135);
+                R q = (R)1 + (R)1.4 * (rr - (R)rE.x) * (rr - (R)rE.x);
+                q += (R)4 * (ps1 - t1) * (ps1 - t1);
+                q += (R)4 * (ps2 - t2) * (ps2 - t2);
+                const R e = (R)rE.y / q;
+                e2 += (double)e;
+                const R w = -e / q;
+                const R tb = (R)2 * ((R)1.4 * w) * ((R)rE.x - rr) / rr;
+                atomic_add_force(F, b[2], v25 * (-tb));
+                atomic_add_force(F, b[5], v25 * tb);
+                const R c1 = (R)2 * ((R)4 * w) * (t1 - ps1), c2 = (R)2 * ((R)4 * w) * (t2 - ps2);
+                const int i1[4] = {b[0], b[1], b[3], b[4]}, i2[4] = {b[6], b[4], b[3], b[1]};
+                for (int c = 0; c < 4; ++c) {
+                    atomic_add_force(F, i1[c], g1[c] * c1);
+                    atomic_add_force(F, i2[c], g2[c] * c2);
+                }
+            }
+        }
+        double s2 = block_sum_double(e2, red);
+        double s3 = block_sum_double(e3, red);
+        if (threadIdx.x == 0) { S.e[EI_E2_ST] = s2; S.e[EI_E3_ST] = s3; }
+        return;
+    }
+    int blk = blockIdx.x - D.n_rep;
+    if (blk < nb_bonded) {
+        if (!(mask & 4)) return;
+        const int g = blk * blockDim.x + threadIdx.x;
+        const int nbond = D.rp[D.n_rep - 1].bond0 + D.rp[D.n_rep - 1].nbond;
+        const int nang = D.rp[D.n_rep - 1].ang0 + D.rp[D.n_rep - 1].nang;
+        int i, k;
+        double eb = 0, ea = 0;
+        if (g < nbond) {
+            const int2 ij = D.bond_ij[g];
+            const double2 kr = D.bond_kr[g];
+            i = ij.x; k = ij.y;
+            V3<R> d = diff<R>(D.pos[i], D.pos[k]);
+            R rr;
+            R t = bond_coef<R>(d, (R)kr.x, (R)kr.y, &rr);
+            atomic_add_force(F, i, d * (-t));
+            atomic_add_force(F, k, d * t);
+            if (eval) eb = kr.x * ((double)rr - kr.y) * ((double)rr - kr.y);
+        } else if (g < nbond + nang) {
+            const int4 t3 = D.ang_ijk[g - nbond];
+            const double2 kt = D.ang_kt[g - nbond];
+            i = t3.x; k = t3.z;
+            const int j = t3.y;
+            V3<R> ba = diff<R>(D.pos[j], D.pos[i]), bc = diff<R>(D.pos[j], D.pos[k]);
+            R c;
+            V3<R> g0, g2;
+            angle_grad<R>(ba, bc, &c, &g0, &g2);
+            R s = sqrt((R)1 - c * c), th = acos(c);
+            R coef = (R)-2 * (R)kt.x * ((R)kt.y - th) / s;
+            V3<R> f0 = g0 * coef, f2 = g2 * coef;
+            atomic_add_force(F, i, f0);
+            atomic_add_force(F, k, f2);
+            atomic_add_force(F, j, (f0 + f2) * (R)-1);
+            if (eval) ea = kt.x * ((double)th - kt.y) * ((double)th - kt.y);
+        } else {
+            return;
+        }
+        // remove the WCA force of the bonded / 1-3 pair (reduce_forces, fast.cpp:880-897)
+        const int r = D.prep[i];
+        const RepParams &P = D.rp[r];
+        double3 dd;
+        const double r2 = min_image_r2(D.pos[i], D.pos[k], P.L, P.hL, &dd);
+        const int ab = (D.ptype[i] & 15) * kNT + (D.ptype[k] & 15);
+        const R f = -wca_factor<R>(r2, ab, P);
+        V3<R> v = {(R)dd.x * f, (R)dd.y * f, (R)dd.z * f};
+        atomic_add_force(F, i, v);
+        atomic_add_force(F, k, v * (R)-1);
+        if (eval) {
+            RepState &S = D.rs[r];
+            unsafeAtomicAdd(&S.e[EI_BOND], eb);
+            unsafeAtomicAdd(&S.e[EI_ANGLE], ea);
+            unsafeAtomicAdd(&S.e[EI_WCA], -wca_energy(r2, ab, P));
+        }
+        return;
+    }
+    blk -= nb_bonded;
+    // ---- pair gather, thread per particle in cell order
+    const int t = blk * blockDim.x + threadIdx.x;
+    if (t >= D.n_tot) return;
+    const int p = D.sorted[t];
+    const int r = D.prep[p];
+    const RepParams &P = D.rp[r];
+    RepState &S = D.rs[r];
+    const double4 xi = D.pos[p];
+    const int pti = D.ptype[p];
+    const int ti = pti & 15;
+    const bool chg_i = (pti >> 8) & 1;
+    const int *nb = D.pool + P.pool0 + D.nb_start[p];
+    const int nsr = D.nb_nsr[p], nlr = D.nb_nlr[p];
+    if (mask & 8) {
+        R fx = 0, fy = 0, fz = 0;
+        double ew = 0, ec = 0;
+        for (int q = 0; q < nsr; ++q) {
+            const int j = nb[q];
+            double3 dd;
+            const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
+            const int ptj = D.ptype[j];
+            const int ab = ti * kNT + (ptj & 15);
+            R f = wca_factor<R>(r2, ab, P);
+            const bool both = chg_i && ((ptj >> 8) & 1) && r2 <= kCoulCut2;
+            if (both) {
+                const R rr = sqrt((R)r2);
+                f += (R)P.qlb[ab] / (rr * (R)r2);
+            }
+            fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
+            if (eval) {
+                ew += wca_energy(r2, ab, P);
+                if (both) ec += P.qlb[ab] / sqrt(r2);
+            }
+        }
+        D.f_sr[p] = make_double4((double)fx, (double)fy, (double)fz, 0.0);
+        if (eval) {
+            unsafeAtomicAdd(&S.e[EI_WCA], 0.5 * ew);
+            unsafeAtomicAdd(&S.e[EI_COUL_SR], 0.5 * ec);
+        }
+    }
+    if ((mask & 16) && (eval || S.lr_due)) {
+        R fx = 0, fy = 0, fz = 0;
+        double ec = 0;
+        for (int q = nsr; q < nsr + nlr; ++q) {
+            const int j = nb[q];
+            double3 dd;
+            const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
+            if (r2 > kCoulCut2) continue;
+            const int ab = ti * kNT + (D.ptype[j] & 15);
+            const R rr = sqrt((R)r2);
+            const R f = (R)P.qlb[ab] / (rr * (R)r2);
+            fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
+            if (eval) ec += P.qlb[ab] / sqrt(r2);
+        }
+        D.f_lr[p] = make_double4((double)fx, (double)fy, (double)fz, 0.0);
+        if (eval) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
+    }
+}
+
+// ------------------------------------------------------------ k_integrate
+TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double out[3]) {
+    const int local = p - P.p0;
+    if (D.rng == 0) {
+        Philox4 o = philox4x32_10((uint32_t)S.klok, (uint32_t)(S.klok >> 32), (uint32_t)local, kTagNoise,
+                                  (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), (uint32_t)(D.seed >> 32));
+        const double a = sqrt(-2.0 * log(u01_open_closed(o.v[0]))), b = 2.0 * kPi * u01_closed_open(o.v[1]);
+        const double c = sqrt(-2.0 * log(u01_open_closed(o.v[2]))), d = 2.0 * kPi * u01_closed_open(o.v[3]);
+        out[0] = a * cos(b);
+        out[1] = a * sin(b);
+        out[2] = c * cos(d);
+    } else {
+        // particle i (1-based) takes slots i % 1000 of the x, y, z blocks (fast.cpp:2616-2623)
+        const int k = (local + 1) % 1000;
+        const double *mw = D.maxwell + 3000 * (size_t)r;
+        out[0] = mw[k];
+        out[1] = mw[k + 1000];
+        out[2] = mw[k + 2000];
+    }
+}
+
+// leap_frog_move_atom (fast.cpp:570-642) for particle p; returns the new position
+// and velocity (position before recentring).
+TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double xn[3],
+                      double vn[3], double xs[3]) {
+    const int t = D.ptype[p] & 15;
+    const double4 f0 = D.f_sr[p], f1 = D.f_lr[p], f2 = D.f_mb[p];
+    double nz[3];
+    noise3(D, r, P, S, p, nz);
+    const double sg = P.sigma[t] * D.noise_scale;
+    const double f[3] = {f0.x + f1.x + f2.x + nz[0] * sg, f0.y + f1.y + f2.y + nz[1] * sg,
+                         f0.z + f1.z + f2.z + nz[2] * sg};
+    const double4 x = D.pos[p], xo = D.old[p], v = D.vel[p];
+    const double xv[3] = {x.x, x.y, x.z}, xov[3] = {xo.x, xo.y, xo.z}, vv[3] = {v.x, v.y, v.z};
+    for (int c = 0; c < 3; ++c) vn[c] = P.K1[t] * vv[c] + P.K2[t] * f[c];
+    double vsq = P.mass[t] * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]) / P.vcap;
+    if (vsq > 1.0) {
+        vsq = 1.0 / sqrt(vsq);
+        for (int c = 0; c < 3; ++c) vn[c] *= vsq;
+    }
+    for (int c = 0; c < 3; ++c) {
+        xs[c] = xv[c];
+        xn[c] = 2.0 * xv[c] - xov[c] + (vn[c] - vv[c]) * P.h;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ bool last;
+    if (p < D.n_tot) {
+        const int r = D.prep[p];
+        const RepParams &P = D.rp[r];
+        RepState &S = D.rs[r];
+        double xn[3], vn[3], xs[3], kx[3], kv[3], ks[3];
+        leapfrog(D, r, P, S, p, xn, vn, xs);
+        leapfrog(D, r, P, S, P.kh, kx, kv, ks);   // the recentring bead (adjust_box, fast.cpp:2632-2656)
+        double on[3];
+        for (int c = 0; c < 3; ++c) {
+            const double s = P.hL[c] - kx[c];
+            xn[c] += s;
+            on[c] = xs[c] + s;
+        }
+        if (p - P.p0 >= P.nbeads) {
+            for (int c = 0; c < 3; ++c) {
+                if (xn[c] < 0) { xn[c] += P.L[c]; on[c] += P.L[c]; }
+                if (xn[c] > P.L[c]) { xn[c] -= P.L[c]; on[c] -= P.L[c]; }
+            }
+        }
+        const double dx = xn[0] - on[0], dy = xn[1] - on[1], dz = xn[2] - on[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(d2));
+        // all reads of the old state of this step happen before any write: the
+        // new state goes to the second buffer set, swapped by the host
+        D.pos[D.n_tot + p] = make_double4(xn[0], xn[1], xn[2], 0.0);
+        D.old[D.n_tot + p] = make_double4(on[0], on[1], on[2], 0.0);
+        D.vel[D.n_tot + p] = make_double4(vn[0], vn[1], vn[2], 0.0);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(D.done_counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
+        RepState &S = D.rs[r];
+        const RepParams &P = D.rp[r];
+        const unsigned long long bits = atomicAdd(&S.maxd2, 0ull);
+        const double m = sqrt(__longlong_as_double((long long)bits));
+        S.progress += m + m;
+        S.need_rebuild = S.progress > P.skin;
+        if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+        S.maxd2 = 0;
+        S.klok += 1;
+        S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
+    }
+    if (threadIdx.x == 0) *D.done_counter = 0;
+}
+
+// Copies the second state buffer set into the first (keeps one buffer layout
+// for every other kernel; cheap: 3 x 32 B per particle).
+__global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.n_tot) return;
+    D.pos[p] = D.pos[D.n_tot + p];
+    D.old[p] = D.old[D.n_tot + p];
+    D.vel[p] = D.vel[D.n_tot + p];
+    D.f_mb[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+}  // namespace tis

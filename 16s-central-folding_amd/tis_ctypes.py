@@ -130,6 +130,9 @@ def load_library(path=LIB_PATH):
         lib.tis_set_uniform_stream.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_longlong]
         lib.tis_get_stream_cursor.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
         lib.tis_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        lib.tis_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        lib.tis_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.tis_philox4x32_10.restype = None
     return lib
 
 
@@ -204,3 +207,106 @@ def default_options(lib):
     o = Options()
     lib.tis_default_options(C.byref(o))
     return o
+
+
+class Engine:
+    """A tis_ctx over one or more replicas (SystemHandle objects)."""
+
+    def __init__(self, lib, handles, precision=PREC_FP64, rng=RNG_PHILOX, seed=1, noise_scale=1.0,
+                 use_graph=1, graph_steps=50, **overrides):
+        self.lib = lib
+        self.handles = list(handles)
+        arr = (TisSystem * len(self.handles))(*[h.struct for h in self.handles])
+        self._arr = arr
+        o = default_options(lib)
+        o.precision, o.rng, o.seed, o.noise_scale = precision, rng, seed, noise_scale
+        o.use_graph, o.graph_steps = use_graph, graph_steps
+        for k, v in overrides.items():
+            setattr(o, k, v)
+        self.opt = o
+        self.ctx = C.c_void_p()
+        check(lib, lib.tis_create(arr, len(self.handles), C.byref(o), C.byref(self.ctx)), "tis_create")
+
+    def close(self):
+        if self.ctx:
+            self.lib.tis_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n(self, rep=0):
+        return self.handles[rep].d["n_particles"]
+
+    def eval(self, mask, rep=0):
+        f = np.zeros(3 * self.n(rep))
+        e = Energies()
+        check(self.lib, self.lib.tis_eval_forces(self.ctx, rep, mask, f.ctypes.data_as(_dp), C.byref(e)),
+              "tis_eval_forces")
+        return f.reshape(-1, 3), e.as_dict()
+
+    def step(self, n):
+        check(self.lib, self.lib.tis_step(self.ctx, n), "tis_step")
+
+    def get_state(self, rep=0):
+        out = [np.zeros(3 * self.n(rep)) for _ in range(3)]
+        check(self.lib, self.lib.tis_get_state(self.ctx, rep, *[a.ctypes.data_as(_dp) for a in out]),
+              "tis_get_state")
+        return [a.reshape(-1, 3) for a in out]
+
+    def set_state(self, pos, old, vel, klok=1, rep=0):
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (pos, old, vel)]
+        check(self.lib, self.lib.tis_set_state(self.ctx, rep, *[x.ctypes.data_as(_dp) for x in a], klok),
+              "tis_set_state")
+
+    def energies(self, rep=0):
+        e = Energies()
+        check(self.lib, self.lib.tis_get_energies(self.ctx, rep, C.byref(e)), "tis_get_energies")
+        return e.as_dict()
+
+    def stack_energies(self, rep=0):
+        n = self.handles[rep].d["n_st3"]
+        out = np.zeros(max(n, 1))
+        check(self.lib, self.lib.tis_get_stack_energies(self.ctx, rep, out.ctypes.data_as(_dp), n),
+              "tis_get_stack_energies")
+        return out[:n]
+
+    def pairs(self, which, rep=0):
+        n = C.c_longlong(0)
+        check(self.lib, self.lib.tis_get_pairs(self.ctx, rep, which, None, 0, C.byref(n)), "tis_get_pairs")
+        out = np.zeros(2 * n.value, dtype=np.int32)
+        check(self.lib, self.lib.tis_get_pairs(self.ctx, rep, which, out.ctypes.data_as(_ip), n.value,
+                                               C.byref(n)), "tis_get_pairs")
+        return out.reshape(-1, 2)
+
+    def hb_status(self, rep=0):
+        n = self.handles[rep].d["n_hb"]
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        check(self.lib, self.lib.tis_get_hb_status(self.ctx, rep, out.ctypes.data_as(_ip), n), "tis_get_hb_status")
+        return out[:n]
+
+    def set_stream(self, raw, rep=0):
+        self._stream = np.ascontiguousarray(raw, dtype=np.int32)
+        check(self.lib, self.lib.tis_set_uniform_stream(
+            self.ctx, rep, self._stream.ctypes.data_as(C.POINTER(C.c_int32)), len(self._stream)),
+            "tis_set_uniform_stream")
+
+    def set_profiling(self, on):
+        check(self.lib, self.lib.tis_set_profiling(self.ctx, int(on)), "tis_set_profiling")
+
+    def kernel_times(self):
+        buf = C.create_string_buffer(8192)
+        check(self.lib, self.lib.tis_kernel_times(self.ctx, buf, len(buf)), "tis_kernel_times")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
+
+    def cursor(self, rep=0):
+        c = C.c_longlong(0)
+        check(self.lib, self.lib.tis_get_stream_cursor(self.ctx, rep, C.byref(c)), "tis_get_stream_cursor")
+        return c.value

@@ -1,0 +1,165 @@
+"""Langevin steps/s of the TIS-RNA + explicit-ion loop on one MI355X.
+
+Workload (the benchmark's batched configuration): 16 replicas of the 16SCD
+system, replica k in a box of 350 + 100 k / 15 A with c_Mg = 0.5 * 40^(k/15) mM
+MgCl2 and 50 mM KCl (3,308 .. 9,469 particles, 86,655 in all), T = 0.596
+kcal/mol (300 K), seeds 1..16, FP32 pair and multi-body forces with FP64
+positions, velocities and sums, counter-based Philox noise, one hipGraph per
+50 steps.  Replicas are independent (no exchange moves yet).
+
+Printed: one JSON line.  value = aggregate replica-steps per second of the
+batch.  Beside it: the plain FP64 one-replica-per-launch path on the 3,659
+particle 5 mM system (no graph, FP64 forces), the roofline of the dominant
+kernel from HIP-event timings, and the FP64 CPU oracle timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "16s-central-folding_amd"))
+
+import tis_ctypes as T  # noqa: E402
+
+METRIC = "Langevin steps/s, 16SCD+ions: batched FP32 replicas vs plain FP64 single"
+FS_PER_STEP = 2.444            # h = 0.05 tau, tau = 48.89 fs (A, kcal/mol, amu)
+HBM_PEAK_GBS = 8000.0
+
+
+def batched_systems(lib, n_rep=16, T_kcal=0.596):
+    out = []
+    for k in range(n_rep):
+        c_mg = 0.5e-3 * 40.0 ** (k / 15.0)
+        box = 350.0 + 100.0 * k / 15.0
+        d, _ = T.build_16scd(lib, T=T_kcal, c_mg=c_mg, c_k=0.050, seed=k + 1, box=box)
+        out.append(d)
+    return out
+
+
+def algorithmic_bytes(eng, handles, kernel, lr_every=10):
+    """Bytes per launch of one kernel over the whole batch, from the per-unit
+    figures: 8 B per SR pair, 8 B per LR pair every lr_every steps, 52 B per
+    stack, 16 B per bond, 20 B per angle (k_force); 56 B per live HB candidate
+    (k_hb); 148 B per particle (k_integrate: FP64 x, x_old, v read + write and
+    the type word)."""
+    tot = 0.0
+    for r, h in enumerate(handles):
+        e = eng.energies(r)
+        d = h.d
+        if kernel == "k_force":
+            tot += 8 * e["n_sr_pairs"] + 8.0 * e["n_lr_pairs"] / lr_every
+            tot += 52 * (d["n_st3"] + d["n_st2"]) + 16 * d["n_bonds"] + 20 * d["n_angles"]
+        elif kernel == "k_hb":
+            tot += 56 * e["n_hb_active"]
+        elif kernel in ("k_integrate", "k_commit"):
+            tot += 148 * d["n_particles"]
+    return tot
+
+
+def cpu_baseline(systems, steps_each):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_ctypes as O
+    total_steps, total_t = 0, 0.0
+    for d in systems:
+        h = T.SystemHandle(d)
+        o = O.Oracle(h)
+        o.set_state(d["pos"], d["pos_old"], d["vel"])
+        o.set_stream_glibc(1, (steps_each + 1) * 5000)
+        t0 = time.perf_counter()
+        rc = o.step(steps_each, 1.0)
+        total_t += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError("oracle step failed")
+        total_steps += steps_each
+    return total_steps / total_t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=250)
+    ap.add_argument("--profile-steps", type=int, default=300)
+    ap.add_argument("--plain-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=500, help="oracle steps per replica for cpu_baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X only: replicas are batched inside one launch on one GPU; "
+              "multi-GPU runs (replica sharding over RCCL) are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    lib = T.load_library()
+    systems = batched_systems(lib)
+    handles = [T.SystemHandle(d) for d in systems]
+    eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024, use_graph=1,
+                   graph_steps=50)
+    eng.step(args.warmup)
+    t0 = time.perf_counter()
+    eng.step(args.steps)          # returns after a stream synchronize
+    dt = time.perf_counter() - t0
+    n_rep = len(handles)
+    value = n_rep * args.steps / dt
+    # per-kernel device times over a separate event-timed run of the same batch
+    eng.set_profiling(True)
+    eng.step(args.profile_steps)
+    kt = eng.kernel_times()
+    eng.set_profiling(False)
+    dominant = max(kt, key=lambda k: kt[k][0] * kt[k][1])
+    n_launch, avg_ms = kt[dominant]
+    bytes_launch = algorithmic_bytes(eng, handles, dominant)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    energies = [eng.energies(r) for r in range(n_rep)]
+    eng.close()
+    # plain FP64 one-replica-per-launch path (5 mM, 350 A, 3,659 particles)
+    d1, _ = T.build_16scd(lib, T=0.596, c_mg=0.005, c_k=0.050, seed=1, box=350.0)
+    h1 = T.SystemHandle(d1)
+    e1 = T.Engine(lib, [h1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, use_graph=0)
+    e1.step(max(50, args.warmup // 5))
+    t0 = time.perf_counter()
+    e1.step(args.plain_steps)
+    plain = args.plain_steps / (time.perf_counter() - t0)
+    e1.close()
+    cpu = None
+    if not args.no_cpu:
+        sample = systems
+        v = cpu_baseline(sample, args.cpu_steps)
+        cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 CPU oracle (reference cell lists, glibc-stream noise), "
+                         f"{args.cpu_steps} steps of each of the {len(sample)} replicas, single thread"}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic: 16SCD chain from the reference PDB, ions placed by the reference's seeded rule",
+        "config": {"workload": "16 x 16SCD replicas, c_Mg = 0.5*40^(k/15) mM MgCl2 + 50 mM KCl, "
+                               "box 350+100k/15 A, T=0.596, independent (no exchange)",
+                   "replicas": n_rep, "particles": int(sum(d["n_particles"] for d in systems)),
+                   "precision": "mixed", "rng": "philox", "graph_steps": 50},
+        "ns_per_day_aggregate": value * FS_PER_STEP * 86400 * 1e-6,
+        "plain_fp64_single": {"value": plain, "unit": "steps/s", "particles": d1["n_particles"],
+                              "ns_per_day": plain * FS_PER_STEP * 86400 * 1e-6},
+        "batched_vs_plain": value / plain,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
+        "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
+        "hb_active": [e["n_hb_active"] for e in energies],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -202,9 +202,17 @@ int tis_get_hb_status(tis_ctx *ctx, int rep, int *status_out, int n);
 int tis_set_uniform_stream(tis_ctx *ctx, int rep, const int32_t *raw, long long n);
 int tis_get_stream_cursor(tis_ctx *ctx, int rep, long long *consumed);
 
-/* Average device time per launch of the named kernel over the last tis_step call
- * (HIP events on the context's stream); name NULL lists kernels in *buf. */
+/* Kernel timing.  While profiling is on, tis_step runs without graph capture and
+ * brackets every kernel launch with HIP events on the stream it is launched on.
+ * tis_kernel_times writes one line "name launches avg_ms" per kernel, accumulated
+ * since profiling was switched on.  (No reference counterpart: the reference has
+ * no live timing, only a commented-out ftime block at fast.cpp:6520-6541.) */
+int tis_set_profiling(tis_ctx *ctx, int on);
 int tis_kernel_times(tis_ctx *ctx, char *buf, int buflen);
+
+/* Philox4x32-10 on the host, the generator the engine uses on the device
+ * (known-answer tests). */
+void tis_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 #ifdef __cplusplus
 }

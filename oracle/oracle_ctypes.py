@@ -34,6 +34,7 @@ def load(path=LIB_PATH):
     lib.oracle_set_state.argtypes = [vp, dp, dp, dp, C.c_longlong, C.c_int]
     lib.oracle_get_state.argtypes = [vp, dp, dp, dp]
     lib.oracle_set_stream.argtypes = [vp, C.POINTER(C.c_int32), C.c_longlong]
+    lib.oracle_set_stream_glibc.argtypes = [vp, C.c_uint, C.c_longlong]
     lib.oracle_stream_cursor.argtypes = [vp]
     lib.oracle_stream_cursor.restype = C.c_longlong
     lib.oracle_eval.argtypes = [vp, C.c_uint, dp, dp]
@@ -93,6 +94,9 @@ class Oracle:
         self._stream = np.ascontiguousarray(raw, dtype=np.int32)
         self.lib.oracle_set_stream(self.ctx, self._stream.ctypes.data_as(C.POINTER(C.c_int32)),
                                    len(self._stream))
+
+    def set_stream_glibc(self, seed, n):
+        assert self.lib.oracle_set_stream_glibc(self.ctx, seed, n) == 0
 
     def cursor(self):
         return self.lib.oracle_stream_cursor(self.ctx)

@@ -705,6 +705,16 @@ int oracle_set_stream(oracle_ctx *o, const int32_t *raw, long long n) {
 
 long long oracle_stream_cursor(const oracle_ctx *o) { return o ? o->cursor : -1; }
 
+int oracle_set_stream_glibc(oracle_ctx *o, unsigned seed, long long n) {
+    if (!o || n < 0) return TIS_EINVAL;
+    o->stream.resize(n);
+    std::srand(seed);
+    for (long long q = 0; q < n; ++q) o->stream[q] = std::rand();
+    o->cursor = 0;
+    o->stream_overflow = false;
+    return TIS_OK;
+}
+
 int oracle_eval(oracle_ctx *o, unsigned mask, double *f_out, double *e_out) {
     if (!o) return TIS_EINVAL;
     std::vector<double> F(3 * (size_t)o->N, 0.0);
