@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/tis_gpu.h"
+#include "glibc_rand.h"
 #include "host_util.h"
 
 namespace tis_host {
@@ -47,4 +48,12 @@ extern "C" void tis_system_free(tis_system *s) {
                     s->hb_site_off, s->hb_site_idx, s->site_hb_off, s->site_hb_idx, s->site_valence};
     for (void *p : ptrs) std::free(p);
     std::free(s);
+}
+
+extern "C" int tis_glibc_rand_fill(unsigned seed, long long skip, int32_t *out, long long n) {
+    if ((n && !out) || n < 0 || skip < 0) { tis_host::set_error("tis_glibc_rand_fill: bad arguments"); return TIS_EINVAL; }
+    tis_host::GlibcRand g(seed);
+    for (long long q = 0; q < skip; ++q) g.next();
+    for (long long q = 0; q < n; ++q) out[q] = g.next();
+    return TIS_OK;
 }

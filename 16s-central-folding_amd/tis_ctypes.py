@@ -112,6 +112,7 @@ def load_library(path=LIB_PATH):
                                     C.POINTER(C.c_longlong)]
     lib.tis_system_free.argtypes = [C.POINTER(TisSystem)]
     lib.tis_system_free.restype = None
+    lib.tis_glibc_rand_fill.argtypes = [C.c_uint, C.c_longlong, C.POINTER(C.c_int32), C.c_longlong]
     if hasattr(lib, "tis_create"):
         lib.tis_default_options.argtypes = [C.POINTER(Options)]
         lib.tis_default_options.restype = None
@@ -201,6 +202,13 @@ def build_16scd(lib, T=0.615, c_mg=0.005, c_k=0.050, seed=1, box=350.0,
     finally:
         lib.tis_system_free(out)
     return d, draws.value
+
+
+def glibc_stream(lib, seed, n, skip=0):
+    out = np.zeros(n, dtype=np.int32)
+    check(lib, lib.tis_glibc_rand_fill(seed, skip, out.ctypes.data_as(C.POINTER(C.c_int32)), n),
+          "tis_glibc_rand_fill")
+    return out
 
 
 def default_options(lib):

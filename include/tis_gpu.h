@@ -128,6 +128,11 @@ int tis_build_16scd(const char *data_dir, const tis_build_args *args,
                     tis_system **out, long long *rand_draws_out);
 void tis_system_free(tis_system *sys);
 
+/* The glibc rand() stream (TYPE_3 additive generator) after srand(seed), values
+ * skip .. skip+n-1.  The reference draws every random number from it; the host
+ * uses it for set-up and to feed TIS_RNG_STREAM contexts. */
+int tis_glibc_rand_fill(unsigned seed, long long skip, int32_t *out, long long n);
+
 /* ------------------------------------------------------------ the engine */
 typedef struct tis_ctx tis_ctx;
 
