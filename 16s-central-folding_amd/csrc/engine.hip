@@ -559,6 +559,18 @@ extern "C" int tis_get_state(tis_ctx *c, int rep, double *pos, double *pos_old, 
     });
 }
 
+extern "C" int tis_set_step_counter(tis_ctx *c, int rep, long long klok) {
+    if (!c || rep < 0 || rep >= c->n_rep || klok < 1) return fail(Fail{TIS_EINVAL, "tis_set_step_counter: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        RepState s = c->get_rs(rep);
+        s.klok = klok;
+        s.lr_due = (klok % c->rp[rep].lr_every) == 1 || s.progress == 0;
+        c->put_rs(rep, s);
+        return TIS_OK;
+    });
+}
+
 extern "C" int tis_step(tis_ctx *c, long long n_steps) {
     if (!c || n_steps < 0) return fail(Fail{TIS_EINVAL, "tis_step: bad arguments"});
     return guarded([&]() -> int {

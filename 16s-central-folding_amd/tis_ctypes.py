@@ -121,6 +121,7 @@ def load_library(path=LIB_PATH):
         lib.tis_destroy.restype = None
         lib.tis_set_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_longlong]
         lib.tis_get_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        lib.tis_set_step_counter.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
         lib.tis_step.argtypes = [C.c_void_p, C.c_longlong]
         lib.tis_get_energies.argtypes = [C.c_void_p, C.c_int, C.POINTER(Energies)]
         lib.tis_get_stack_energies.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]

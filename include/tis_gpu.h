@@ -169,6 +169,9 @@ void tis_destroy(tis_ctx *ctx);
 int tis_set_state(tis_ctx *ctx, int rep, const double *pos, const double *pos_old,
                   const double *vel, long long klok);
 int tis_get_state(tis_ctx *ctx, int rep, double *pos, double *pos_old, double *vel);
+/* Sets the step counter klok only (lists, PROGRESS and the held long-range force
+ * continue), as the second deck of the _DD driver does (DD.cpp:6428). */
+int tis_set_step_counter(tis_ctx *ctx, int rep, long long klok);
 
 /* Advances every replica n_steps Langevin steps: the body of fast.cpp:6274-6387
  * without its output block (deterministic_forces_MPI, full_forces,
