@@ -56,8 +56,8 @@ void upload(T *dst, const std::vector<T> &src) {
 struct tis_ctx {
     int n_rep = 0;
     tis_options opt{};
-    hipStream_t s_main = nullptr, s_hb = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t s_main = nullptr, s_hb = nullptr, s_mb = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_mb = nullptr;
     std::vector<void *> owned;
     Dev D{};
     std::vector<RepParams> rp;
@@ -88,8 +88,10 @@ struct tis_ctx {
         for (void *p : owned) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_join_mb) (void)hipEventDestroy(ev_join_mb);
         if (s_main) (void)hipStreamDestroy(s_main);
         if (s_hb) (void)hipStreamDestroy(s_hb);
+        if (s_mb) (void)hipStreamDestroy(s_mb);
     }
 
     int grid(int n) const { return (n + kBlock - 1) / kBlock; }
@@ -128,7 +130,7 @@ struct tis_ctx {
 
     void launch_rebuild() {
         timed("k_bin", s_main, [&] { k_bin<<<n_rep, 1024, 0, s_main>>>(D); });
-        timed("k_list", s_main, [&] { k_list<<<grid(D.n_tot), kBlock, 0, s_main>>>(D, 1); });
+        timed("k_list", s_main, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, s_main>>>(D, 1); });
         timed("k_hbrebuild", s_main, [&] { k_hbrebuild<<<n_rep, 1024, max_site_lds(), s_main>>>(D); });
     }
     size_t max_site_lds() const {
@@ -137,22 +139,28 @@ struct tis_ctx {
         return (size_t)m * sizeof(int);
     }
 
-    // One Langevin step.  HB resolution runs on its own stream, concurrently
-    // with stacking, backbone and the pair gather; the integrator joins both.
+    // One Langevin step.  The HB resolution and the multi-body terms run on
+    // their own streams, concurrently with the pair gather; the integrator joins all.
     void launch_step() {
         HIP_TRY(hipEventRecord(ev_fork, s_main));
         HIP_TRY(hipStreamWaitEvent(s_hb, ev_fork, 0));
+        HIP_TRY(hipStreamWaitEvent(s_mb, ev_fork, 0));
         timed("k_hb", s_hb, [&] {
             if (D.precision == TIS_PREC_FP64) k_hb<double><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
             else k_hb<float><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
         });
         HIP_TRY(hipEventRecord(ev_join, s_hb));
-        const int nblk = n_rep + nb_bonded + grid(D.n_tot);
-        timed("k_force", s_main, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double><<<nblk, kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
-            else k_force<float><<<nblk, kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+        timed("k_multibody", s_mb, [&] {
+            if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
+            else k_force<float, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
+        });
+        HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
+        timed("k_pairs", s_main, [&] {
+            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+            else k_force<float, 1><<<grid(D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
+        HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         launch_rebuild();
@@ -277,6 +285,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         HIP_TRY(hipGetDevice(&c->dev_id));
         HIP_TRY(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&c->s_hb, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&c->s_mb, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_join_mb, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         HIP_TRY(hipFuncSetAttribute((const void *)k_hb<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
@@ -465,7 +475,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         upload(D.rs, rs);
         HIP_TRY(hipMemcpy(D.rp, c->rp.data(), n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
         k_bin<<<n_rep, 1024, 0, c->s_main>>>(D);
-        k_list<<<c->grid(N), kBlock, 0, c->s_main>>>(D, 0);
+        k_list<<<kListWaves * 64 / kBlock, kBlock, 0, c->s_main>>>(D, 0);
         HIP_TRY(hipStreamSynchronize(c->s_main));
         long long pool_total = 0;
         for (int r = 0; r < n_rep; ++r) {
@@ -582,6 +592,7 @@ extern "C" int tis_step(tis_ctx *c, long long n_steps) {
                 for (long long q = 0; q < m; ++q) c->launch_step();
                 HIP_TRY(hipStreamSynchronize(c->s_main));
                 HIP_TRY(hipStreamSynchronize(c->s_hb));
+                HIP_TRY(hipStreamSynchronize(c->s_mb));
                 c->collect_times();
                 done += m;
             }
@@ -677,10 +688,15 @@ extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out
             if (D.precision == TIS_PREC_FP64) k_hb<double><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
             else k_hb<float><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
         }
-        const int nblk = c->n_rep + c->nb_bonded + c->grid(D.n_tot);
         const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
-        if (D.precision == TIS_PREC_FP64) k_force<double><<<nblk, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
-        else k_force<float><<<nblk, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(D.n_tot);
+        if (D.precision == TIS_PREC_FP64) {
+            k_force<double, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+            k_force<double, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+        } else {
+            k_force<float, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+            k_force<float, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(c->s_main));
         std::vector<double4> a(P.n), b(P.n), d(P.n);

@@ -143,15 +143,22 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
 }
 
 // ------------------------------------------------------------ k_list
-// Visits every particle of the 27 (fewer when a dimension has < 3 cells)
-// cells around particle p's cell.
+// One wavefront per particle: the lanes test 64 candidates of a cell at a time,
+// so the dependent index/position loads of many candidates are in flight
+// together (a thread-per-particle loop is latency-bound on the dense RNA cells).
+// The grid is fixed (kListWaves wavefronts) and strides over the particles of
+// the replicas whose displacement trigger fired; the others cost one flag read.
+// mode 0: count only (sizes the pool), mode 1: build.
+constexpr int kListWaves = 4096;
+
 template <class F>
-TIS_DEV void for_neighbour_cells(const Dev &D, const RepParams &P, int cell, F &&visit) {
+TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
     const int lo0 = M0 >= 3 ? -1 : 0, hi0 = M0 >= 2 ? 1 : 0;
     const int lo1 = M1 >= 3 ? -1 : 0, hi1 = M1 >= 2 ? 1 : 0;
     const int lo2 = M2 >= 3 ? -1 : 0, hi2 = M2 >= 2 ? 1 : 0;
+    const int lane = threadIdx.x & 63;
     for (int a = lo0; a <= hi0; ++a) {
         int n0 = c0 + a; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0;
         for (int b = lo1; b <= hi1; ++b) {
@@ -160,49 +167,74 @@ TIS_DEV void for_neighbour_cells(const Dev &D, const RepParams &P, int cell, F &
                 int n2 = c2 + c; n2 += n2 < 0 ? M2 : 0; n2 -= n2 >= M2 ? M2 : 0;
                 const int nc = (n0 * M1 + n1) * M2 + n2;
                 const int s = D.cell_start[P.cell0 + nc], e = D.cell_start[P.cell0 + nc + 1];
-                for (int q = s; q < e; ++q) visit(D.sorted[P.p0 + q]);
+                for (int q0 = s; q0 < e; q0 += 64) {
+                    const int q = q0 + lane;
+                    visit(q < e ? D.sorted[P.p0 + q] : -1);
+                }
             }
         }
     }
 }
 
-// mode 0: count only (sizes the pool), mode 1: build.
 __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= D.n_tot) return;
-    const int r = D.prep[t];   // sorted slot t lies in replica r's particle range
-    RepState &S = D.rs[r];
-    if (!S.need_rebuild) return;
-    const RepParams &P = D.rp[r];
-    const int p = D.sorted[t];
-    const double4 xi = D.pos[p];
-    const bool chg_i = (D.ptype[p] >> 8) & 1;
-    const int cell = D.cell_of[p];
-    int nsr = 0, nlr = 0;
-    for_neighbour_cells(D, P, cell, [&](int j) {
-        if (j == p) return;
-        double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
-        if (r2 <= P.rsr2) ++nsr;
-        else if (chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1)) ++nlr;
-    });
-    const long long base = atomicAdd((unsigned long long *)&S.pool_cursor, (unsigned long long)(nsr + nlr));
-    if (mode == 0) return;
-    if (base + nsr + nlr > P.pool_cap) {
-        atomicOr(&S.overflow, 1);
-        D.nb_start[p] = 0; D.nb_nsr[p] = 0; D.nb_nlr[p] = 0;
-        return;
+    const int lane = threadIdx.x & 63;
+    const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int r = 0; r < D.n_rep; ++r) {
+        RepState &S = D.rs[r];
+        if (!S.need_rebuild) continue;
+        const RepParams &P = D.rp[r];
+        for (int t = w0; t < P.n; t += nw) {
+            const int p = D.sorted[P.p0 + t];
+            const double4 xi = D.pos[p];
+            const bool chg_i = (D.ptype[p] >> 8) & 1;
+            const int cell = D.cell_of[p];
+            int nsr = 0, nlr = 0;
+            for_neighbour_cells_wave(D, P, cell, [&](int j) {
+                bool isr = false, ilr = false;
+                if (j >= 0 && j != p) {
+                    const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
+                    isr = r2 <= P.rsr2;
+                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1);
+                }
+                nsr += __popcll(__ballot(isr));
+                nlr += __popcll(__ballot(ilr));
+            });
+            long long base = 0;
+            if (lane == 0)
+                base = (long long)atomicAdd((unsigned long long *)&S.pool_cursor, (unsigned long long)(nsr + nlr));
+            base = __shfl(base, 0, 64);
+            if (mode == 0) continue;
+            if (base + nsr + nlr > P.pool_cap) {
+                if (lane == 0) {
+                    atomicOr(&S.overflow, 1);
+                    D.nb_start[p] = 0; D.nb_nsr[p] = 0; D.nb_nlr[p] = 0;
+                }
+                continue;
+            }
+            int *out = D.pool + P.pool0 + base;
+            int a = 0, b = nsr;
+            for_neighbour_cells_wave(D, P, cell, [&](int j) {
+                bool isr = false, ilr = false;
+                if (j >= 0 && j != p) {
+                    const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
+                    isr = r2 <= P.rsr2;
+                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1);
+                }
+                const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                if (isr) out[a + __popcll(ms & below)] = j;
+                if (ilr) out[b + __popcll(ml & below)] = j;
+                a += __popcll(ms);
+                b += __popcll(ml);
+            });
+            if (lane == 0) {
+                D.nb_start[p] = (int)base;
+                D.nb_nsr[p] = nsr;
+                D.nb_nlr[p] = nlr;
+            }
+        }
     }
-    int *out = D.pool + P.pool0 + base;
-    int a = 0, b = nsr;
-    for_neighbour_cells(D, P, cell, [&](int j) {
-        if (j == p) return;
-        double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
-        if (r2 <= P.rsr2) out[a++] = j;
-        else if (chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1)) out[b++] = j;
-    });
-    D.nb_start[p] = (int)base;
-    D.nb_nsr[p] = nsr;
-    D.nb_nlr[p] = nlr;
 }
 
 // ------------------------------------------------------------ k_hbrebuild
@@ -289,21 +321,6 @@ __global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
         S.n_ainc = n_inc;
         if (n_act > kActMax || n_as > kASiteMax || n_inc > kAIncMax || n_ai > kAIncMax) atomicOr(&S.overflow, 2);
     }
-}
-
-// ------------------------------------------------------------ uniform draws
-// Draw number idx (0-based within the current step) of replica r:
-// Philox keyed by (seed, replica) with the step and index in the counter, or
-// the caller-fed glibc stream at the replica's cursor.
-TIS_DEV double hb_uniform(const Dev &D, int r, const RepState &S, long long idx, bool *overflow) {
-    if (D.rng == 0) {
-        Philox4 o = philox4x32_10((uint32_t)S.klok, (uint32_t)(S.klok >> 32), (uint32_t)idx, kTagHb,
-                                  (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), (uint32_t)(D.seed >> 32));
-        return u01_closed_open(o.v[0]);
-    }
-    long long c = S.stream_cursor + idx;
-    if (c >= D.stream_len[r]) { *overflow = true; return 0.5; }
-    return (double)D.stream[D.stream_base[r] + c] / kRandMax;
 }
 
 TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fast.cpp:2257)
@@ -396,6 +413,221 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
     }
 }
 
+// Uniform source of one step's HB resolution, with the replica's counters
+// hoisted into registers: Philox draw idx, or the caller-fed stream at its cursor.
+struct HbDraws {
+    int rng;
+    uint32_t c0, c1, k0, k1;
+    const int32_t *stream;
+    long long cur, lim;
+    TIS_DEV HbDraws(const Dev &D, int r, const RepState &S) {
+        rng = D.rng;
+        c0 = (uint32_t)S.klok;
+        c1 = (uint32_t)(S.klok >> 32);
+        k0 = (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1));
+        k1 = (uint32_t)(D.seed >> 32);
+        stream = rng ? D.stream + D.stream_base[r] : nullptr;
+        cur = S.stream_cursor;
+        lim = rng ? D.stream_len[r] : 0;
+    }
+    TIS_DEV double at(long long idx, bool *ovf) const {
+        if (rng == 0) return u01_closed_open(philox4x32_10(c0, c1, (uint32_t)idx, kTagHb, k0, k1).v[0]);
+        const long long c = cur + idx;
+        if (c >= lim) { *ovf = true; return 0.5; }
+        return (double)stream[c] / kRandMax;
+    }
+};
+
+TIS_DEV int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+TIS_DEV double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Valence resolution with the whole working set in registers of one
+// wavefront: slot (row, lane) holds one over-subscribed site in site order, its
+// live candidates (static HB_ATOM order, multiplicity kept) and its excess.
+// One pass: pick the j-th over-subscribed site, shuffle its live candidates,
+// walk them keeping k and moving to j with probability exp((E_j - E_k)/T), switch
+// the pick off everywhere it is listed.  Returns false (nothing changed) when a
+// site has more than KMAX live candidates or there are more than 64*ROWS sites.
+template <int ROWS, int KMAX>
+TIS_DEV bool resolve_in_registers(int nA0, const short *A0, const int *excess, const short *aoff,
+                                  const short *alist, unsigned char *flag, const double *E, double T,
+                                  const HbDraws &dr, int *iters_out, long long *base_out, bool *ovf) {
+    static_assert(KMAX * KMAX <= 64 && 2 * KMAX <= 64, "pairwise table must fit one wavefront");
+    const int lane = threadIdx.x & 63;
+    if (nA0 > 64 * ROWS) return false;
+    int bnd[ROWS][KMAX];
+    int exc[ROWS];
+    unsigned long long act[ROWS];
+    bool too_long = false;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int idx = r * 64 + lane;
+        exc[r] = 0;
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) bnd[r][s] = -1;
+        if (idx < nA0) {
+            const int site = A0[idx];
+            exc[r] = excess[site];
+            int k = 0;
+            for (int q = aoff[site]; q < aoff[site + 1]; ++q) {
+                const int a = alist[q];
+                if (flag[a] != FLAG_LIVE) continue;
+#pragma unroll
+                for (int s = 0; s < KMAX; ++s)
+                    if (s == k) bnd[r][s] = a;
+                ++k;
+            }
+            too_long |= k > KMAX;
+        }
+        act[r] = __ballot(idx < nA0 && exc[r] > 0);
+    }
+    if (__ballot(too_long)) return false;
+    int nA = 0;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) nA += __popcll(act[r]);
+    long long base = 0;
+    int iters = 0;
+    while (nA > 0) {
+        ++iters;
+        const double u = lane < 2 * KMAX ? dr.at(base + lane, ovf) : 0.0;
+        const double u0 = readlane_d(u, 0);
+        const int j = ceil_index(u0, nA) - 1;
+        // slot of the j-th over-subscribed site
+        int row = 0, jr = j, cum = 0;
+        unsigned long long m = 0;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int c = __popcll(act[r]);
+            if (j >= cum && j < cum + c) { row = r; jr = j - cum; m = act[r]; }
+            cum += c;
+        }
+        const bool here = ((m >> lane) & 1ull) && __popcll(m & ((1ull << lane) - 1ull)) == jr;
+        const int ls = __ffsll((long long)__ballot(here)) - 1;
+        // its live candidates, compacted into lanes 0..nK-1 in list order
+        int v = -1, nK = 0;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            if (r != row) continue;
+#pragma unroll
+            for (int s = 0; s < KMAX; ++s) {
+                const int b = readlane_i(bnd[r][s], ls);
+                if (b >= 0) {
+                    if (lane == nK) v = b;
+                    ++nK;
+                }
+            }
+        }
+        if (nK == 0) { *ovf = true; break; }
+        // shuffle: swap i with ceil(u nK) - 1, i = 0..nK-1 (draws 1..nK)
+        for (int i = 0; i < nK; ++i) {
+            const int jj = ceil_index(readlane_d(u, 1 + i), nK) - 1;
+            const int bi = readlane_i(v, i), bj = readlane_i(v, jj);
+            if (lane == i) v = bj;
+            else if (lane == jj) v = bi;
+        }
+        // all pairwise acceptance probabilities at once: lane i*KMAX + c holds
+        // exp((E_i - E_c) / T), the reference's expression for candidate i against
+        // the current pick c
+        const double Ev = lane < nK ? E[v] : 0.0;
+        const double Ei = __shfl(Ev, lane / KMAX, 64), Ec = __shfl(Ev, lane % KMAX, 64);
+        const double Pv = (lane / KMAX < nK && lane % KMAX < nK) ? exp((Ei - Ec) / T) : 0.0;
+        int k = 0;
+        for (int i = 1; i < nK; ++i) {
+            const double ui = readlane_d(u, nK + i);   // selection draws follow the shuffle draws
+            if (ui < readlane_d(Pv, i * KMAX + k)) k = i;
+        }
+        const int off = readlane_i(v, k);
+        if (lane == 0) flag[off] = FLAG_OFF;
+        nA = 0;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+            for (int s = 0; s < KMAX; ++s)
+                if (bnd[r][s] == off) { bnd[r][s] = -1; --exc[r]; }
+            act[r] &= __ballot(exc[r] > 0);
+            nA += __popcll(act[r]);
+        }
+        base += 2 * nK;
+    }
+    *iters_out = iters;
+    *base_out = base;
+    return true;
+}
+
+// The same resolution with its working set in LDS (any number of sites and
+// up to 64 live candidates per site); used when the register form does not fit.
+TIS_DEV void resolve_in_lds(int nA, short *A, short *B, int *excess, const short *aoff, const short *alist,
+                            const short *hoff, const short *hinc, unsigned char *flag, const double *E, double T,
+                            const HbDraws &dr, int *iters_out, long long *base_out, bool *ovf) {
+    const int lane = threadIdx.x & 63;
+    long long base = 0;
+    int iters = 0;
+    while (nA > 0) {
+        __builtin_amdgcn_wave_barrier();
+        if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
+        const double u_lo = dr.at(base + lane, ovf);
+        const double u_hi = dr.at(base + 64 + lane, ovf);
+        const double u0 = readlane_d(u_lo, 0);
+        const int site = A[ceil_index(u0, nA) - 1];
+        const int o0 = aoff[site], len = aoff[site + 1] - o0;
+        if (len > 64) { *ovf = true; break; }
+        int bond = -1;
+        bool keep = false;
+        if (lane < len) { bond = alist[o0 + lane]; keep = flag[bond] == FLAG_LIVE; }
+        const unsigned long long km = __ballot(keep);
+        const int nK = __popcll(km);
+        if (nK == 0) { *ovf = true; break; }
+        // forward permute: a kept lane sends its bond to lane pos, the others to
+        // lane 63, which is never read (nK == 64 only if every lane keeps)
+        const int pos = __popcll(km & ((1ull << lane) - 1ull));
+        int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
+        for (int i = 0; i < nK; ++i) {
+            const int di = 1 + i;
+            const double ui = di < 64 ? readlane_d(u_lo, di) : readlane_d(u_hi, di - 64);
+            const int jj = ceil_index(ui, nK) - 1;
+            const int bi = readlane_i(bv, i), bj = readlane_i(bv, jj);
+            if (lane == i) bv = bj;
+            else if (lane == jj) bv = bi;
+        }
+        const double Ev = lane < nK ? E[bv] : 0.0;
+        int kk = 0;
+        double Ek = readlane_d(Ev, 0);
+        for (int i = 1; i < nK; ++i) {
+            const int di = nK + i;
+            const double ui = di < 64 ? readlane_d(u_lo, di) : readlane_d(u_hi, di - 64);
+            const double Ei = readlane_d(Ev, i);
+            const double pr = exp((Ei - Ek) / T);
+            if (ui < pr) { kk = i; Ek = Ei; }
+        }
+        const int off_bond = readlane_i(bv, kk);
+        base += 2 * nK;
+        if (lane == 0) flag[off_bond] = FLAG_OFF;
+        const int h0 = hoff[off_bond], hn = hoff[off_bond + 1] - h0;
+        for (int q = lane; q < hn; q += 64) atomicSub(&excess[hinc[h0 + q]], 1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        int nB = 0;
+        for (int s0 = 0; s0 < nA; s0 += 64) {
+            const int e = s0 + lane;
+            const int sv = e < nA ? A[e] : 0;
+            const bool kp = e < nA && excess[sv] > 0;
+            const unsigned long long m = __ballot(kp);
+            if (kp) B[nB + __popcll(m & ((1ull << lane) - 1ull))] = (short)sv;
+            nB += __popcll(m);
+        }
+        short *t = A; A = B; B = t;
+        nA = nB;
+    }
+    *iters_out = iters;
+    *base_out = base;
+}
+
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     // mode bit 0: step (advance the stream cursor, build the noise table); bit 1: forces on
@@ -461,72 +693,12 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
             if (keep) A0[nA + __popcll(m & ((1ull << lane) - 1ull))] = (short)s;
             nA += __popcll(m);
         }
-        short *A = A0, *B = A1;
-        long long base = 0;
+        HbDraws dr(D, r, S);
         int iters = 0;
+        long long base = 0;
         bool ovf = false;
-        const double T = P.T;
-        while (nA > 0) {
-            __builtin_amdgcn_wave_barrier();
-            if (++iters > 1000000) { ovf = true; break; }   // each pass removes >= 1 excess unit
-            // draws for this iteration: lane l holds draws base+l and base+64+l
-            const double u_lo = hb_uniform(D, r, S, base + lane, &ovf);
-            const double u_hi = hb_uniform(D, r, S, base + 64 + lane, &ovf);
-            const double u0 = __shfl(u_lo, 0, 64);
-            const int site = A[ceil_index(u0, nA) - 1];
-            const int o0 = aoff[site], len = aoff[site + 1] - o0;
-            if (len > 64) { ovf = true; break; }
-            int bond = -1;
-            bool keep = false;
-            if (lane < len) { bond = alist[o0 + lane]; keep = flag[bond] == FLAG_LIVE; }
-            const unsigned long long km = __ballot(keep);
-            const int nK = __popcll(km);
-            if (nK == 0) { ovf = true; break; }
-            // bonds[i] held by lane i (order of the static site list)
-            const int pos = __popcll(km & ((1ull << lane) - 1ull));
-            // forward permute: kept lane sends its bond to lane pos; the others to
-            // lane 63, which is never read (nK <= len <= 64, and nK == 64 only if
-            // every lane keeps)
-            int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
-            // shuffle: swap i with ceil(u nK) - 1 for i = 0..nK-1 (draws 1..nK)
-            for (int i = 0; i < nK; ++i) {
-                const int di = 1 + i;
-                const double ui = di < 64 ? __shfl(u_lo, di, 64) : __shfl(u_hi, di - 64, 64);
-                const int jj = ceil_index(ui, nK) - 1;
-                const int bi = __shfl(bv, i, 64), bj = __shfl(bv, jj, 64);
-                if (lane == i) bv = bj;
-                else if (lane == jj) bv = bi;
-            }
-            const double Ev = lane < nK ? E[bv] : 0.0;
-            int kk = 0;
-            double Ek = __shfl(Ev, 0, 64);
-            for (int i = 1; i < nK; ++i) {
-                const int di = nK + i;   // selection draws follow the shuffle draws
-                const double ui = di < 64 ? __shfl(u_lo, di, 64) : __shfl(u_hi, di - 64, 64);
-                const double Ei = __shfl(Ev, i, 64);
-                const double pr = exp((Ei - Ek) / T);
-                if (ui < pr) { kk = i; Ek = Ei; }
-            }
-            const int off_bond = __shfl(bv, kk, 64);
-            base += 2 * nK;
-            if (lane == 0) flag[off_bond] = FLAG_OFF;
-            const int h0 = hoff[off_bond], hn = hoff[off_bond + 1] - h0;
-            for (int q = lane; q < hn; q += 64) atomicSub(&excess[hinc[h0 + q]], 1);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            int nB = 0;
-            for (int s0 = 0; s0 < nA; s0 += 64) {
-                const int e = s0 + lane;
-                const int sv = e < nA ? A[e] : 0;
-                const bool kp = e < nA && excess[sv] > 0;
-                const unsigned long long m = __ballot(kp);
-                if (kp) B[nB + __popcll(m & ((1ull << lane) - 1ull))] = (short)sv;
-                nB += __popcll(m);
-            }
-            short *t = A; A = B; B = t;
-            nA = nB;
-        }
+        if (!resolve_in_registers<8, 8>(nA, A0, excess, aoff, alist, flag, E, P.T, dr, &iters, &base, &ovf))
+            resolve_in_lds(nA, A0, A1, excess, aoff, alist, hoff, hinc, flag, E, P.T, dr, &iters, &base, &ovf);
         if (lane == 0) {
             sh_iters = iters;
             sh_draws = base;
@@ -557,13 +729,35 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
         if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);
     }
     // ---- reference-stream noise table: 1500 Box-Muller pairs after the HB draws
+    // (full_forces, fast.cpp:2598-2611).  r1 == 0 is redrawn in the reference; when
+    // no r1 slot of this step's block holds a 0 (all but ~1e-6 of steps) the pairs
+    // are independent and are formed in parallel, otherwise one thread walks them.
     if ((mode & 1) && D.rng == 1) {
+        __shared__ int sh_flags;
+        if (threadIdx.x == 0) sh_flags = 0;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            long long c = S.stream_cursor + sh_draws;
-            const long long lim = D.stream_len[r];
-            const int32_t *st = D.stream + D.stream_base[r];
-            double *mw = D.maxwell + 3000 * (size_t)r;
+        const long long c0 = S.stream_cursor + sh_draws;
+        const long long lim = D.stream_len[r];
+        const int32_t *st = D.stream + D.stream_base[r];
+        double *mw = D.maxwell + 3000 * (size_t)r;
+        for (int q = threadIdx.x; q < 1500; q += blockDim.x) {
+            const long long c = c0 + 2 * q;
+            if (c + 1 >= lim) atomicOr(&sh_flags, 2);
+            else if (st[c] == 0) atomicOr(&sh_flags, 1);
+        }
+        __syncthreads();
+        if (sh_flags == 0) {
+            for (int q = threadIdx.x; q < 1500; q += blockDim.x) {
+                const long long c = c0 + 2 * q;
+                const double r1 = (double)st[c] / kRandMax, r2 = (double)st[c + 1] / kRandMax;
+                const double a = sqrt(-2.0 * log(r1)), b = 2.0 * kPi * r2;
+                mw[2 * q] = a * cos(b);
+                mw[2 * q + 1] = a * sin(b);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) S.stream_cursor = c0 + 3000;
+        } else if (threadIdx.x == 0) {
+            long long c = c0;
             bool bad = false;
             for (int i = 0; i < 3000; i += 2) {
                 double r1;
@@ -586,11 +780,14 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
 }
 
 // ------------------------------------------------------------ k_force
-// Blocks [0, n_rep): stacking of replica b.  Then backbone terms, then the pair gather.
-template <class R>
+// ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
+// backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
+// that the register-heavy stacking code does not lower the occupancy of the
+// latency-bound gather; they run concurrently on two streams.
+template <class R, int ROLE>
 __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned mask, int eval) {
     double4 *F = D.f_mb;
-    if ((int)blockIdx.x < D.n_rep) {
+    if (ROLE == 0 && (int)blockIdx.x < D.n_rep) {
         if (!(mask & 1)) return;
         const int r = blockIdx.x;
         const RepParams &P = D.rp[r];
@@ -677,8 +874,9 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         if (threadIdx.x == 0) { S.e[EI_E2_ST] = s2; S.e[EI_E3_ST] = s3; }
         return;
     }
-    int blk = blockIdx.x - D.n_rep;
-    if (blk < nb_bonded) {
+    int blk = ROLE == 0 ? (int)blockIdx.x - D.n_rep : (int)blockIdx.x;
+    if (ROLE == 0) {
+        if (blk >= nb_bonded) return;
         if (!(mask & 4)) return;
         const int g = blk * blockDim.x + threadIdx.x;
         const int nbond = D.rp[D.n_rep - 1].bond0 + D.rp[D.n_rep - 1].nbond;
@@ -732,7 +930,6 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         return;
     }
-    blk -= nb_bonded;
     // ---- pair gather, thread per particle in cell order
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= D.n_tot) return;
@@ -862,7 +1059,17 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         }
         const double dx = xn[0] - on[0], dy = xn[1] - on[1], dz = xn[2] - on[2];
         const double d2 = dx * dx + dy * dy + dz * dz;
-        atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(d2));
+        // max over the wavefront first when it lies in one replica (d2 >= 0, so its
+        // bits order like the values), then one atomic per wavefront
+        const int r0 = __builtin_amdgcn_readfirstlane(r);
+        if (__ballot(r == r0) == __ballot(true)) {
+            double m = d2;
+            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+            if ((threadIdx.x & 63) == __ffsll((long long)__ballot(true)) - 1)
+                atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(m));
+        } else {
+            atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(d2));
+        }
         // all reads of the old state of this step happen before any write: the
         // new state goes to the second buffer set, swapped by the host
         D.pos[D.n_tot + p] = make_double4(xn[0], xn[1], xn[2], 0.0);

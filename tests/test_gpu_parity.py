@@ -118,3 +118,65 @@ def test_philox_run_is_stable(lib, sys1):
     assert e["klok"] == 501
     assert 30.0 < e["rg"] < 50.0
     eng.close()
+
+
+def test_batched_replicas_equal_single_runs(lib):
+    """Three uneven replicas in one context give what each gives alone
+    (reference-stream mode, FP64: same arithmetic, same draws)."""
+    systems = []
+    for k, (c, box) in enumerate([(0.0005, 350.0), (0.004, 400.0), (0.02, 450.0)]):
+        d, _ = T.build_16scd(lib, T=0.596, c_mg=c, c_k=0.050, seed=k + 1, box=box)
+        systems.append(T.SystemHandle(d))
+    raws = [O.glibc_stream(100 + k, 30 * 4200) for k in range(3)]
+    batch = T.Engine(lib, systems, rng=T.RNG_STREAM, use_graph=0)
+    for k in range(3):
+        batch.set_stream(raws[k], rep=k)
+    batch.step(30)
+    for k in range(3):
+        one = T.Engine(lib, [systems[k]], rng=T.RNG_STREAM, use_graph=0)
+        one.set_stream(raws[k])
+        one.step(30)
+        pb, _, vb = batch.get_state(k)
+        po, _, vo = one.get_state()
+        print(k, systems[k].d["n_particles"], np.abs(pb - po).max())
+        assert batch.cursor(k) == one.cursor()
+        assert np.abs(pb - po).max() < 1e-9
+        one.close()
+    batch.close()
+
+
+def test_high_salt_450A_lists_and_forces(lib):
+    """The 450 A, 20 mM MgCl2 + 100 mM KCl system (14,957 particles): pair sets
+    bit-exact and mixed-precision forces within 1e-5 of the oracle."""
+    d, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.100, seed=1, box=450.0)
+    assert d["n_particles"] == 14957
+    h = T.SystemHandle(d)
+    o = O.Oracle(h)
+    o.set_state(d["pos"], d["pos_old"], d["vel"])
+    eng = T.Engine(lib, [h], precision=T.PREC_MIXED)
+    assert np.array_equal(eng.pairs(0), o.pairs(0))
+    assert np.array_equal(eng.pairs(1), o.pairs(1))
+    mask = T.TERM_STACK | T.TERM_BACKBONE | T.TERM_SR | T.TERM_LR
+    f, _ = eng.eval(mask)
+    _, fo, _ = o.eval(mask)
+    print("rel", rel(f, fo))
+    assert rel(f, fo) < 1e-5
+    eng.close()
+
+
+def test_graph_replays_match_eager_steps(lib, sys1):
+    """hipGraph replays (50 steps each) give the same trajectory as eager launches."""
+    raw = O.glibc_stream(31, 100 * 4200)
+    a = T.Engine(lib, [sys1], rng=T.RNG_STREAM, use_graph=1, graph_steps=50)
+    b = T.Engine(lib, [sys1], rng=T.RNG_STREAM, use_graph=0)
+    a.set_stream(raw)
+    b.set_stream(raw)
+    for _ in range(2):
+        a.step(50)
+        b.step(50)
+        pa, _, _ = a.get_state()
+        pb, _, _ = b.get_state()
+        assert a.cursor() == b.cursor()
+        assert np.abs(pa - pb).max() < 1e-9
+    a.close()
+    b.close()
