@@ -181,7 +181,9 @@ struct tis_ctx {
             if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour pool overflow in replica " + std::to_string(r)};
             if (s[r].overflow & 2) throw Fail{TIS_EOVERFLOW, "HB working set above the LDS caps in replica " + std::to_string(r)};
             if (s[r].overflow & 4) throw Fail{TIS_EOVERFLOW, "uniform stream exhausted in replica " + std::to_string(r)};
-            if (s[r].overflow & 16) throw Fail{TIS_EOVERFLOW, "HB site with more than 64 candidates in replica " + std::to_string(r)};
+            if (s[r].overflow & 16)
+                throw Fail{TIS_EOVERFLOW, "HB valence resolution: a site has more live candidates than the kernel holds "
+                                          "(64 in the sequential form, 8 in the component form) in replica " + std::to_string(r)};
         }
     }
 };
@@ -633,6 +635,7 @@ extern "C" int tis_get_energies(tis_ctx *c, int rep, tis_energies *e) {
         e->e_bond = s.e[EI_BOND]; e->e_angle = s.e[EI_ANGLE];
         e->klok = s.klok; e->n_rebuilds = s.n_rebuilds; e->progress = s.progress;
         e->n_hb_active = s.n_act;
+        e->hb_rounds = s.hb_iters;
         // Rg of the beads (radius_of_gyration, fast.cpp:2844-2876)
         std::vector<double4> x(P.nbeads);
         HIP_TRY(hipMemcpy(x.data(), c->D.pos + P.p0, P.nbeads * sizeof(double4), hipMemcpyDeviceToHost));

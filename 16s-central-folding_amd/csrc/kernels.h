@@ -331,11 +331,14 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 }
 
 // ------------------------------------------------------------ k_hb
-// LDS layout (bytes): E[kActMax] double | flag[kActMax] u8 | excess[kASiteMax] int |
-// A0, A1 [kASiteMax] short | asite_off[kASiteMax+1] short | asite_list[kAIncMax] short |
-// ahb_off[kActMax+1] short | ahb_inc[kAIncMax] short
-constexpr size_t kHbLds = kActMax * 8 + kActMax + kASiteMax * 4 + 2 * kASiteMax * 2 + (kASiteMax + 1) * 2 +
-                          kAIncMax * 2 + (kActMax + 1) * 2 + kAIncMax * 2 + 64;
+// LDS layout of one replica's HB working set (bytes, in this order):
+//   E[kActMax] double | lab[kASiteMax] int | excess[kASiteMax] int | flag[kActMax] u8 |
+//   A0, A1, croot, csites [kASiteMax] short | aoff, coff [kASiteMax+1] short |
+//   hoff[kActMax+1] short | alist, hinc [kAIncMax] short | tbuf[kBlock * kKMax] short
+constexpr int kKMax = 8;   // live candidates per over-subscribed site (component mode)
+constexpr size_t kHbLds = kActMax * 8 + 2 * kASiteMax * 4 + kActMax + 4 * kASiteMax * 2 + 2 * (kASiteMax + 1) * 2 +
+                          (kActMax + 1) * 2 + 2 * kAIncMax * 2 + kBlock * kKMax * 2 + 64;
+static_assert(kHbLds <= 160 * 1024 - 512, "HB working set must fit the LDS of one CU");
 
 enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
 
@@ -628,6 +631,163 @@ TIS_DEV void resolve_in_lds(int nA, short *A, short *B, int *excess, const short
     *base_out = base;
 }
 
+// Valence resolution by independent components (production, Philox draws).
+//
+// Over-subscribed sites interact only through live candidates they share, and
+// a site whose excess is <= 0 can never be picked again.  Split the
+// over-subscribed sites into connected components of that sharing graph: an
+// iteration of the reference's process picks a uniformly random
+// over-subscribed site, so, conditioned on landing in component C, it is a
+// uniform pick among C's sites, and its update touches only C.  Running each
+// component's own process (same pick / shuffle / acceptance rules) therefore
+// gives the same joint distribution of switched-off candidates as the
+// sequential loop; the components run concurrently, one thread each, and the
+// serial depth drops from the total iteration count to the largest component's.
+// Draws: Philox keyed by (seed, replica), counter (step, component, iteration,
+// draw), so results do not depend on thread timing.  Returns false if a site
+// has more than kKMax live candidates.
+TIS_DEV bool resolve_by_components(long long klok, uint64_t seed, int r, int n_act, int n_as, int *excess, int *lab,
+                                   unsigned char *flag, short *A0, short *A1, short *croot, short *csites,
+                                   const short *aoff, short *coff, const short *alist, const short *hoff,
+                                   const short *hinc, short *tbuf, const double *E, double T, int *iters_out) {
+    __shared__ int scratch[20];
+    __shared__ int sh_changed, sh_rounds, sh_bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // over-subscribed sites in site order; A1 maps an active site to its index or -1
+    int nA0 = 0;
+    for (int s0 = 0; s0 < n_as; s0 += nt) {
+        const int s = s0 + tid;
+        const bool c = s < n_as && excess[s] > 0;
+        int tot;
+        const int rank = block_excl_sum(c, scratch, &tot);
+        if (s < n_as) {
+            A1[s] = (short)(c ? nA0 + rank : -1);
+            if (c) A0[nA0 + rank] = (short)s;
+        }
+        nA0 += tot;
+    }
+    for (int ci = tid; ci < nA0; ci += nt) lab[ci] = ci;
+    if (tid == 0) { sh_rounds = 0; sh_bad = 0; }
+    __syncthreads();
+    // component labels: minimum index, by propagation over the live candidates
+    for (;;) {
+        if (tid == 0) sh_changed = 0;
+        __syncthreads();
+        for (int a = tid; a < n_act; a += nt) {
+            if (flag[a] != FLAG_LIVE) continue;
+            int m = 1 << 30, n = 0;
+            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
+                const int ci = A1[hinc[q]];
+                if (ci >= 0) { m = min(m, lab[ci]); ++n; }
+            }
+            if (n < 2) continue;
+            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
+                const int ci = A1[hinc[q]];
+                if (ci >= 0 && lab[ci] > m) { atomicMin(&lab[ci], m); sh_changed = 1; }
+            }
+        }
+        __syncthreads();
+        for (int ci = tid; ci < nA0; ci += nt) {   // pointer jumping (labels only decrease)
+            const int l = lab[ci], ll = lab[l];
+            if (ll < l) atomicMin(&lab[ci], ll);
+        }
+        __syncthreads();
+        if (!sh_changed) break;
+    }
+    // components in order of their smallest site; members in site order
+    int ncomp = 0;
+    for (int c0 = 0; c0 < nA0; c0 += nt) {
+        const int ci = c0 + tid;
+        const bool root = ci < nA0 && lab[ci] == ci;
+        int tot;
+        const int rank = block_excl_sum(root, scratch, &tot);
+        if (root) croot[ncomp + rank] = (short)ci;
+        ncomp += tot;
+    }
+    __syncthreads();
+    int run = 0;
+    for (int k0 = 0; k0 < ncomp; k0 += nt) {
+        const int k = k0 + tid;
+        int cnt = 0;
+        if (k < ncomp)
+            for (int ci = croot[k]; ci < nA0; ++ci) cnt += lab[ci] == croot[k];
+        int tot;
+        const int off = block_excl_sum(cnt, scratch, &tot);
+        if (k < ncomp) coff[k] = (short)(run + off);
+        run += tot;
+    }
+    if (tid == 0) coff[ncomp] = (short)run;
+    __syncthreads();
+    for (int k = tid; k < ncomp; k += nt) {
+        int w = coff[k];
+        for (int ci = croot[k]; ci < nA0; ++ci)
+            if (lab[ci] == croot[k]) csites[w++] = (short)ci;
+    }
+    __syncthreads();
+    // each component's own process
+    const uint32_t k0 = (uint32_t)seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), k1 = (uint32_t)(seed >> 32);
+    short *tb = tbuf + tid * kKMax;
+    for (int k = tid; k < ncomp; k += nt) {
+        const int q0 = coff[k], q1 = coff[k + 1];
+        int it = 0;
+        for (;;) {
+            int nA = 0;
+            for (int q = q0; q < q1; ++q) nA += excess[A0[csites[q]]] > 0;
+            if (nA == 0) break;
+            // draw i of this iteration: Philox block i/4, word i%4
+            Philox4 blk = philox4x32_10((uint32_t)klok, (uint32_t)(klok >> 32),
+                                        ((uint32_t)k << 20) | ((uint32_t)it << 6), kTagHb, k0, k1);
+            int blk_id = 0;
+            auto draw = [&](int i) {
+                if ((i >> 2) != blk_id) {
+                    blk_id = i >> 2;
+                    blk = philox4x32_10((uint32_t)klok, (uint32_t)(klok >> 32),
+                                        ((uint32_t)k << 20) | ((uint32_t)it << 6) | (uint32_t)blk_id, kTagHb, k0, k1);
+                }
+                return u01_closed_open(blk.v[i & 3]);
+            };
+            int j = ceil_index(draw(0), nA) - 1;
+            int site = -1;
+            for (int q = q0; q < q1; ++q) {
+                const int s = A0[csites[q]];
+                if (excess[s] > 0) {
+                    if (j == 0) { site = s; break; }
+                    --j;
+                }
+            }
+            int nK = 0;
+            for (int q = aoff[site]; q < aoff[site + 1]; ++q) {
+                const int b = alist[q];
+                if (flag[b] != FLAG_LIVE) continue;
+                if (nK < kKMax) tb[nK] = (short)b;
+                ++nK;
+            }
+            if (nK > kKMax || nK == 0) { sh_bad = 1; break; }
+            for (int i = 0; i < nK; ++i) {
+                const int jj = ceil_index(draw(1 + i), nK) - 1;
+                const short t = tb[i];
+                tb[i] = tb[jj];
+                tb[jj] = t;
+            }
+            int kk = tb[0];
+            for (int i = 1; i < nK; ++i) {
+                const int jb = tb[i];
+                if (draw(nK + i) < exp((E[jb] - E[kk]) / T)) kk = jb;
+            }
+            flag[kk] = FLAG_OFF;
+            for (int q = hoff[kk]; q < hoff[kk + 1]; ++q) {
+                const int s = hinc[q];
+                if (A1[s] >= 0) excess[s] -= 1;
+            }
+            if (++it >= (1 << 14)) { sh_bad = 1; break; }
+        }
+        atomicMax(&sh_rounds, it);
+    }
+    __syncthreads();
+    *iters_out = sh_rounds;
+    return sh_bad == 0;
+}
+
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     // mode bit 0: step (advance the stream cursor, build the noise table); bit 1: forces on
@@ -637,14 +797,19 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     double4 *F = D.f_mb;
     extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
-    unsigned char *flag = (unsigned char *)(E + kActMax);
-    int *excess = (int *)(flag + kActMax);
-    short *A0 = (short *)(excess + kASiteMax);
+    int *lab = (int *)(E + kActMax);
+    int *excess = lab + kASiteMax;
+    unsigned char *flag = (unsigned char *)(excess + kASiteMax);
+    short *A0 = (short *)(flag + kActMax);
     short *A1 = A0 + kASiteMax;
-    short *aoff = A1 + kASiteMax;
-    short *alist = aoff + kASiteMax + 1;
-    short *hoff = alist + kAIncMax;
-    short *hinc = hoff + kActMax + 1;
+    short *croot = A1 + kASiteMax;
+    short *csites = croot + kASiteMax;
+    short *aoff = csites + kASiteMax;
+    short *coff = aoff + kASiteMax + 1;
+    short *hoff = coff + kASiteMax + 1;
+    short *alist = hoff + kActMax + 1;
+    short *hinc = alist + kAIncMax;
+    short *tbuf = hinc + kAIncMax;
     __shared__ double red[8];
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
@@ -682,27 +847,39 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
         excess[s] = c - D.site_val[D.asite_id[P.site0 + s]];
     }
     __syncthreads();
-    // ---- valence resolution (fast.cpp:2239-2315), one wavefront
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        int nA = 0;
-        for (int s0 = 0; s0 < n_as; s0 += 64) {
-            const int s = s0 + lane;
-            const bool keep = s < n_as && excess[s] > 0;
-            const unsigned long long m = __ballot(keep);
-            if (keep) A0[nA + __popcll(m & ((1ull << lane) - 1ull))] = (short)s;
-            nA += __popcll(m);
+    // ---- valence resolution (fast.cpp:2239-2315)
+    if (D.rng == 1) {
+        // reference stream: the exact sequential process, one wavefront
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            int nA = 0;
+            for (int s0 = 0; s0 < n_as; s0 += 64) {
+                const int s = s0 + lane;
+                const bool keep = s < n_as && excess[s] > 0;
+                const unsigned long long m = __ballot(keep);
+                if (keep) A0[nA + __popcll(m & ((1ull << lane) - 1ull))] = (short)s;
+                nA += __popcll(m);
+            }
+            HbDraws dr(D, r, S);
+            int iters = 0;
+            long long base = 0;
+            bool ovf = false;
+            if (!resolve_in_registers<8, 8>(nA, A0, excess, aoff, alist, flag, E, P.T, dr, &iters, &base, &ovf))
+                resolve_in_lds(nA, A0, A1, excess, aoff, alist, hoff, hinc, flag, E, P.T, dr, &iters, &base, &ovf);
+            if (lane == 0) {
+                sh_iters = iters;
+                sh_draws = base;
+                if (ovf) sh_err = 1;
+            }
         }
-        HbDraws dr(D, r, S);
+    } else {
         int iters = 0;
-        long long base = 0;
-        bool ovf = false;
-        if (!resolve_in_registers<8, 8>(nA, A0, excess, aoff, alist, flag, E, P.T, dr, &iters, &base, &ovf))
-            resolve_in_lds(nA, A0, A1, excess, aoff, alist, hoff, hinc, flag, E, P.T, dr, &iters, &base, &ovf);
-        if (lane == 0) {
+        const bool ok = resolve_by_components(S.klok, D.seed, r, n_act, n_as, excess, lab, flag, A0, A1, croot, csites,
+                                              aoff, coff, alist, hoff, hinc, tbuf, E, P.T, &iters);
+        if (threadIdx.x == 0) {
             sh_iters = iters;
-            sh_draws = base;
-            if (ovf) sh_err = 1;
+            sh_draws = 0;
+            if (!ok) sh_err = 1;
         }
     }
     __syncthreads();

@@ -60,7 +60,7 @@ class Energies(C.Structure):
                 ("e_coul_lr", C.c_double), ("e_bond", C.c_double), ("e_angle", C.c_double),
                 ("rg", C.c_double), ("klok", C.c_longlong), ("n_rebuilds", C.c_longlong),
                 ("progress", C.c_double), ("n_sr_pairs", C.c_longlong),
-                ("n_lr_pairs", C.c_longlong), ("n_hb_active", C.c_longlong)]
+                ("n_lr_pairs", C.c_longlong), ("n_hb_active", C.c_longlong), ("hb_rounds", C.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -270,6 +270,9 @@ class Engine:
         a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (pos, old, vel)]
         check(self.lib, self.lib.tis_set_state(self.ctx, rep, *[x.ctypes.data_as(_dp) for x in a], klok),
               "tis_set_state")
+
+    def set_step_counter(self, klok, rep=0):
+        check(self.lib, self.lib.tis_set_step_counter(self.ctx, rep, klok), "tis_set_step_counter")
 
     def energies(self, rep=0):
         e = Energies()

@@ -104,14 +104,16 @@ def main():
     n_rep = len(handles)
     value = n_rep * args.steps / dt
     # per-kernel device times over a separate event-timed run of the same batch
-    eng.set_profiling(True)
-    eng.step(args.profile_steps)
-    kt = eng.kernel_times()
-    eng.set_profiling(False)
-    dominant = max(kt, key=lambda k: kt[k][0] * kt[k][1])
-    n_launch, avg_ms = kt[dominant]
-    bytes_launch = algorithmic_bytes(eng, handles, dominant)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    kt, dominant, avg_ms, bytes_launch, achieved = {}, None, None, None, None
+    if args.profile_steps > 0:
+        eng.set_profiling(True)
+        eng.step(args.profile_steps)
+        kt = eng.kernel_times()
+        eng.set_profiling(False)
+        dominant = max(kt, key=lambda k: kt[k][0] * kt[k][1])
+        n_launch, avg_ms = kt[dominant]
+        bytes_launch = algorithmic_bytes(eng, handles, dominant)
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     energies = [eng.energies(r) for r in range(n_rep)]
     eng.close()
     # plain FP64 one-replica-per-launch path (5 mM, 350 A, 3,659 particles)
@@ -151,9 +153,10 @@ def main():
         "plain_fp64_single": {"value": plain, "unit": "steps/s", "particles": d1["n_particles"],
                               "ns_per_day": plain * FS_PER_STEP * 86400 * 1e-6},
         "batched_vs_plain": value / plain,
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
+        "roofline": None if dominant is None else {
+            "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": [e["n_hb_active"] for e in energies],
         "cpu_baseline": cpu,

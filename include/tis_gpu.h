@@ -188,6 +188,7 @@ typedef struct tis_energies {
     long long n_rebuilds;                      /* list rebuilds so far */
     double progress;                           /* PROGRESS */
     long long n_sr_pairs, n_lr_pairs, n_hb_active;
+    long long hb_rounds;                       /* serial depth of the last HB valence resolution */
 } tis_energies;
 int tis_get_energies(tis_ctx *ctx, int rep, tis_energies *e);
 /* Per-stack energies st_energy[1..s3_N] (the Contacts file, fast.cpp:6290). */

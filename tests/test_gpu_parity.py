@@ -180,3 +180,47 @@ def test_graph_replays_match_eager_steps(lib, sys1):
         assert np.abs(pa - pb).max() < 1e-9
     a.close()
     b.close()
+
+
+def test_component_resolution_has_the_sequential_distribution(lib, sys1):
+    """Production HB valence resolution (independent components, Philox draws)
+    against the reference's sequential process (oracle, glibc draws), on the same
+    configuration: the distribution of the resolved HB energy E2_HB + E3_HB over
+    400 independent draws each must agree (two-sample KS, and the means within 4
+    standard errors).  The serial depth must drop well below the iteration count."""
+    from scipy.stats import ks_2samp
+    eng = T.Engine(lib, [sys1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, seed=11)
+    gpu = []
+    for s in range(400):
+        eng.set_step_counter(1000 + s)
+        _, e = eng.eval(T.TERM_HB)
+        gpu.append(e["e2_hb"] + e["e3_hb"])
+    rounds = eng.energies()["hb_rounds"]
+    eng.close()
+    o = O.Oracle(sys1)
+    d = sys1.d
+    o.set_state(d["pos"], d["pos_old"], d["vel"])
+    ref = []
+    for s in range(400):
+        o.set_stream(T.glibc_stream(lib, 5000 + s, 5000))
+        _, _, e = o.eval(T.TERM_HB)
+        ref.append(e["e2_hb"] + e["e3_hb"])
+    iters = o.hb_iterations()
+    gpu, ref = np.array(gpu), np.array(ref)
+    se = np.sqrt(gpu.var() / len(gpu) + ref.var() / len(ref))
+    print("mean gpu", gpu.mean(), "ref", ref.mean(), "se", se, "KS p", ks_2samp(gpu, ref).pvalue,
+          "rounds", rounds, "sequential iterations", iters)
+    assert abs(gpu.mean() - ref.mean()) < 4 * se
+    assert ks_2samp(gpu, ref).pvalue > 1e-3
+    assert rounds < iters / 4
+
+
+def test_philox_resolution_is_deterministic(lib, sys1):
+    a = T.Engine(lib, [sys1], rng=T.RNG_PHILOX, seed=5)
+    b = T.Engine(lib, [sys1], rng=T.RNG_PHILOX, seed=5)
+    fa, ea = a.eval(T.TERM_HB)
+    fb, eb = b.eval(T.TERM_HB)
+    assert ea["e2_hb"] == eb["e2_hb"] and ea["e3_hb"] == eb["e3_hb"]
+    assert np.abs(fa - fb).max() < 1e-12
+    a.close()
+    b.close()
