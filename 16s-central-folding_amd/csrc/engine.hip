@@ -448,6 +448,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.site_val = dalloc<int>(own, sv.size()); upload(D.site_val, sv);
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
         D.hb_status = dalloc<signed char>(own, nhb_tot);
+        D.hb_out = dalloc<signed char>(own, nhb_tot);
         D.act_hb = dalloc<int>(own, nhb_tot);
         D.act_idx = dalloc<int>(own, nhb_tot);
         D.asite_id = dalloc<int>(own, nsite_tot);
@@ -766,6 +767,20 @@ extern "C" int tis_get_hb_status(tis_ctx *c, int rep, int *out, int n) {
         std::vector<signed char> s(P.nhb);
         if (P.nhb) HIP_TRY(hipMemcpy(s.data(), c->D.hb_status + P.hb0, P.nhb, hipMemcpyDeviceToHost));
         for (int k = 0; k < P.nhb; ++k) out[k] = s[k];
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_hb_outcome(tis_ctx *c, int rep, int *out, int n) {
+    if (!c || rep < 0 || rep >= c->n_rep || !out) return fail(Fail{TIS_EINVAL, "tis_get_hb_outcome: bad arguments"});
+    return guarded([&]() -> int {
+        const RepParams &P = c->rp[rep];
+        if (n < P.nhb) throw Fail{TIS_EINVAL, "tis_get_hb_outcome: buffer too small"};
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        HIP_TRY(hipStreamSynchronize(c->s_hb));
+        std::vector<signed char> v(P.nhb);
+        if (P.nhb) HIP_TRY(hipMemcpy(v.data(), c->D.hb_out + P.hb0, P.nhb, hipMemcpyDeviceToHost));
+        for (int k = 0; k < P.nhb; ++k) out[k] = v[k];
         return TIS_OK;
     });
 }

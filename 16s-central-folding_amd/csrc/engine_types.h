@@ -98,6 +98,7 @@ struct Dev {
     int *site_val;
     // HB dynamic (rebuild products)
     signed char *hb_status;       // -1 / 0 per candidate
+    signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off
     int *act_hb;                  // per replica region [hb0, hb0 + nhb): live candidate ids
     int *act_idx;                 // candidate id -> live index or -1
     int *asite_id;                // region [site0, ...): site of each active site

@@ -257,6 +257,7 @@ __global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
             const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
             live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
             D.hb_status[g] = live ? 0 : -1;
+            D.hb_out[g] = live ? 0 : -1;
         }
         int tot;
         int rank = block_excl_sum(live, scratch, &tot);
@@ -883,6 +884,7 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
         }
     }
     __syncthreads();
+    for (int a = threadIdx.x; a < n_act; a += blockDim.x) D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
     // ---- forces of the surviving HBs and the E2/E3 sums
     double e2 = 0, e3 = 0;
     if (mode & 2) {

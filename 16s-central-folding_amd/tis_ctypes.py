@@ -129,6 +129,7 @@ def load_library(path=LIB_PATH):
         lib.tis_get_pairs.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_longlong,
                                       C.POINTER(C.c_longlong)]
         lib.tis_get_hb_status.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        lib.tis_get_hb_outcome.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
         lib.tis_set_uniform_stream.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_longlong]
         lib.tis_get_stream_cursor.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
         lib.tis_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
@@ -298,6 +299,12 @@ class Engine:
         n = self.handles[rep].d["n_hb"]
         out = np.zeros(max(n, 1), dtype=np.int32)
         check(self.lib, self.lib.tis_get_hb_status(self.ctx, rep, out.ctypes.data_as(_ip), n), "tis_get_hb_status")
+        return out[:n]
+
+    def hb_outcome(self, rep=0):
+        n = self.handles[rep].d["n_hb"]
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        check(self.lib, self.lib.tis_get_hb_outcome(self.ctx, rep, out.ctypes.data_as(_ip), n), "tis_get_hb_outcome")
         return out[:n]
 
     def set_stream(self, raw, rep=0):

@@ -204,6 +204,10 @@ int tis_eval_forces(tis_ctx *ctx, int rep, unsigned term_mask, double *f_out, ti
 int tis_get_pairs(tis_ctx *ctx, int rep, int which, int *ij_out, long long cap, long long *n_out);
 /* Parity hook: hb_status (-1 / 0) of every candidate after the last list build. */
 int tis_get_hb_status(tis_ctx *ctx, int rep, int *status_out, int n);
+/* Parity hook: outcome of every candidate in the last HB evaluation: -1 not in the
+ * SR list, 0 outside the distance gate, 1 kept, 2 switched off by the valence
+ * resolution (hydrogen_bonds_interactions, fast.cpp:2165-2320). */
+int tis_get_hb_outcome(tis_ctx *ctx, int rep, int *out, int n);
 
 /* TIS_RNG_STREAM: raw glibc rand() values (0..RAND_MAX) consumed in the reference's
  * order (HB valence resolution, then 1500 Box-Muller pairs per step).  Resets the

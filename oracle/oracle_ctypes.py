@@ -49,6 +49,7 @@ def load(path=LIB_PATH):
     lib.oracle_klok.argtypes = [vp]
     lib.oracle_klok.restype = C.c_longlong
     lib.oracle_stack_energies.argtypes = [vp, dp]
+    lib.oracle_hb_outcome.argtypes = [vp, ip]
     lib.oracle_hb_iterations.argtypes = [vp]
     lib.oracle_hb_iterations.restype = C.c_longlong
     _lib = lib
@@ -119,6 +120,11 @@ class Oracle:
     def hb_status(self):
         out = np.zeros(self.n_hb, dtype=np.int32)
         self.lib.oracle_hb_status(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def hb_outcome(self):
+        out = np.zeros(self.n_hb, dtype=np.int32)
+        self.lib.oracle_hb_outcome(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int)))
         return out
 
     def stack_energies(self):

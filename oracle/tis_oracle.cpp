@@ -50,7 +50,7 @@ struct oracle_ctx {
     double r_sr2 = 0, r_lr2 = 0, r_cell = 0, skin = 3.0;
     int lr_every = 10;
     std::vector<int> list_sr, list_lr;   // flat (i,j) in the reference's visiting order
-    std::vector<int> hb_status, hb_excess0;
+    std::vector<int> hb_status, hb_excess0, hb_outcome;
     // ---- state
     std::vector<double> x, xo, v, f, fl;
     long long klok = 1, rebuilds = 0, hb_iters = 0;
@@ -351,6 +351,7 @@ struct oracle_ctx {
     void hydrogen_bonds(std::vector<double> &F) {
         excess = hb_excess0;
         std::fill(hb_energy.begin(), hb_energy.end(), 0.0);
+        for (int k = 0; k < nh; ++k) hb_outcome[k] = hb_status[k] == -1 ? -1 : 1;
         for (int k = 0; k < nh; ++k) {
             if (hb_status[k] == -1) continue;
             const int *b = &hbb[6 * k];
@@ -358,6 +359,7 @@ struct oracle_ctx {
             double r = dist(b[1], b[4]);
             if (std::fabs(r - g[0]) > 2.0) {
                 hb_status[k] = 1;
+                hb_outcome[k] = 0;
                 for (int q = hbso[k]; q < hbso[k + 1]; ++q) excess[hbsi[q]] -= 1;
                 continue;
             }
@@ -413,6 +415,7 @@ struct oracle_ctx {
                 if (u < pr) k = jb;
             }
             hb_status[k] = 1;
+            hb_outcome[k] = 2;
             for (int q = hbso[k]; q < hbso[k + 1]; ++q) excess[hbsi[q]] -= 1;
         }
         for (int k = 0; k < nh; ++k) {
@@ -660,6 +663,7 @@ oracle_ctx *oracle_create(const tis_system *s, double r_sr, double r_lr, double 
     o->lr_every = lr_every;
     o->hb_status.assign(o->nh, -1);
     o->hb_excess0.assign(o->ns, 0);
+    o->hb_outcome.assign(o->nh, -1);
     o->hb_energy.assign(o->nh, 0.0);
     o->hb_psi0.assign(3 * (size_t)o->nh, 0.0);
     o->st_energy.assign(o->n3, 0.0);
@@ -779,5 +783,10 @@ int oracle_stack_energies(const oracle_ctx *o, double *out) {
     return TIS_OK;
 }
 long long oracle_hb_iterations(const oracle_ctx *o) { return o ? o->hb_iters : -1; }
+int oracle_hb_outcome(const oracle_ctx *o, int *out) {
+    if (!o || !out) return TIS_EINVAL;
+    std::copy(o->hb_outcome.begin(), o->hb_outcome.end(), out);
+    return TIS_OK;
+}
 
 }  // extern "C"

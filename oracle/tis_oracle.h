@@ -67,6 +67,9 @@ double oracle_progress(const oracle_ctx *o);
 long long oracle_klok(const oracle_ctx *o);
 /* Per-tertiary-stack energies of the last force evaluation (n_st3 values). */
 int oracle_stack_energies(const oracle_ctx *o, double *out);
+/* Outcome of every candidate in the last HB evaluation: -1 not in the SR list,
+ * 0 outside the distance gate, 1 kept, 2 switched off by the valence resolution. */
+int oracle_hb_outcome(const oracle_ctx *o, int *out);
 /* Number of HB valence-resolution iterations in the last HB evaluation. */
 long long oracle_hb_iterations(const oracle_ctx *o);
 
