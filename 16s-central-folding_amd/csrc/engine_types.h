@@ -57,7 +57,7 @@ enum { EI_E2_HB, EI_E3_HB, EI_E2_ST, EI_E3_ST, EI_WCA, EI_COUL_SR, EI_COUL_LR, E
 
 // Caps of the HB resolution working set held in LDS (per replica).
 constexpr int kActMax = 3072;     // live candidates
-constexpr int kASiteMax = 4096;   // sites with live candidates
+constexpr int kASiteMax = 3072;   // sites with live candidates
 constexpr int kAIncMax = 8192;    // incidences of live candidates
 
 struct Dev {

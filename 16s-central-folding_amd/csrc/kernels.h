@@ -336,7 +336,7 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 //   E[kActMax] double | lab[kASiteMax] int | excess[kASiteMax] int | flag[kActMax] u8 |
 //   A0, A1, croot, csites [kASiteMax] short | aoff, coff [kASiteMax+1] short |
 //   hoff[kActMax+1] short | alist, hinc [kAIncMax] short | tbuf[kBlock * kKMax] short
-constexpr int kKMax = 8;   // live candidates per over-subscribed site (component mode)
+constexpr int kKMax = 32;  // live candidates per over-subscribed site (component mode)
 constexpr size_t kHbLds = kActMax * 8 + 2 * kASiteMax * 4 + kActMax + 4 * kASiteMax * 2 + 2 * (kASiteMax + 1) * 2 +
                           (kActMax + 1) * 2 + 2 * kAIncMax * 2 + kBlock * kKMax * 2 + 64;
 static_assert(kHbLds <= 160 * 1024 - 512, "HB working set must fit the LDS of one CU");
@@ -670,8 +670,11 @@ TIS_DEV bool resolve_by_components(long long klok, uint64_t seed, int r, int n_a
     for (int ci = tid; ci < nA0; ci += nt) lab[ci] = ci;
     if (tid == 0) { sh_rounds = 0; sh_bad = 0; }
     __syncthreads();
-    // component labels: minimum index, by propagation over the live candidates
+    // component labels: minimum index, by propagation over the live candidates.
+    // The barrier at the top of each pass keeps thread 0 from clearing the flag
+    // before every thread has read it for the previous pass's exit test.
     for (;;) {
+        __syncthreads();
         if (tid == 0) sh_changed = 0;
         __syncthreads();
         for (int a = tid; a < n_act; a += nt) {

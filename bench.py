@@ -78,6 +78,10 @@ def cpu_baseline(systems, steps_each):
     return total_steps / total_t
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,13 +98,16 @@ def main():
         sys.exit(2)
     lib = T.load_library()
     systems = batched_systems(lib)
+    log(f"built {len(systems)} replicas")
     handles = [T.SystemHandle(d) for d in systems]
     eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024, use_graph=1,
                    graph_steps=50)
     eng.step(args.warmup)
+    log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
     eng.step(args.steps)          # returns after a stream synchronize
     dt = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {dt:.2f} s")
     n_rep = len(handles)
     value = n_rep * args.steps / dt
     # per-kernel device times over a separate event-timed run of the same batch
@@ -120,6 +127,7 @@ def main():
     d1, _ = T.build_16scd(lib, T=0.596, c_mg=0.005, c_k=0.050, seed=1, box=350.0)
     h1 = T.SystemHandle(d1)
     e1 = T.Engine(lib, [h1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, use_graph=0)
+    log("plain FP64 single-replica leg")
     e1.step(max(50, args.warmup // 5))
     t0 = time.perf_counter()
     e1.step(args.plain_steps)
@@ -127,6 +135,7 @@ def main():
     e1.close()
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline leg")
         sample = systems
         v = cpu_baseline(sample, args.cpu_steps)
         cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
