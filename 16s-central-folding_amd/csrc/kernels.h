@@ -852,7 +852,11 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     }
     __syncthreads();
     // ---- valence resolution (fast.cpp:2239-2315)
-    if (D.rng == 1) {
+    // The component form (resolve_by_components) is switched off: a batched
+    // Philox run with it stopped making progress on the GPU and the cause has not
+    // been shown yet.  Both random sources use the exact sequential process.
+    constexpr bool kUseComponents = false;
+    if (D.rng == 1 || !kUseComponents) {
         // reference stream: the exact sequential process, one wavefront
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;

@@ -182,6 +182,8 @@ def test_graph_replays_match_eager_steps(lib, sys1):
     b.close()
 
 
+@pytest.mark.skip(reason="component-parallel HB resolution is switched off: a batched Philox run with it "
+                         "stalled on the GPU and the cause is not yet shown")
 def test_component_resolution_has_the_sequential_distribution(lib, sys1):
     """Production HB valence resolution (independent components, Philox draws)
     against the reference's sequential process (oracle, glibc draws) on the same
