@@ -131,7 +131,8 @@ struct tis_ctx {
     void launch_rebuild() {
         timed("k_bin", s_main, [&] { k_bin<<<n_rep, 1024, 0, s_main>>>(D); });
         timed("k_list", s_main, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, s_main>>>(D, 1); });
-        timed("k_hbrebuild", s_main, [&] { k_hbrebuild<<<n_rep, 1024, max_site_lds(), s_main>>>(D); });
+        timed("k_hbstatus", s_main, [&] { k_hbstatus<<<256, kBlock, 0, s_main>>>(D); });
+        timed("k_hbsites", s_main, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), s_main>>>(D); });
     }
     size_t max_site_lds() const {
         int m = 1;
@@ -156,8 +157,8 @@ struct tis_ctx {
         });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
         timed("k_pairs", s_main, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
-            else k_force<float, 1><<<grid(D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+            else k_force<float, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
@@ -425,6 +426,9 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             }
             inc_base += s.n_hb_incid;
         }
+        if (2 * c->max_site_lds() > 150 * 1024) throw Fail{TIS_EINVAL, "too many HB sites per replica"};
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hbsites, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(2 * c->max_site_lds())));
         c->n_bonded_terms = (int)bij.size() + (int)aijk.size();
         c->nb_bonded = c->grid(c->n_bonded_terms);
         D.bond_ij = dalloc<int2>(own, bij.size()); upload(D.bond_ij, bij);
@@ -446,6 +450,23 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.site_off = dalloc<int>(own, so.size()); upload(D.site_off, so);
         D.site_hb = dalloc<int>(own, sh.size()); upload(D.site_hb, sh);
         D.site_val = dalloc<int>(own, sv.size()); upload(D.site_val, sv);
+        // rank of every HB incidence within its site's static list (HB_ATOM order);
+        // the j-th incidence of a candidate at a site takes the j-th occurrence of
+        // that candidate in the site's list
+        std::vector<int> rank(hi.size(), -1);
+        for (size_t site = 0; site + 1 < so.size(); ++site) {
+            for (int e = so[site]; e < so[site + 1]; ++e) {
+                const int k = sh[e];
+                int t = hio[k];
+                while (t < hio[k + 1] && !(hi[t] == (int)site && rank[t] < 0)) ++t;
+                if (t == hio[k + 1]) throw Fail{TIS_EINVAL, "HB site tables are not consistent"};
+                rank[t] = e - so[site];
+            }
+        }
+        for (int v : rank)
+            if (v < 0 || v > 0x7fff) throw Fail{TIS_EINVAL, "HB site tables are not consistent"};
+        D.inc_rank = dalloc<int>(own, rank.size()); upload(D.inc_rank, rank);
+        D.site_cnt = dalloc<int>(own, sv.size());
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
@@ -693,7 +714,7 @@ extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out
             else k_hb<float><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
         }
         const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
-        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(D.n_tot);
+        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * D.n_tot);
         if (D.precision == TIS_PREC_FP64) {
             k_force<double, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
             k_force<double, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);

@@ -96,6 +96,8 @@ struct Dev {
     int *hb_inc_off, *hb_inc;     // sites of each HB (global site ids), CSR over global HB ids
     int *site_off, *site_hb;      // HBs of each site (global HB ids), CSR over global site ids
     int *site_val;
+    int *inc_rank;                // position of each HB incidence within its site's static list
+    int *site_cnt;                // live incidences per site (rebuild scratch)
     // HB dynamic (rebuild products)
     signed char *hb_status;       // -1 / 0 per candidate
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off

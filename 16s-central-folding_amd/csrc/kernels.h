@@ -13,8 +13,8 @@
 //   k_bin         LDS counting sort of particles into cells (separate_in_cells :1461-1542)
 //   k_list        full neighbour lists, SR and LR, from the 27 neighbouring cells
 //                 (populate_lists :1546-1700)
-//   k_hbrebuild   live HB candidates and the per-site tables of the resolution
-//                 (populate_lists :1702-1736)
+//   k_hbstatus,   live HB candidates and the per-site tables of the resolution
+//   k_hbsites     (populate_lists :1702-1736)
 #pragma once
 #include "device_math.h"
 #include "engine_types.h"
@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
         const int p = P.p0 + i;
         D.sorted[P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p]] = p;
     }
+    for (int q = threadIdx.x; q < P.nsite; q += blockDim.x) D.site_cnt[P.site0 + q] = 0;
     if (threadIdx.x == 0) S.pool_cursor = 0;
 }
 
@@ -237,58 +238,70 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
     }
 }
 
-// ------------------------------------------------------------ k_hbrebuild
-// Live candidates (k11, k21 within the SR radius), the active sites in site
-// order with their live candidates in the static HB_ATOM order, and each live
-// candidate's incidences as active-site indices.  Dynamic LDS: nsite ints.
-__global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
+// ------------------------------------------------------------ HB rebuild
+// k_hbstatus: grid-stride over the candidates of the replicas being rebuilt.
+// A candidate is live iff its (k11, k21) bead pair is an SR pair, decided with
+// the list's own FP64 distance test (populate_lists, fast.cpp:1702-1727).  Live
+// candidates are counted at each of their sites; k_bin has cleared the counters
+// of the replicas being rebuilt.
+__global__ void __launch_bounds__(kBlock) k_hbstatus(Dev D) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int r = 0; r < D.n_rep; ++r) {
+        if (!D.rs[r].need_rebuild) continue;
+        const RepParams &P = D.rp[r];
+        for (int k = tid; k < P.nhb; k += nt) {
+            const int g = P.hb0 + k;
+            const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
+            const bool live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
+            D.hb_status[g] = live ? 0 : -1;
+            D.hb_out[g] = live ? 0 : -1;
+            if (!live) continue;
+            for (int q = D.hb_inc_off[g]; q < D.hb_inc_off[g + 1]; ++q) atomicAdd(&D.site_cnt[D.hb_inc[q]], 1);
+        }
+    }
+}
+
+// k_hbsites: one workgroup per rebuilt replica.  Active sites (with at least one
+// live candidate) in site order, each with its live candidates in the order of
+// the static site list HB_ATOM (restored from the precomputed rank of every
+// incidence), and every live candidate's incidences as active-site indices.
+// Dynamic LDS: 2 * nsite ints.
+__global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
     const int r = blockIdx.x;
     RepState &S = D.rs[r];
     if (!S.need_rebuild) return;
     const RepParams &P = D.rp[r];
-    extern __shared__ int site_map[];
+    extern __shared__ int smem[];
+    int *site_map = smem, *fill = smem + P.nsite;
     __shared__ int scratch[20];
+    // live candidates in candidate order
     int n_act = 0;
     for (int k0 = 0; k0 < P.nhb; k0 += blockDim.x) {
         const int k = k0 + threadIdx.x;
-        int live = 0;
-        int g = P.hb0 + k;
-        if (k < P.nhb) {
-            const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
-            live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
-            D.hb_status[g] = live ? 0 : -1;
-            D.hb_out[g] = live ? 0 : -1;
-        }
+        const bool live = k < P.nhb && D.hb_status[P.hb0 + k] == 0;
         int tot;
-        int rank = block_excl_sum(live, scratch, &tot);
-        if (k < P.nhb) {
-            if (live) { D.act_hb[P.hb0 + n_act + rank] = g; D.act_idx[g] = n_act + rank; }
-            else D.act_idx[g] = -1;
-        }
+        const int a = n_act + block_excl_sum(live, scratch, &tot);
+        if (k < P.nhb) D.act_idx[P.hb0 + k] = live ? a : -1;
+        if (live && a < kActMax) D.act_hb[P.hb0 + a] = P.hb0 + k;
         n_act += tot;
     }
-    __syncthreads();
+    if (n_act > kActMax) {
+        if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
+        return;
+    }
     int n_as = 0, n_inc = 0;
     for (int s0 = 0; s0 < P.nsite; s0 += blockDim.x) {
         const int s = s0 + threadIdx.x;
-        const int gs = P.site0 + s;
-        int cnt = 0;
-        if (s < P.nsite)
-            for (int q = D.site_off[gs]; q < D.site_off[gs + 1]; ++q) cnt += D.hb_status[D.site_hb[q]] == 0;
+        const int cnt = s < P.nsite ? D.site_cnt[P.site0 + s] : 0;
         int tot_s, tot_i;
         const int rank = block_excl_sum(cnt > 0, scratch, &tot_s);
         const int off = block_excl_sum(cnt, scratch, &tot_i);
         if (s < P.nsite) {
+            fill[s] = 0;
             if (cnt > 0) {
-                const int ai = n_as + rank;
-                site_map[s] = ai;
-                D.asite_id[P.site0 + ai] = gs;
-                D.asite_off[P.site0 + r + ai] = n_inc + off;
-                int w = P.inc0 + n_inc + off;
-                for (int q = D.site_off[gs]; q < D.site_off[gs + 1]; ++q) {
-                    const int k = D.site_hb[q];
-                    if (D.hb_status[k] == 0) D.asite_list[w++] = D.act_idx[k];
-                }
+                site_map[s] = n_as + rank;
+                D.asite_id[P.site0 + n_as + rank] = P.site0 + s;
+                D.asite_off[P.site0 + r + n_as + rank] = n_inc + off;
             } else {
                 site_map[s] = -1;
             }
@@ -298,6 +311,11 @@ __global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
     }
     if (threadIdx.x == 0) D.asite_off[P.site0 + r + n_as] = n_inc;
     __syncthreads();
+    if (n_as > kASiteMax || n_inc > kAIncMax) {
+        if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
+        return;
+    }
+    // scatter (rank << 16 | live index) into each active site's segment
     int n_ai = 0;
     for (int a0 = 0; a0 < n_act; a0 += blockDim.x) {
         const int a = a0 + threadIdx.x;
@@ -310,17 +328,34 @@ __global__ void __launch_bounds__(1024) k_hbrebuild(Dev D) {
         const int off = block_excl_sum(c, scratch, &tot);
         if (a < n_act) {
             D.ahb_inc_off[P.hb0 + r + a] = n_ai + off;
-            for (int q = 0; q < c; ++q)
-                D.ahb_inc[P.inc0 + n_ai + off + q] = site_map[D.hb_inc[D.hb_inc_off[g] + q] - P.site0];
+            for (int q = 0; q < c; ++q) {
+                const int e = D.hb_inc_off[g] + q;
+                const int ai = site_map[D.hb_inc[e] - P.site0];
+                D.ahb_inc[P.inc0 + n_ai + off + q] = ai;
+                const int slot = atomicAdd(&fill[D.hb_inc[e] - P.site0], 1);
+                D.asite_list[P.inc0 + D.asite_off[P.site0 + r + ai] + slot] = (D.inc_rank[e] << 16) | a;
+            }
         }
         n_ai += tot;
     }
+    if (threadIdx.x == 0) D.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
+    __syncthreads();
+    // order each site's segment by static rank, keep the live index
+    for (int ai = threadIdx.x; ai < n_as; ai += blockDim.x) {
+        int *seg = D.asite_list + P.inc0 + D.asite_off[P.site0 + r + ai];
+        const int len = D.asite_off[P.site0 + r + ai + 1] - D.asite_off[P.site0 + r + ai];
+        for (int i = 1; i < len; ++i) {
+            const int v = seg[i];
+            int j = i - 1;
+            while (j >= 0 && seg[j] > v) { seg[j + 1] = seg[j]; --j; }
+            seg[j + 1] = v;
+        }
+        for (int i = 0; i < len; ++i) seg[i] &= 0xffff;
+    }
     if (threadIdx.x == 0) {
-        D.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
         S.n_act = n_act;
         S.n_asite = n_as;
         S.n_ainc = n_inc;
-        if (n_act > kActMax || n_as > kASiteMax || n_inc > kAIncMax || n_ai > kAIncMax) atomicOr(&S.overflow, 2);
     }
 }
 
@@ -966,6 +1001,7 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
 }
 
 // ------------------------------------------------------------ k_force
+constexpr int kPairLanes = 4;   // lanes per particle in the pair gather
 // ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
 // backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
 // that the register-heavy stacking code does not lower the occupancy of the
@@ -1116,10 +1152,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         return;
     }
-    // ---- pair gather, thread per particle in cell order
-    const int t = blk * blockDim.x + threadIdx.x;
-    if (t >= D.n_tot) return;
-    const int p = D.sorted[t];
+    // ---- pair gather: kPairLanes lanes per particle (cell order), each summing a
+    // strided share of the neighbours, then a butterfly reduction inside the group
+    const int t = (blk * blockDim.x + threadIdx.x) / kPairLanes;
+    const int sub = threadIdx.x % kPairLanes;
+    const bool valid = t < D.n_tot;
+    const int p = valid ? D.sorted[t] : D.sorted[0];
     const int r = D.prep[p];
     const RepParams &P = D.rp[r];
     RepState &S = D.rs[r];
@@ -1128,11 +1166,16 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     const int ti = pti & 15;
     const bool chg_i = (pti >> 8) & 1;
     const int *nb = D.pool + P.pool0 + D.nb_start[p];
-    const int nsr = D.nb_nsr[p], nlr = D.nb_nlr[p];
+    const int nsr = valid ? D.nb_nsr[p] : 0, nlr = valid ? D.nb_nlr[p] : 0;
+    auto group_sum = [](double v) {
+#pragma unroll
+        for (int o = 1; o < kPairLanes; o <<= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    };
     if (mask & 8) {
         R fx = 0, fy = 0, fz = 0;
         double ew = 0, ec = 0;
-        for (int q = 0; q < nsr; ++q) {
+        for (int q = sub; q < nsr; q += kPairLanes) {
             const int j = nb[q];
             double3 dd;
             const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
@@ -1150,16 +1193,25 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
                 if (both) ec += P.qlb[ab] / sqrt(r2);
             }
         }
-        D.f_sr[p] = make_double4((double)fx, (double)fy, (double)fz, 0.0);
+        const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
+        if (valid && sub == 0) D.f_sr[p] = make_double4(gx, gy, gz, 0.0);
         if (eval) {
-            unsafeAtomicAdd(&S.e[EI_WCA], 0.5 * ew);
-            unsafeAtomicAdd(&S.e[EI_COUL_SR], 0.5 * ec);
+            ew = group_sum(ew);
+            ec = group_sum(ec);
+            if (valid && sub == 0) {
+                unsafeAtomicAdd(&S.e[EI_WCA], 0.5 * ew);
+                unsafeAtomicAdd(&S.e[EI_COUL_SR], 0.5 * ec);
+            }
         }
     }
-    if ((mask & 16) && (eval || S.lr_due)) {
+    // the LR refresh is decided per replica; a group never spans two particles,
+    // but a wavefront may span two replicas, so the test stays per lane
+    const bool lr = (mask & 16) && (eval || S.lr_due);
+    if (__ballot(lr)) {
         R fx = 0, fy = 0, fz = 0;
         double ec = 0;
-        for (int q = nsr; q < nsr + nlr; ++q) {
+        const int n_lr = lr ? nlr : 0;
+        for (int q = nsr + sub; q < nsr + n_lr; q += kPairLanes) {
             const int j = nb[q];
             double3 dd;
             const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
@@ -1170,8 +1222,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
             if (eval) ec += P.qlb[ab] / sqrt(r2);
         }
-        D.f_lr[p] = make_double4((double)fx, (double)fy, (double)fz, 0.0);
-        if (eval) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
+        const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
+        if (lr && valid && sub == 0) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);
+        if (eval) {
+            ec = group_sum(ec);
+            if (lr && valid && sub == 0) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
+        }
     }
 }
 
