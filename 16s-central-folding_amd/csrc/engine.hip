@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/tis_gpu.h"
@@ -429,6 +430,29 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         if (2 * c->max_site_lds() > 150 * 1024) throw Fail{TIS_EINVAL, "too many HB sites per replica"};
         HIP_TRY(hipFuncSetAttribute((const void *)k_hbsites, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(2 * c->max_site_lds())));
+        // bonded / 1-3 exclusions: every bond (i, k) and angle end pair (i, k) has its
+        // WCA removed once per term; the step path leaves the pair's WCA out of the
+        // pair sum and removes it again only for a second or later term of the same pair
+        {
+            std::vector<int> ex((size_t)N * kExcl, -1);
+            std::vector<unsigned char> bs(bij.size() + aijk.size(), 0);
+            std::map<std::pair<int, int>, int> seen;
+            auto add = [&](int i, int k, size_t term) {
+                const std::pair<int, int> key(std::min(i, k), std::max(i, k));
+                if (seen[key]++ > 0) { bs[term] = 1; return; }
+                for (int a : {i, k}) {
+                    const int b = a == i ? k : i;
+                    int e = 0;
+                    while (e < kExcl && ex[(size_t)a * kExcl + e] >= 0) ++e;
+                    if (e == kExcl) throw Fail{TIS_EINVAL, "a particle has more than 8 bonded / 1-3 partners"};
+                    ex[(size_t)a * kExcl + e] = b;
+                }
+            };
+            for (size_t q = 0; q < bij.size(); ++q) add(bij[q].x, bij[q].y, q);
+            for (size_t q = 0; q < aijk.size(); ++q) add(aijk[q].x, aijk[q].z, bij.size() + q);
+            D.excl = dalloc<int>(own, ex.size()); upload(D.excl, ex);
+            D.bsub = dalloc<unsigned char>(own, std::max<size_t>(bs.size(), 1)); upload(D.bsub, bs);
+        }
         c->n_bonded_terms = (int)bij.size() + (int)aijk.size();
         c->nb_bonded = c->grid(c->n_bonded_terms);
         D.bond_ij = dalloc<int2>(own, bij.size()); upload(D.bond_ij, bij);
@@ -764,7 +788,8 @@ extern "C" int tis_get_pairs(tis_ctx *c, int rep, int which, int *ij_out, long l
         for (int i = 0; i < P.n; ++i) {
             const int b = st[i] + (which ? ns[i] : 0), n = which ? nl[i] : ns[i];
             for (int q = 0; q < n; ++q) {
-                const int j = pool[b + q] - P.p0;
+                const int e = pool[b + q];
+                const int j = (e < 0 ? ~e : e) - P.p0;
                 if (i < j) pr.push_back({i, j});
             }
         }

@@ -13,6 +13,7 @@ namespace tis {
 constexpr int kNT = 9;            // parameter types
 constexpr int kMaxCells = 8192;   // cells per replica (LDS counting sort)
 constexpr int kBlock = 256;
+constexpr int kExcl = 8;          // bonded and 1-3 partners per particle
 
 // Static parameters of one replica (read-only on the device).
 struct RepParams {
@@ -76,6 +77,8 @@ struct Dev {
     int *prep;                    // replica of each particle
     // cells and lists
     int *cell_of, *cell_rank, *cell_start, *sorted;
+    int *excl;                    // kExcl bonded / 1-3 partners per particle (global, -1 none)
+    unsigned char *bsub;          // per bonded term: subtract its pair's WCA in the step path too
     int *nb_start, *nb_nsr, *nb_nlr;
     int *pool;
     // topology

@@ -214,6 +214,11 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
                 continue;
             }
             int *out = D.pool + P.pool0 + base;
+            // bonded / 1-3 partners of p: their SR entries are stored as ~j, so
+            // the step path leaves out their WCA term
+            int ex[kExcl];
+#pragma unroll
+            for (int e = 0; e < kExcl; ++e) ex[e] = D.excl[kExcl * p + e];
             int a = 0, b = nsr;
             for_neighbour_cells_wave(D, P, cell, [&](int j) {
                 bool isr = false, ilr = false;
@@ -224,7 +229,10 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
                 }
                 const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
                 const unsigned long long below = (1ull << lane) - 1ull;
-                if (isr) out[a + __popcll(ms & below)] = j;
+                bool is_ex = false;
+#pragma unroll
+                for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
+                if (isr) out[a + __popcll(ms & below)] = is_ex ? ~j : j;
                 if (ilr) out[b + __popcll(ml & below)] = j;
                 a += __popcll(ms);
                 b += __popcll(ml);
@@ -453,7 +461,8 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
 }
 
 // Uniform source of one step's HB resolution, with the replica's counters
-// hoisted into registers: Philox draw idx, or the caller-fed stream at its cursor.
+// hoisted into registers.  Draw idx is word idx % 4 of Philox block idx / 4
+// (counter = step, block, tag), or value cur + idx of the caller-fed stream.
 struct HbDraws {
     int rng;
     uint32_t c0, c1, k0, k1;
@@ -469,150 +478,114 @@ struct HbDraws {
         cur = S.stream_cursor;
         lim = rng ? D.stream_len[r] : 0;
     }
-    TIS_DEV double at(long long idx, bool *ovf) const {
-        if (rng == 0) return u01_closed_open(philox4x32_10(c0, c1, (uint32_t)idx, kTagHb, k0, k1).v[0]);
-        const long long c = cur + idx;
-        if (c >= lim) { *ovf = true; return 0.5; }
-        return (double)stream[c] / kRandMax;
+    // draws available (stream mode); draws at or past it read 0.5 and the caller
+    // reports the exhausted stream
+    TIS_DEV long long avail() const { return rng ? lim - cur : (1ll << 62); }
+    // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4); threads tid, tid + nt, ...
+    TIS_DEV void fill(double *ubuf, long long lo, int n, int tid, int nt) const {
+        if (rng == 0) {
+            for (int q = tid; q < n / 4; q += nt) {
+                const Philox4 blk = philox4x32_10(c0, c1, (uint32_t)((lo >> 2) + q), kTagHb, k0, k1);
+                for (int w = 0; w < 4; ++w) ubuf[4 * q + w] = u01_closed_open(blk.v[w]);
+            }
+        } else {
+            for (int i = tid; i < n; i += nt) {
+                const long long c = cur + lo + i;
+                ubuf[i] = c < lim ? (double)stream[c] / kRandMax : 0.5;
+            }
+        }
     }
 };
 
 TIS_DEV int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-TIS_DEV double readlane_d(double v, int l) {
+TIS_DEV unsigned long long readlane_u64(unsigned long long v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+TIS_DEV double readlane_d(double v, int l) { return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l)); }
+TIS_DEV double shfl_d(double v, int l) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    const int lo = __shfl((int)(b & 0xffffffffLL), l, 64), hi = __shfl((int)(b >> 32), l, 64);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Valence resolution with the whole working set in registers of one
-// wavefront: slot (row, lane) holds one over-subscribed site in site order, its
-// live candidates (static HB_ATOM order, multiplicity kept) and its excess.
-// One pass: pick the j-th over-subscribed site, shuffle its live candidates,
-// walk them keeping k and moving to j with probability exp((E_j - E_k)/T), switch
-// the pick off everywhere it is listed.  Returns false (nothing changed) when a
-// site has more than KMAX live candidates or there are more than 64*ROWS sites.
-template <int ROWS, int KMAX>
-TIS_DEV bool resolve_in_registers(int nA0, const short *A0, const int *excess, const short *aoff,
-                                  const short *alist, unsigned char *flag, const double *E, double T,
-                                  const HbDraws &dr, int *iters_out, long long *base_out, bool *ovf) {
-    static_assert(KMAX * KMAX <= 64 && 2 * KMAX <= 64, "pairwise table must fit one wavefront");
-    const int lane = threadIdx.x & 63;
-    if (nA0 > 64 * ROWS) return false;
-    int bnd[ROWS][KMAX];
-    int exc[ROWS];
-    unsigned long long act[ROWS];
-    bool too_long = false;
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const int idx = r * 64 + lane;
-        exc[r] = 0;
-#pragma unroll
-        for (int s = 0; s < KMAX; ++s) bnd[r][s] = -1;
-        if (idx < nA0) {
-            const int site = A0[idx];
-            exc[r] = excess[site];
-            int k = 0;
-            for (int q = aoff[site]; q < aoff[site + 1]; ++q) {
-                const int a = alist[q];
-                if (flag[a] != FLAG_LIVE) continue;
-#pragma unroll
-                for (int s = 0; s < KMAX; ++s)
-                    if (s == k) bnd[r][s] = a;
-                ++k;
-            }
-            too_long |= k > KMAX;
-        }
-        act[r] = __ballot(idx < nA0 && exc[r] > 0);
-    }
-    if (__ballot(too_long)) return false;
-    int nA = 0;
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) nA += __popcll(act[r]);
-    long long base = 0;
-    int iters = 0;
-    while (nA > 0) {
-        ++iters;
-        const double u = lane < 2 * KMAX ? dr.at(base + lane, ovf) : 0.0;
-        const double u0 = readlane_d(u, 0);
-        const int j = ceil_index(u0, nA) - 1;
-        // slot of the j-th over-subscribed site
-        int row = 0, jr = j, cum = 0;
-        unsigned long long m = 0;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const int c = __popcll(act[r]);
-            if (j >= cum && j < cum + c) { row = r; jr = j - cum; m = act[r]; }
-            cum += c;
-        }
-        const bool here = ((m >> lane) & 1ull) && __popcll(m & ((1ull << lane) - 1ull)) == jr;
-        const int ls = __ffsll((long long)__ballot(here)) - 1;
-        // its live candidates, compacted into lanes 0..nK-1 in list order
-        int v = -1, nK = 0;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            if (r != row) continue;
-#pragma unroll
-            for (int s = 0; s < KMAX; ++s) {
-                const int b = readlane_i(bnd[r][s], ls);
-                if (b >= 0) {
-                    if (lane == nK) v = b;
-                    ++nK;
-                }
-            }
-        }
-        if (nK == 0) { *ovf = true; break; }
-        // shuffle: swap i with ceil(u nK) - 1, i = 0..nK-1 (draws 1..nK)
-        for (int i = 0; i < nK; ++i) {
-            const int jj = ceil_index(readlane_d(u, 1 + i), nK) - 1;
-            const int bi = readlane_i(v, i), bj = readlane_i(v, jj);
-            if (lane == i) v = bj;
-            else if (lane == jj) v = bi;
-        }
-        // all pairwise acceptance probabilities at once: lane i*KMAX + c holds
-        // exp((E_i - E_c) / T), the reference's expression for candidate i against
-        // the current pick c
-        const double Ev = lane < nK ? E[v] : 0.0;
-        const double Ei = __shfl(Ev, lane / KMAX, 64), Ec = __shfl(Ev, lane % KMAX, 64);
-        const double Pv = (lane / KMAX < nK && lane % KMAX < nK) ? exp((Ei - Ec) / T) : 0.0;
-        int k = 0;
-        for (int i = 1; i < nK; ++i) {
-            const double ui = readlane_d(u, nK + i);   // selection draws follow the shuffle draws
-            if (ui < readlane_d(Pv, i * KMAX + k)) k = i;
-        }
-        const int off = readlane_i(v, k);
-        if (lane == 0) flag[off] = FLAG_OFF;
-        nA = 0;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-#pragma unroll
-            for (int s = 0; s < KMAX; ++s)
-                if (bnd[r][s] == off) { bnd[r][s] = -1; --exc[r]; }
-            act[r] &= __ballot(exc[r] > 0);
-            nA += __popcll(act[r]);
-        }
-        base += 2 * nK;
-    }
-    *iters_out = iters;
-    *base_out = base;
-    return true;
+// Inclusive prefix sum over the 64 lanes of a wavefront: Hillis-Steele within
+// rows of 16 (row_shr 1, 2, 4, 8), then row 0's and rows 0-1's totals carried
+// into the rows above them (row_bcast 15 into rows 1 and 3, row_bcast 31 into
+// rows 2 and 3).  Lanes with no source read the old operand, 0.
+TIS_DEV int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
 }
 
-// The same resolution with its working set in LDS (any number of sites and
-// up to 64 live candidates per site); used when the register form does not fit.
-TIS_DEV void resolve_in_lds(int nA, short *A, short *B, int *excess, const short *aoff, const short *alist,
-                            const short *hoff, const short *hinc, unsigned char *flag, const double *E, double T,
-                            const HbDraws &dr, int *iters_out, long long *base_out, bool *ovf) {
+constexpr int kDrawWin = 1024;     // draws held in LDS by the resolution
+constexpr int kOmaskWords = 64;    // over-subscription bitmask: 64 x 64 active sites
+static_assert(kASiteMax <= 64 * kOmaskWords, "active sites must fit the bitmask");
+static_assert(kDrawWin * 8 + kOmaskWords * 8 <= kASiteMax * 4, "draw window and bitmask alias lab[]");
+
+// Valence resolution, the reference's sequential process (fast.cpp:2239-2315),
+// run by one wavefront.  Loop: pick the j-th over-subscribed site in site order
+// (j = ceil(u nA)), shuffle its live candidates (static HB_ATOM order,
+// multiplicity kept) with nK draws, walk them keeping k and moving to i when
+// u < exp((E_i - E_k)/T), switch the pick off and lower the excess of every site
+// it is listed at.  The over-subscribed sites are a bitmask (lane w holds sites
+// 64w..64w+63), so the pick is a wavefront prefix count; all acceptance tests
+// of a pass are evaluated at once (lane t holds pair (i, c), c < i, triangular
+// index t = i(i-1)/2 + c) and the walk reads their bits.  Draws come from an LDS
+// window refilled by the wavefront.  Sets *ovf on a site with more than 64 live
+// candidates or an inconsistent table.
+TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *aoff, const short *alist,
+                          const short *hoff, const short *hinc, unsigned char *flag, const double *E, double T,
+                          double *ubuf, const HbDraws &dr, int n_as, int *iters_out, long long *base_out,
+                          bool *ovf) {
     const int lane = threadIdx.x & 63;
-    long long base = 0;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // pairs (i, c) of passes 0..3 handled by this lane
+    int pi[4], pc[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int t = 64 * p + lane;
+        int i = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)t)) * 0.5f);
+        while (i * (i - 1) / 2 > t) --i;
+        while ((i + 1) * i / 2 <= t) ++i;
+        pi[p] = i;
+        pc[p] = t - i * (i - 1) / 2;
+    }
+    unsigned long long w = omask[lane];
+    long long base = 0, win_lo = 0;
     int iters = 0;
-    while (nA > 0) {
-        __builtin_amdgcn_wave_barrier();
+    for (;;) {
+        const int cnt = __popcll(w);
+        const int incl = wave_incl_scan(cnt);
+        const int nA = readlane_i(incl, 63);
+        if (nA == 0) break;
         if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
-        const double u_lo = dr.at(base + lane, ovf);
-        const double u_hi = dr.at(base + 64 + lane, ovf);
-        const double u0 = readlane_d(u_lo, 0);
-        const int site = A[ceil_index(u0, nA) - 1];
+        if (base + 128 > win_lo + kDrawWin) {
+            win_lo = base & ~3ll;
+            dr.fill(ubuf, win_lo, kDrawWin, lane, 64);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        const double *u = ubuf + (base - win_lo);
+        // ---- the j-th over-subscribed site
+        const int j = ceil_index(u[0], nA) - 1;
+        const unsigned long long hit = __ballot(incl > j);
+        if (hit == 0) { *ovf = true; break; }
+        const int L = __builtin_ctzll(hit);
+        const int jj = j - (readlane_i(incl, L) - readlane_i(cnt, L));
+        const unsigned long long wl = readlane_u64(w, L);
+        const unsigned long long hb = __ballot(((wl >> lane) & 1ull) && __popcll(wl & lt_mask) == jj);
+        if (hb == 0) { *ovf = true; break; }
+        const int site = 64 * L + __builtin_ctzll(hb);
+        if (site >= n_as) { *ovf = true; break; }
+        // ---- its live candidates, compacted to lanes 0..nK-1 in static order
         const int o0 = aoff[site], len = aoff[site + 1] - o0;
         if (len > 64) { *ovf = true; break; }
         int bond = -1;
@@ -621,48 +594,60 @@ TIS_DEV void resolve_in_lds(int nA, short *A, short *B, int *excess, const short
         const unsigned long long km = __ballot(keep);
         const int nK = __popcll(km);
         if (nK == 0) { *ovf = true; break; }
-        // forward permute: a kept lane sends its bond to lane pos, the others to
-        // lane 63, which is never read (nK == 64 only if every lane keeps)
-        const int pos = __popcll(km & ((1ull << lane) - 1ull));
+        // a kept lane sends its bond to lane pos, the others to lane 63, which is
+        // never read (nK == 64 only if every lane keeps)
+        const int pos = __popcll(km & lt_mask);
         int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
+        // ---- shuffle: step i swaps i with ceil(u_{1+i} nK) - 1; targets computed at once
+        const int Jl = lane < nK ? ceil_index(u[1 + lane], nK) - 1 : 0;
         for (int i = 0; i < nK; ++i) {
-            const int di = 1 + i;
-            const double ui = di < 64 ? readlane_d(u_lo, di) : readlane_d(u_hi, di - 64);
-            const int jj = ceil_index(ui, nK) - 1;
-            const int bi = readlane_i(bv, i), bj = readlane_i(bv, jj);
+            const int tj = readlane_i(Jl, i);
+            const int bi = readlane_i(bv, i), bj = readlane_i(bv, tj);
             if (lane == i) bv = bj;
-            else if (lane == jj) bv = bi;
+            else if (lane == tj) bv = bi;
         }
         const double Ev = lane < nK ? E[bv] : 0.0;
-        int kk = 0;
-        double Ek = readlane_d(Ev, 0);
-        for (int i = 1; i < nK; ++i) {
-            const int di = nK + i;
-            const double ui = di < 64 ? readlane_d(u_lo, di) : readlane_d(u_hi, di - 64);
-            const double Ei = readlane_d(Ev, i);
-            const double pr = exp((Ei - Ek) / T);
-            if (ui < pr) { kk = i; Ek = Ei; }
+        // ---- walk: k moves to i when u_{nK+i} < exp((E_i - E_k)/T)
+        int k = 0;
+        const int npair = nK * (nK - 1) / 2;
+        if (npair <= 256) {
+            unsigned long long m[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                if (64 * p < npair) {
+                    const int i = pi[p] < nK ? pi[p] : 0, c = pi[p] < nK ? pc[p] : 0;
+                    const double Ei = shfl_d(Ev, i), Ec = shfl_d(Ev, c);
+                    const bool acc = pi[p] < nK && u[nK + i] < exp((Ei - Ec) / T);
+                    m[p] = __ballot(acc);
+                }
+            }
+            for (int i = 1; i < nK; ++i) {
+                const int t = i * (i - 1) / 2 + k;
+                const unsigned long long mt = t < 128 ? (t < 64 ? m[0] : m[1]) : (t < 192 ? m[2] : m[3]);
+                if ((mt >> (t & 63)) & 1ull) k = i;
+            }
+        } else {
+            double Ek = readlane_d(Ev, 0);
+            for (int i = 1; i < nK; ++i) {
+                const double Ei = readlane_d(Ev, i);
+                if (u[nK + i] < exp((Ei - Ek) / T)) { k = i; Ek = Ei; }
+            }
         }
-        const int off_bond = readlane_i(bv, kk);
+        const int off = readlane_i(bv, k);
         base += 2 * nK;
-        if (lane == 0) flag[off_bond] = FLAG_OFF;
-        const int h0 = hoff[off_bond], hn = hoff[off_bond + 1] - h0;
-        for (int q = lane; q < hn; q += 64) atomicSub(&excess[hinc[h0 + q]], 1);
+        // ---- switch it off; sites whose excess reaches 0 leave the bitmask
+        if (lane == 0) flag[off] = FLAG_OFF;
+        const int h0 = hoff[off], hn = hoff[off + 1] - h0;
+        for (int q = lane; q < hn; q += 64) {
+            const int st = hinc[h0 + q];
+            if (atomicSub(&excess[st], 1) == 1) atomicAnd(&omask[st >> 6], ~(1ull << (st & 63)));
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        int nB = 0;
-        for (int s0 = 0; s0 < nA; s0 += 64) {
-            const int e = s0 + lane;
-            const int sv = e < nA ? A[e] : 0;
-            const bool kp = e < nA && excess[sv] > 0;
-            const unsigned long long m = __ballot(kp);
-            if (kp) B[nB + __popcll(m & ((1ull << lane) - 1ull))] = (short)sv;
-            nB += __popcll(m);
-        }
-        short *t = A; A = B; B = t;
-        nA = nB;
+        w = omask[lane];
     }
+    if (base > dr.avail()) *ovf = true;
     *iters_out = iters;
     *base_out = base;
 }
@@ -892,24 +877,26 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     // been shown yet.  Both random sources use the exact sequential process.
     constexpr bool kUseComponents = false;
     if (D.rng == 1 || !kUseComponents) {
-        // reference stream: the exact sequential process, one wavefront
+        // the exact sequential process, one wavefront; the draw window and the
+        // bitmask live in the LDS of lab[], which only the component form uses
+        double *ubuf = (double *)lab;
+        unsigned long long *omask = (unsigned long long *)(ubuf + kDrawWin);
+        HbDraws dr(D, r, S);
+        dr.fill(ubuf, 0, kDrawWin, threadIdx.x, blockDim.x);
+        for (int q = threadIdx.x; q < kOmaskWords; q += blockDim.x) omask[q] = 0;
+        __syncthreads();
+        for (int s0 = 0; s0 < n_as; s0 += blockDim.x) {
+            const int s = s0 + threadIdx.x;
+            const unsigned long long m = __ballot(s < n_as && excess[s] > 0);
+            if ((threadIdx.x & 63) == 0 && s < n_as) omask[s >> 6] = m;
+        }
+        __syncthreads();
         if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            int nA = 0;
-            for (int s0 = 0; s0 < n_as; s0 += 64) {
-                const int s = s0 + lane;
-                const bool keep = s < n_as && excess[s] > 0;
-                const unsigned long long m = __ballot(keep);
-                if (keep) A0[nA + __popcll(m & ((1ull << lane) - 1ull))] = (short)s;
-                nA += __popcll(m);
-            }
-            HbDraws dr(D, r, S);
             int iters = 0;
             long long base = 0;
             bool ovf = false;
-            if (!resolve_in_registers<8, 8>(nA, A0, excess, aoff, alist, flag, E, P.T, dr, &iters, &base, &ovf))
-                resolve_in_lds(nA, A0, A1, excess, aoff, alist, hoff, hinc, flag, E, P.T, dr, &iters, &base, &ovf);
-            if (lane == 0) {
+            resolve_wave(excess, omask, aoff, alist, hoff, hinc, flag, E, P.T, ubuf, dr, n_as, &iters, &base, &ovf);
+            if (threadIdx.x == 0) {
                 sh_iters = iters;
                 sh_draws = base;
                 if (ovf) sh_err = 1;
@@ -1134,7 +1121,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         } else {
             return;
         }
-        // remove the WCA force of the bonded / 1-3 pair (reduce_forces, fast.cpp:880-897)
+        // remove the WCA force of the bonded / 1-3 pair (reduce_forces, fast.cpp:880-897).
+        // The step path leaves that pair's WCA out of the pair sum instead, which is
+        // the same force without the cancellation of two terms that diverge as the
+        // shifted WCA distance r + 1.5852 - D_LJ goes to 0; a pair listed by several
+        // terms is still removed once per extra term.
+        if (!eval && !D.bsub[g]) return;   // energies are only summed in evaluations
         const int r = D.prep[i];
         const RepParams &P = D.rp[r];
         double3 dd;
@@ -1176,12 +1168,17 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         R fx = 0, fy = 0, fz = 0;
         double ew = 0, ec = 0;
         for (int q = sub; q < nsr; q += kPairLanes) {
-            const int j = nb[q];
+            const int e = nb[q];
+            const bool bonded = e < 0;
+            const int j = bonded ? ~e : e;
             double3 dd;
             const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
             const int ptj = D.ptype[j];
             const int ab = ti * kNT + (ptj & 15);
-            R f = wca_factor<R>(r2, ab, P);
+            // a bonded / 1-3 pair carries no net WCA force (the reference adds it
+            // here and removes it in the bonded terms); only evaluations, which
+            // report the reference's split by term, compute it
+            R f = (bonded && !eval) ? (R)0 : wca_factor<R>(r2, ab, P);
             const bool both = chg_i && ((ptj >> 8) & 1) && r2 <= kCoulCut2;
             if (both) {
                 const R rr = sqrt((R)r2);

@@ -120,6 +120,33 @@ def test_philox_run_is_stable(lib, sys1):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
+def test_one_three_pair_at_the_wca_singularity_stays_finite(lib, sys1, prec):
+    """P 833 and B 835 (a 1-3 pair through S 834) placed 1e-4 A beyond the point
+    where the shifted WCA distance r + 1.5852 - D_LJ (D_LJ = 3.2) vanishes.  The
+    reference adds that pair's WCA in the pair sum and removes it in the bonded
+    terms; at this distance both are ~1e58 (inf in FP32).  The step path leaves the
+    pair out of the sum, so the trajectory stays finite.  A weak P-S-B angle
+    (k = 5 kcal/mol/rad^2) lets a batched FP32 run reach this point."""
+    d = sys1.d
+    pos, old, vel = (d[k].reshape(-1, 3).copy() for k in ("pos", "pos_old", "vel"))
+    assert tuple(d["type"][833:836]) == (0, 1, 4)
+    u = pos[833] - pos[835]
+    u /= np.linalg.norm(u)
+    shift = pos[835] + u * (3.2 - 1.5852 + 1e-4) - pos[833]
+    pos[833] += shift
+    old[833] += shift
+    eng = T.Engine(lib, [sys1], precision=prec, use_graph=0)
+    eng.set_state(pos, old, vel)
+    f, _ = eng.eval(T.TERM_SR)          # the reference's split: the pair's own WCA
+    assert not np.isfinite(np.linalg.norm(f[833])) or np.linalg.norm(f[833]) > 1e40
+    eng.step(20)
+    p, _, v = eng.get_state()
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    assert np.linalg.norm(p[833] - p[835]) < 6.0
+    eng.close()
+
+
 def test_batched_replicas_equal_single_runs(lib):
     """Three uneven replicas in one context give what each gives alone
     (reference-stream mode, FP64: same arithmetic, same draws)."""
