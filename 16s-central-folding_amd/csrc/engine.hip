@@ -180,12 +180,12 @@ struct tis_ctx {
         std::vector<RepState> s(n_rep);
         HIP_TRY(hipMemcpy(s.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
         for (int r = 0; r < n_rep; ++r) {
-            if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour pool overflow in replica " + std::to_string(r)};
+            if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour slot overflow (more SR/LR partners than the slot set up at creation) in replica " + std::to_string(r)};
             if (s[r].overflow & 2) throw Fail{TIS_EOVERFLOW, "HB working set above the LDS caps in replica " + std::to_string(r)};
             if (s[r].overflow & 4) throw Fail{TIS_EOVERFLOW, "uniform stream exhausted in replica " + std::to_string(r)};
             if (s[r].overflow & 16)
                 throw Fail{TIS_EOVERFLOW, "HB valence resolution: a site has more live candidates than the kernel holds "
-                                          "(64 in the sequential form, 8 in the component form) in replica " + std::to_string(r)};
+                                          "(64 in the sequential form, 32 in the component form) in replica " + std::to_string(r)};
         }
     }
 };
@@ -360,6 +360,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.cell_rank = dalloc<int>(own, N);
         D.cell_start = dalloc<int>(own, cell0);
         D.sorted = dalloc<int>(own, N);
+        D.cpos = dalloc<double4>(own, N);
         D.nb_start = dalloc<int>(own, N);
         D.nb_nsr = dalloc<int>(own, N);
         D.nb_nlr = dalloc<int>(own, N);
@@ -505,7 +506,6 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.maxwell = dalloc<double>(own, 3000 * (size_t)n_rep);
         D.stream_base = dalloc<long long>(own, n_rep);
         D.stream_len = dalloc<long long>(own, n_rep);
-        D.done_counter = dalloc<unsigned int>(own, 1);
         c->stream_cap.assign(n_rep, 0);
         upload(D.ptype, ptype);
         upload(D.prep, prep);
@@ -525,12 +525,18 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         k_bin<<<n_rep, 1024, 0, c->s_main>>>(D);
         k_list<<<kListWaves * 64 / kBlock, kBlock, 0, c->s_main>>>(D, 0);
         HIP_TRY(hipStreamSynchronize(c->s_main));
+        // fixed per-particle slots: twice the largest count at set-up plus margin
+        // (compaction under added salt raises the LR counts of the RNA core)
         long long pool_total = 0;
         for (int r = 0; r < n_rep; ++r) {
             RepState s = c->get_rs(r);
             RepParams &P = c->rp[r];
+            P.cap_sr = (2 * s.max_sr + 64 + 15) / 16 * 16;
+            P.cap_lr = (2 * s.max_lr + 128 + 15) / 16 * 16;
+            const long long stride = (long long)P.cap_sr + P.cap_lr;
+            if (stride * P.n > 0x7fffffffLL) throw Fail{TIS_EINVAL, "neighbour slots of one replica exceed 2^31 entries"};
             P.pool0 = pool_total;
-            P.pool_cap = 3 * s.pool_cursor + 64LL * P.n;
+            P.pool_cap = stride * P.n;
             pool_total += P.pool_cap;
         }
         D.pool = dalloc<int>(own, (size_t)pool_total);
@@ -780,13 +786,11 @@ extern "C" int tis_get_pairs(tis_ctx *c, int rep, int which, int *ij_out, long l
         HIP_TRY(hipMemcpy(st.data(), c->D.nb_start + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(ns.data(), c->D.nb_nsr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(nl.data(), c->D.nb_nlr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
-        RepState s = c->get_rs(rep);
-        std::vector<int> pool((size_t)s.pool_cursor);
-        if (s.pool_cursor)
-            HIP_TRY(hipMemcpy(pool.data(), c->D.pool + P.pool0, pool.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> pool((size_t)P.pool_cap);
+        HIP_TRY(hipMemcpy(pool.data(), c->D.pool + P.pool0, pool.size() * sizeof(int), hipMemcpyDeviceToHost));
         std::vector<std::pair<int, int>> pr;
         for (int i = 0; i < P.n; ++i) {
-            const int b = st[i] + (which ? ns[i] : 0), n = which ? nl[i] : ns[i];
+            const int b = st[i] + (which ? P.cap_sr : 0), n = which ? nl[i] : ns[i];
             for (int q = 0; q < n; ++q) {
                 const int e = pool[b + q];
                 const int j = (e < 0 ? ~e : e) - P.p0;

@@ -24,7 +24,8 @@ struct RepParams {
     int M[3];
     double lc[3];
     int cell0;                    // offset into the cell-start array (ncell + 1 entries)
-    long long pool0, pool_cap;    // neighbour-pool region
+    long long pool0, pool_cap;    // neighbour-pool region: n slots of cap_sr + cap_lr entries
+    int cap_sr, cap_lr;           // SR and LR entries per particle slot
     int bond0, nbond, ang0, nang;
     int st30, nst3, st20, nst2;
     int hb0, nhb, site0, nsite, inc0, ninc;  // HB candidates, sites, site incidences
@@ -43,7 +44,6 @@ struct RepState {
     int need_rebuild;
     int lr_due;
     long long n_rebuilds;
-    long long pool_cursor;
     int n_act;                    // live HB candidates after the last rebuild
     int n_asite;                  // sites with at least one live candidate
     int n_ainc;                   // entries of the per-site live lists
@@ -51,6 +51,7 @@ struct RepState {
     long long stream_cursor;
     long long hb_draws;           // Philox draws of the current step's HB resolution
     int hb_iters;
+    int max_sr, max_lr;           // largest per-particle SR / LR counts of the last build
     int pad;
     double e[12];                 // energies of the last force evaluation
 };
@@ -77,6 +78,7 @@ struct Dev {
     int *prep;                    // replica of each particle
     // cells and lists
     int *cell_of, *cell_rank, *cell_start, *sorted;
+    double4 *cpos;                // positions in cell order, .w = particle index
     int *excl;                    // kExcl bonded / 1-3 partners per particle (global, -1 none)
     unsigned char *bsub;          // per bonded term: subtract its pair's WCA in the step path too
     int *nb_start, *nb_nsr, *nb_nlr;
@@ -116,7 +118,6 @@ struct Dev {
     double *maxwell;              // 3000 per replica (stream mode)
     int32_t *stream;              // raw glibc values, region per replica
     long long *stream_base, *stream_len;
-    unsigned int *done_counter;
 };
 
 }  // namespace tis

@@ -8,8 +8,8 @@
 //                 (stacking_interactions :1911-2141, polymer_constraints :2354-2520,
 //                 SR_/LR_interactions_lists :1783-1907)
 //   k_integrate   noise, leap-frog with speed cap, recentring, ion wrap, displacement
-//                 reduction and rebuild decision (full_forces :2590-2625,
-//                 move_rigid_units :2660-2732, check_shifts :1741-1776)
+//                 reduction (full_forces :2590-2625, move_rigid_units :2660-2732)
+//   k_commit      new state into place, rebuild decision (check_shifts :1741-1776)
 //   k_bin         LDS counting sort of particles into cells (separate_in_cells :1461-1542)
 //   k_list        full neighbour lists, SR and LR, from the 27 neighbouring cells
 //                 (populate_lists :1546-1700)
@@ -23,6 +23,19 @@ namespace tis {
 
 // ------------------------------------------------------------ block helpers
 TIS_DEV int lane_id() { return threadIdx.x & 63; }
+TIS_DEV int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+TIS_DEV unsigned long long readlane_u64(unsigned long long v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+TIS_DEV double readlane_d(double v, int l) { return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l)); }
+TIS_DEV double shfl_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __shfl((int)(b & 0xffffffffLL), l, 64), hi = __shfl((int)(b >> 32), l, 64);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 TIS_DEV int wave_id() { return threadIdx.x >> 6; }
 
 // Ordered exclusive prefix of a per-thread count across the block; *total gets
@@ -137,46 +150,86 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
     __syncthreads();
     for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
         const int p = P.p0 + i;
-        D.sorted[P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p]] = p;
+        const int q = P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p];
+        const double4 x = D.pos[p];
+        D.sorted[q] = p;
+        D.cpos[q] = make_double4(x.x, x.y, x.z, (double)p);
     }
     for (int q = threadIdx.x; q < P.nsite; q += blockDim.x) D.site_cnt[P.site0 + q] = 0;
-    if (threadIdx.x == 0) S.pool_cursor = 0;
+    if (threadIdx.x == 0) {
+        S.max_sr = 0;
+        S.max_lr = 0;
+    }
 }
 
-// ------------------------------------------------------------ k_list
-// One wavefront per particle: the lanes test 64 candidates of a cell at a time,
-// so the dependent index/position loads of many candidates are in flight
-// together (a thread-per-particle loop is latency-bound on the dense RNA cells).
-// The grid is fixed (kListWaves wavefronts) and strides over the particles of
-// the replicas whose displacement trigger fired; the others cost one flag read.
-// mode 0: count only (sizes the pool), mode 1: build.
-constexpr int kListWaves = 4096;
-
-template <class F>
-TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
+// ------------------------------------------------------------ neighbour cells
+// The <= 27 cells around `cell` (fewer when a dimension has < 3 cells), one per
+// lane: lane c < *ncell gets the cell's first cell-order index and length.
+TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ncell, int *cstart, int *clen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
     const int lo0 = M0 >= 3 ? -1 : 0, hi0 = M0 >= 2 ? 1 : 0;
     const int lo1 = M1 >= 3 ? -1 : 0, hi1 = M1 >= 2 ? 1 : 0;
     const int lo2 = M2 >= 3 ? -1 : 0, hi2 = M2 >= 2 ? 1 : 0;
+    const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1, w2 = hi2 - lo2 + 1;
+    *ncell = w0 * w1 * w2;
     const int lane = threadIdx.x & 63;
-    for (int a = lo0; a <= hi0; ++a) {
+    *cstart = 0;
+    *clen = 0;
+    if (lane < *ncell) {
+        const int a = lane / (w1 * w2) + lo0, b = (lane / w2) % w1 + lo1, c = lane % w2 + lo2;
         int n0 = c0 + a; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0;
-        for (int b = lo1; b <= hi1; ++b) {
-            int n1 = c1 + b; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0;
-            for (int c = lo2; c <= hi2; ++c) {
-                int n2 = c2 + c; n2 += n2 < 0 ? M2 : 0; n2 -= n2 >= M2 ? M2 : 0;
-                const int nc = (n0 * M1 + n1) * M2 + n2;
-                const int s = D.cell_start[P.cell0 + nc], e = D.cell_start[P.cell0 + nc + 1];
-                for (int q0 = s; q0 < e; q0 += 64) {
-                    const int q = q0 + lane;
-                    visit(q < e ? D.sorted[P.p0 + q] : -1);
-                }
-            }
-        }
+        int n1 = c1 + b; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0;
+        int n2 = c2 + c; n2 += n2 < 0 ? M2 : 0; n2 -= n2 >= M2 ? M2 : 0;
+        const int nc = (n0 * M1 + n1) * M2 + n2;
+        *cstart = D.cell_start[P.cell0 + nc];
+        *clen = D.cell_start[P.cell0 + nc + 1] - *cstart;
     }
 }
 
+// Visits every particle of the neighbour cells, 64 per call of visit(j, x),
+// lanes past a cell's end getting j = -1.  Two chunks are in flight: the load of
+// the next chunk is issued before the current one is tested.
+template <class F>
+TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
+    const int lane = threadIdx.x & 63;
+    int ncell, cstart, clen;
+    neighbour_cells(D, P, cell, &ncell, &cstart, &clen);
+    const double4 *cp = D.cpos + P.p0;
+    // scalar cursor (cell c, offset o within it); advance skips empty cells
+    int c = 0, o = 0;
+    auto skip_empty = [&]() {
+        while (c < ncell && o >= readlane_i(clen, c)) { ++c; o = 0; }
+    };
+    auto load = [&](int cc, int oo, double4 *x) {
+        const int s = readlane_i(cstart, cc), n = readlane_i(clen, cc);
+        const bool ok = oo + lane < n;
+        *x = ok ? cp[s + oo + lane] : make_double4(0, 0, 0, -1.0);
+    };
+    skip_empty();
+    if (c >= ncell) return;
+    double4 cur;
+    load(c, o, &cur);
+    for (;;) {
+        o += 64;
+        skip_empty();
+        const bool more = c < ncell;
+        double4 nxt = make_double4(0, 0, 0, -1.0);
+        if (more) load(c, o, &nxt);
+        visit((int)cur.w, cur);
+        if (!more) break;
+        cur = nxt;
+    }
+}
+
+// ------------------------------------------------------------ k_list
+// One wavefront per particle: the lanes test 64 candidates at a time, two
+// chunks in flight.  Every particle owns a fixed slot of the pool (cap_sr SR
+// then cap_lr LR entries), so there is no counting pass and no shared cursor.
+// The grid is fixed (kListWaves wavefronts) and strides over the particles of
+// the replicas whose displacement trigger fired; the others cost one flag read.
+// mode 0: record the largest per-particle counts only (sizes the slots), mode 1: build.
+constexpr int kListWaves = 4096;
 __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
     const int lane = threadIdx.x & 63;
     const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -185,62 +238,50 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
         RepState &S = D.rs[r];
         if (!S.need_rebuild) continue;
         const RepParams &P = D.rp[r];
+        const long long stride = (long long)P.cap_sr + P.cap_lr;
         for (int t = w0; t < P.n; t += nw) {
-            const int p = D.sorted[P.p0 + t];
-            const double4 xi = D.pos[p];
+            const double4 xi = D.cpos[P.p0 + t];
+            const int p = (int)xi.w;
             const bool chg_i = (D.ptype[p] >> 8) & 1;
             const int cell = D.cell_of[p];
-            int nsr = 0, nlr = 0;
-            for_neighbour_cells_wave(D, P, cell, [&](int j) {
-                bool isr = false, ilr = false;
-                if (j >= 0 && j != p) {
-                    const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
-                    isr = r2 <= P.rsr2;
-                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1);
-                }
-                nsr += __popcll(__ballot(isr));
-                nlr += __popcll(__ballot(ilr));
-            });
-            long long base = 0;
-            if (lane == 0)
-                base = (long long)atomicAdd((unsigned long long *)&S.pool_cursor, (unsigned long long)(nsr + nlr));
-            base = __shfl(base, 0, 64);
-            if (mode == 0) continue;
-            if (base + nsr + nlr > P.pool_cap) {
-                if (lane == 0) {
-                    atomicOr(&S.overflow, 1);
-                    D.nb_start[p] = 0; D.nb_nsr[p] = 0; D.nb_nlr[p] = 0;
-                }
-                continue;
-            }
-            int *out = D.pool + P.pool0 + base;
             // bonded / 1-3 partners of p: their SR entries are stored as ~j, so
             // the step path leaves out their WCA term
             int ex[kExcl];
 #pragma unroll
             for (int e = 0; e < kExcl; ++e) ex[e] = D.excl[kExcl * p + e];
-            int a = 0, b = nsr;
-            for_neighbour_cells_wave(D, P, cell, [&](int j) {
+            int *out_sr = D.pool + P.pool0 + (long long)(p - P.p0) * stride;
+            int *out_lr = out_sr + P.cap_sr;
+            int a = 0, b = 0;
+            for_neighbour_cells_wave(D, P, cell, [&](int j, const double4 &xj) {
                 bool isr = false, ilr = false;
                 if (j >= 0 && j != p) {
-                    const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL);
+                    const double r2 = min_image_r2(xi, xj, P.L, P.hL);
                     isr = r2 <= P.rsr2;
                     ilr = !isr && chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1);
                 }
                 const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
-                const unsigned long long below = (1ull << lane) - 1ull;
-                bool is_ex = false;
+                if (mode == 1) {
+                    const unsigned long long below = (1ull << lane) - 1ull;
+                    bool is_ex = false;
 #pragma unroll
-                for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
-                if (isr) out[a + __popcll(ms & below)] = is_ex ? ~j : j;
-                if (ilr) out[b + __popcll(ml & below)] = j;
+                    for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
+                    const int ka = a + __popcll(ms & below), kb = b + __popcll(ml & below);
+                    if (isr && ka < P.cap_sr) out_sr[ka] = is_ex ? ~j : j;
+                    if (ilr && kb < P.cap_lr) out_lr[kb] = j;
+                }
                 a += __popcll(ms);
                 b += __popcll(ml);
             });
             if (lane == 0) {
-                D.nb_start[p] = (int)base;
-                D.nb_nsr[p] = nsr;
-                D.nb_nlr[p] = nlr;
+                if (mode == 0) {
+                    atomicMax(&S.max_sr, a);
+                    atomicMax(&S.max_lr, b);
+                } else {
+                    if (a > P.cap_sr || b > P.cap_lr) atomicOr(&S.overflow, 1);
+                    D.nb_start[p] = (int)((p - P.p0) * stride);
+                    D.nb_nsr[p] = min(a, P.cap_sr);
+                    D.nb_nlr[p] = min(b, P.cap_lr);
+                }
             }
         }
     }
@@ -496,19 +537,6 @@ struct HbDraws {
         }
     }
 };
-
-TIS_DEV int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-TIS_DEV unsigned long long readlane_u64(unsigned long long v, int l) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
-    return ((unsigned long long)hi << 32) | lo;
-}
-TIS_DEV double readlane_d(double v, int l) { return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l)); }
-TIS_DEV double shfl_d(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __shfl((int)(b & 0xffffffffLL), l, 64), hi = __shfl((int)(b >> 32), l, 64);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 // Inclusive prefix sum over the 64 lanes of a wavefront: Hillis-Steele within
 // rows of 16 (row_shr 1, 2, 4, 8), then row 0's and rows 0-1's totals carried
@@ -1208,8 +1236,9 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         R fx = 0, fy = 0, fz = 0;
         double ec = 0;
         const int n_lr = lr ? nlr : 0;
-        for (int q = nsr + sub; q < nsr + n_lr; q += kPairLanes) {
-            const int j = nb[q];
+        const int *nb_lr = nb + P.cap_sr;
+        for (int q = sub; q < n_lr; q += kPairLanes) {
+            const int j = nb_lr[q];
             double3 dd;
             const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
             if (r2 > kCoulCut2) continue;
@@ -1276,7 +1305,6 @@ TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S
 
 __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ bool last;
     if (p < D.n_tot) {
         const int r = D.prep[p];
         const RepParams &P = D.rp[r];
@@ -1315,31 +1343,28 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         D.old[D.n_tot + p] = make_double4(on[0], on[1], on[2], 0.0);
         D.vel[D.n_tot + p] = make_double4(vn[0], vn[1], vn[2], 0.0);
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(D.done_counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
-        RepState &S = D.rs[r];
-        const RepParams &P = D.rp[r];
-        const unsigned long long bits = atomicAdd(&S.maxd2, 0ull);
-        const double m = sqrt(__longlong_as_double((long long)bits));
-        S.progress += m + m;
-        S.need_rebuild = S.progress > P.skin;
-        if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
-        S.maxd2 = 0;
-        S.klok += 1;
-        S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
-    }
-    if (threadIdx.x == 0) *D.done_counter = 0;
 }
 
 // Copies the second state buffer set into the first (keeps one buffer layout
-// for every other kernel; cheap: 3 x 32 B per particle).
+// for every other kernel; cheap: 3 x 32 B per particle).  Workgroup 0 also
+// closes the step of every replica: the list-rebuild decision from the largest
+// displacement k_integrate recorded (check_shifts, fast.cpp:1741-1776), the step
+// counter and the LR refresh flag.
 __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
+            RepState &S = D.rs[r];
+            const RepParams &P = D.rp[r];
+            const double m = sqrt(__longlong_as_double((long long)S.maxd2));
+            S.progress += m + m;
+            S.need_rebuild = S.progress > P.skin;
+            if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+            S.maxd2 = 0;
+            S.klok += 1;
+            S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
+        }
+    }
     if (p >= D.n_tot) return;
     D.pos[p] = D.pos[D.n_tot + p];
     D.old[p] = D.old[D.n_tot + p];
