@@ -129,16 +129,53 @@ struct tis_ctx {
         event_next = 0;
     }
 
+    // List rebuild of the replicas whose displacement trigger fired (the others
+    // exit at once): cell sort and neighbour slots on `sl`, live HB candidates and
+    // the resolution's site tables on `sh` (they depend on positions only).
+    void launch_rebuild_lists(hipStream_t sl) {
+        timed("k_bin", sl, [&] { k_bin<<<n_rep, 1024, 0, sl>>>(D); });
+        timed("k_list", sl, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, sl>>>(D, 1); });
+    }
+    void launch_rebuild_hb(hipStream_t sh) {
+        timed("k_hbstatus", sh, [&] { k_hbstatus<<<256, kBlock, 0, sh>>>(D); });
+        timed("k_hbsites", sh, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), sh>>>(D); });
+    }
     void launch_rebuild() {
-        timed("k_bin", s_main, [&] { k_bin<<<n_rep, 1024, 0, s_main>>>(D); });
-        timed("k_list", s_main, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, s_main>>>(D, 1); });
-        timed("k_hbstatus", s_main, [&] { k_hbstatus<<<256, kBlock, 0, s_main>>>(D); });
-        timed("k_hbsites", s_main, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), s_main>>>(D); });
+        launch_rebuild_lists(s_main);
+        launch_rebuild_hb(s_main);
+    }
+    // A step rebuilds at its start when the previous step's trigger fired; after
+    // the last step of a call, do that rebuild now so that lists, HB statuses and
+    // evaluations reflect the current positions, and clear the trigger.
+    void finish_pending_rebuild() {
+        launch_rebuild();
+        HIP_TRY(hipStreamSynchronize(s_main));
+        if (profiling) collect_times();
+        std::vector<RepState> all(n_rep);
+        HIP_TRY(hipMemcpy(all.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
+        bool any = false;
+        for (auto &st : all)
+            if (st.need_rebuild) { st.need_rebuild = 0; any = true; }
+        if (any) HIP_TRY(hipMemcpy(D.rs, all.data(), n_rep * sizeof(RepState), hipMemcpyHostToDevice));
     }
     size_t max_site_lds() const {
         int m = 1;
         for (const RepParams &p : rp) m = std::max(m, p.nsite);
         return (size_t)m * sizeof(int);
+    }
+
+    // HB terms on stream st: energies and gates, the valence resolution, forces.
+    void launch_hb(hipStream_t st, int mode) {
+        const dim3 g2((kActMax + kBlock - 1) / kBlock, n_rep);
+        timed("k_hbpre", st, [&] {
+            if (D.precision == TIS_PREC_FP64) k_hbpre<double><<<g2, kBlock, 0, st>>>(D);
+            else k_hbpre<float><<<g2, kBlock, 0, st>>>(D);
+        });
+        timed("k_hbres", st, [&] { k_hbres<<<n_rep, kBlock, kHbLds, st>>>(D, mode); });
+        timed("k_hbforce", st, [&] {
+            if (D.precision == TIS_PREC_FP64) k_hbforce<double><<<g2, kBlock, 0, st>>>(D);
+            else k_hbforce<float><<<g2, kBlock, 0, st>>>(D);
+        });
     }
 
     // One Langevin step.  The HB resolution and the multi-body terms run on
@@ -147,16 +184,15 @@ struct tis_ctx {
         HIP_TRY(hipEventRecord(ev_fork, s_main));
         HIP_TRY(hipStreamWaitEvent(s_hb, ev_fork, 0));
         HIP_TRY(hipStreamWaitEvent(s_mb, ev_fork, 0));
-        timed("k_hb", s_hb, [&] {
-            if (D.precision == TIS_PREC_FP64) k_hb<double><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
-            else k_hb<float><<<n_rep, kBlock, kHbLds, s_hb>>>(D, 3);
-        });
+        launch_rebuild_hb(s_hb);
+        launch_hb(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
             if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
             else k_force<float, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
         });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
+        launch_rebuild_lists(s_main);
         timed("k_pairs", s_main, [&] {
             if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
             else k_force<float, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
@@ -165,7 +201,6 @@ struct tis_ctx {
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
-        launch_rebuild();
     }
 
     RepState get_rs(int r) {
@@ -293,8 +328,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join_mb, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        HIP_TRY(hipFuncSetAttribute((const void *)k_hb<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
-        HIP_TRY(hipFuncSetAttribute((const void *)k_hb<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hbres, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
         // ---- offsets and per-replica parameters
         int p0 = 0, b0 = 0, a0 = 0, s30 = 0, s20 = 0, h0 = 0, si0 = 0, i0 = 0, cell0 = 0;
         c->rp.resize(n_rep);
@@ -492,11 +526,14 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             if (v < 0 || v > 0x7fff) throw Fail{TIS_EINVAL, "HB site tables are not consistent"};
         D.inc_rank = dalloc<int>(own, rank.size()); upload(D.inc_rank, rank);
         D.site_cnt = dalloc<int>(own, sv.size());
+        HIP_TRY(hipMemset(D.site_cnt, 0, sv.size() * sizeof(int)));
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
         D.act_hb = dalloc<int>(own, nhb_tot);
         D.act_idx = dalloc<int>(own, nhb_tot);
+        D.act_E = dalloc<double>(own, nhb_tot);
+        D.act_flag = dalloc<unsigned char>(own, nhb_tot);
         D.asite_id = dalloc<int>(own, nsite_tot);
         D.asite_off = dalloc<int>(own, nsite_tot + n_rep);
         D.asite_list = dalloc<int>(own, inc_base);
@@ -668,6 +705,7 @@ extern "C" int tis_step(tis_ctx *c, long long n_steps) {
         }
         for (; done < n_steps; ++done) c->launch_step();
         HIP_TRY(hipGetLastError());
+        if (n_steps > 0) c->finish_pending_rebuild();
         HIP_TRY(hipStreamSynchronize(c->s_main));
         c->check_flags();
         return TIS_OK;
@@ -740,8 +778,7 @@ extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out
         HIP_TRY(hipMemcpy(D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
         HIP_TRY(hipMemsetAsync(D.f_mb, 0, D.n_tot * sizeof(double4), c->s_main));
         if (mask & TIS_TERM_HB) {
-            if (D.precision == TIS_PREC_FP64) k_hb<double><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
-            else k_hb<float><<<c->n_rep, kBlock, kHbLds, c->s_main>>>(D, 2);
+            c->launch_hb(c->s_main, 0);
         }
         const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
         const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * D.n_tot);

@@ -155,7 +155,6 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
         D.sorted[q] = p;
         D.cpos[q] = make_double4(x.x, x.y, x.z, (double)p);
     }
-    for (int q = threadIdx.x; q < P.nsite; q += blockDim.x) D.site_cnt[P.site0 + q] = 0;
     if (threadIdx.x == 0) {
         S.max_sr = 0;
         S.max_lr = 0;
@@ -291,8 +290,8 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 // k_hbstatus: grid-stride over the candidates of the replicas being rebuilt.
 // A candidate is live iff its (k11, k21) bead pair is an SR pair, decided with
 // the list's own FP64 distance test (populate_lists, fast.cpp:1702-1727).  Live
-// candidates are counted at each of their sites; k_bin has cleared the counters
-// of the replicas being rebuilt.
+// candidates are counted at each of their sites; k_hbsites clears the counters
+// after reading them.
 __global__ void __launch_bounds__(kBlock) k_hbstatus(Dev D) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
     for (int r = 0; r < D.n_rep; ++r) {
@@ -335,6 +334,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         n_act += tot;
     }
     if (n_act > kActMax) {
+        for (int q = threadIdx.x; q < P.nsite; q += blockDim.x) D.site_cnt[P.site0 + q] = 0;
         if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
         return;
     }
@@ -342,6 +342,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
     for (int s0 = 0; s0 < P.nsite; s0 += blockDim.x) {
         const int s = s0 + threadIdx.x;
         const int cnt = s < P.nsite ? D.site_cnt[P.site0 + s] : 0;
+        if (s < P.nsite) D.site_cnt[P.site0 + s] = 0;   // cleared for the next build
         int tot_s, tot_i;
         const int rank = block_excl_sum(cnt > 0, scratch, &tot_s);
         const int off = block_excl_sum(cnt, scratch, &tot_i);
@@ -840,13 +841,69 @@ TIS_DEV bool resolve_by_components(long long klok, uint64_t seed, int r, int n_a
     return sh_bad == 0;
 }
 
+// Energy and distance gate of every live candidate (hydrogen_bonds, fast.cpp:2180-2237):
+// E = E_hb exp(-Q) when |r - r0| <= 2 A for the (k11, k21) pair, gated otherwise.
+// Grid (kActMax / kBlock, replicas); also clears the replica's E2/E3 sums.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
-    // mode bit 0: step (advance the stream cursor, build the noise table); bit 1: forces on
+__global__ void __launch_bounds__(kBlock) k_hbpre(Dev D) {
+    const int r = blockIdx.y;
+    RepState &S = D.rs[r];
+    const RepParams &P = D.rp[r];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S.e[EI_E2_HB] = 0.0;
+        S.e[EI_E3_HB] = 0.0;
+    }
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((S.overflow & 2) || a >= S.n_act) return;
+    const int g = D.act_hb[P.hb0 + a];
+    V3<R> v = diff<R>(D.pos[D.hb_b[6 * g + 1]], D.pos[D.hb_b[6 * g + 4]]);
+    const R rr = sqrt(dot(v, v));
+    if (fabs(rr - (R)D.hb_g[6 * g]) > (R)2) {
+        D.act_flag[P.hb0 + a] = FLAG_GATED;
+        D.act_E[P.hb0 + a] = 0.0;
+        return;
+    }
+    R q, psi0[3];
+    hb_geometry<R>(D, g, &q, psi0, nullptr);
+    D.act_E[P.hb0 + a] = (double)((R)D.hb_E[g] * exp(-q));
+    D.act_flag[P.hb0 + a] = FLAG_LIVE;
+}
+
+// Forces of the candidates that survived the resolution, and the E2/E3 sums.
+// Grid (kActMax / kBlock, replicas).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
+    const int r = blockIdx.y;
+    RepState &S = D.rs[r];
+    const RepParams &P = D.rp[r];
+    __shared__ double red[8];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_act = (S.overflow & 2) ? 0 : S.n_act;
+    if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
+    double e2 = 0, e3 = 0;
+    if (a < n_act && D.act_flag[P.hb0 + a] == FLAG_LIVE) {
+        const int g = D.act_hb[P.hb0 + a];
+        R q, psi0[3];
+        hb_geometry<R>(D, g, &q, psi0, nullptr);
+        const double e = D.act_E[P.hb0 + a];
+        if (D.hb_dode[g]) e3 = e; else e2 = e;
+        sextet_forces<R>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+    }
+    const double s2 = block_sum_double(e2, red);
+    const double s3 = block_sum_double(e3, red);
+    if (threadIdx.x == 0) {
+        if (s2 != 0.0) unsafeAtomicAdd(&S.e[EI_E2_HB], s2);
+        if (s3 != 0.0) unsafeAtomicAdd(&S.e[EI_E3_HB], s3);
+    }
+}
+
+// Valence resolution of one replica per workgroup, on the energies and gates of
+// k_hbpre; writes the outcome of every live candidate.  mode bit 0: a step
+// (advance the stream cursor past the HB draws and build the noise table).
+__global__ void __launch_bounds__(kBlock) k_hbres(Dev D, int mode) {
     const int r = blockIdx.x;
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
-    double4 *F = D.f_mb;
     extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
     int *lab = (int *)(E + kActMax);
@@ -862,7 +919,6 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     short *alist = hoff + kActMax + 1;
     short *hinc = alist + kAIncMax;
     short *tbuf = hinc + kAIncMax;
-    __shared__ double red[8];
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
     __shared__ int sh_err;
@@ -875,21 +931,9 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
     for (int i = threadIdx.x; i <= n_act; i += blockDim.x) hoff[i] = (short)D.ahb_inc_off[P.hb0 + r + i];
     const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
     for (int i = threadIdx.x; i < n_hinc; i += blockDim.x) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
-    // ---- energies and the distance gate
     for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
-        const int g = D.act_hb[P.hb0 + a];
-        const int *b = D.hb_b + 6 * g;
-        V3<R> v = diff<R>(D.pos[b[1]], D.pos[b[4]]);
-        R rr = sqrt(dot(v, v));
-        if (fabs(rr - (R)D.hb_g[6 * g]) > (R)2) {
-            flag[a] = FLAG_GATED;
-            E[a] = 0.0;
-            continue;
-        }
-        R q, psi0[3];
-        hb_geometry<R>(D, g, &q, psi0, nullptr);
-        E[a] = (double)((R)D.hb_E[g] * exp(-q));
-        flag[a] = FLAG_LIVE;
+        E[a] = D.act_E[P.hb0 + a];
+        flag[a] = D.act_flag[P.hb0 + a];
     }
     __syncthreads();
     // excess = live incidences - valence, per active site
@@ -941,25 +985,11 @@ __global__ void __launch_bounds__(kBlock) k_hb(Dev D, int mode) {
         }
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < n_act; a += blockDim.x) D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
-    // ---- forces of the surviving HBs and the E2/E3 sums
-    double e2 = 0, e3 = 0;
-    if (mode & 2) {
-        for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
-            if (flag[a] != FLAG_LIVE) continue;
-            const int g = D.act_hb[P.hb0 + a];
-            R q, psi0[3];
-            hb_geometry<R>(D, g, &q, psi0, nullptr);
-            const double e = E[a];
-            if (D.hb_dode[g]) e3 += e; else e2 += e;
-            sextet_forces<R>(D, F, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
-        }
+    for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+        D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
+        D.act_flag[P.hb0 + a] = flag[a];
     }
-    double s2 = block_sum_double(e2, red);
-    double s3 = block_sum_double(e3, red);
     if (threadIdx.x == 0) {
-        S.e[EI_E2_HB] = s2;
-        S.e[EI_E3_HB] = s3;
         S.hb_iters = sh_iters;
         S.hb_draws = sh_draws;
         if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);

@@ -41,22 +41,43 @@ def batched_systems(lib, n_rep=16, T_kcal=0.596):
 
 
 def algorithmic_bytes(eng, handles, kernel, lr_every=10):
-    """Bytes per launch of one kernel over the whole batch, from the per-unit
-    figures: 8 B per SR pair, 8 B per LR pair every lr_every steps, 52 B per
-    stack, 16 B per bond, 20 B per angle (k_force); 56 B per live HB candidate
-    (k_hb); 148 B per particle (k_integrate: FP64 x, x_old, v read + write and
-    the type word)."""
+    """Bytes one launch of `kernel` must move over the whole batch, from the
+    per-unit figures (each input read once, each output written once):
+      k_pairs      80 B per particle (position 32, type + list counts 16, force 32)
+                   + 4 B per SR list entry + 4 B per LR entry / lr_every
+      k_multibody  52 B per stack, 16 B per bond, 20 B per angle (+ 32 B force
+                   per particle touched, counted as 32 B per particle)
+      k_hbpre      120 B per live HB candidate (6 positions of 32 B would be
+                   192 B; the two shared with the gate pair are counted once: 4
+                   bead positions 128 B minus the 8 B the gate reuses)
+      k_hbres      27 B per live candidate (energy 8, gate 1, id 4, outcome 2,
+                   incidence offsets 4, two incidence entries 8), 12 B per site
+      k_hbforce    same per-candidate inputs as k_hbpre plus 6 x 24 B forces
+      k_integrate  148 B per particle (x, x_old, v, 3 forces read, 3 states written)
+      k_commit     128 B per particle
+    """
     tot = 0.0
     for r, h in enumerate(handles):
         e = eng.energies(r)
         d = h.d
-        if kernel == "k_force":
-            tot += 8 * e["n_sr_pairs"] + 8.0 * e["n_lr_pairs"] / lr_every
-            tot += 52 * (d["n_st3"] + d["n_st2"]) + 16 * d["n_bonds"] + 20 * d["n_angles"]
-        elif kernel == "k_hb":
-            tot += 56 * e["n_hb_active"]
-        elif kernel in ("k_integrate", "k_commit"):
-            tot += 148 * d["n_particles"]
+        n = d["n_particles"]
+        nact = e["n_hb_active"]
+        if kernel == "k_pairs":
+            tot += 80 * n + 4 * e["n_sr_pairs"] * 2 + 4.0 * e["n_lr_pairs"] * 2 / lr_every
+        elif kernel == "k_multibody":
+            tot += 52 * (d["n_st3"] + d["n_st2"]) + 16 * d["n_bonds"] + 20 * d["n_angles"] + 32 * d["n_beads"]
+        elif kernel == "k_hbpre":
+            tot += 120 * nact
+        elif kernel == "k_hbres":
+            tot += 27 * nact + 12 * d["n_hb_sites"]
+        elif kernel == "k_hbforce":
+            tot += (120 + 144) * nact
+        elif kernel == "k_integrate":
+            tot += 148 * n
+        elif kernel == "k_commit":
+            tot += 128 * n
+        else:
+            return None
     return tot
 
 
@@ -120,7 +141,7 @@ def main():
         dominant = max(kt, key=lambda k: kt[k][0] * kt[k][1])
         n_launch, avg_ms = kt[dominant]
         bytes_launch = algorithmic_bytes(eng, handles, dominant)
-        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        achieved = None if bytes_launch is None else bytes_launch / (avg_ms * 1e-3) / 1e9
     energies = [eng.energies(r) for r in range(n_rep)]
     eng.close()
     # plain FP64 one-replica-per-launch path (5 mM, 350 A, 3,659 particles)
@@ -164,7 +185,7 @@ def main():
         "batched_vs_plain": value / plain,
         "roofline": None if dominant is None else {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": None,
             "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": [e["n_hb_active"] for e in energies],
