@@ -171,7 +171,7 @@ struct tis_ctx {
             if (D.precision == TIS_PREC_FP64) k_hbpre<double><<<g2, kBlock, 0, st>>>(D);
             else k_hbpre<float><<<g2, kBlock, 0, st>>>(D);
         });
-        timed("k_hbres", st, [&] { k_hbres<<<n_rep, kBlock, kHbLds, st>>>(D, mode); });
+        timed("k_hbres", st, [&] { k_hbres<<<n_rep, kResBlock, kHbLds, st>>>(D, mode); });
         timed("k_hbforce", st, [&] {
             if (D.precision == TIS_PREC_FP64) k_hbforce<double><<<g2, kBlock, 0, st>>>(D);
             else k_hbforce<float><<<g2, kBlock, 0, st>>>(D);
@@ -220,7 +220,7 @@ struct tis_ctx {
             if (s[r].overflow & 4) throw Fail{TIS_EOVERFLOW, "uniform stream exhausted in replica " + std::to_string(r)};
             if (s[r].overflow & 16)
                 throw Fail{TIS_EOVERFLOW, "HB valence resolution: a site has more live candidates than the kernel holds "
-                                          "(64 in the sequential form, 32 in the component form) in replica " + std::to_string(r)};
+                                          "(64) in replica " + std::to_string(r)};
         }
     }
 };

@@ -417,14 +417,20 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 }
 
 // ------------------------------------------------------------ k_hb
-// LDS layout of one replica's HB working set (bytes, in this order):
-//   E[kActMax] double | lab[kASiteMax] int | excess[kASiteMax] int | flag[kActMax] u8 |
-//   A0, A1, croot, csites [kASiteMax] short | aoff, coff [kASiteMax+1] short |
-//   hoff[kActMax+1] short | alist, hinc [kAIncMax] short | tbuf[kBlock * kKMax] short
-constexpr int kKMax = 32;  // live candidates per over-subscribed site (component mode)
-constexpr size_t kHbLds = kActMax * 8 + 2 * kASiteMax * 4 + kActMax + 4 * kASiteMax * 2 + 2 * (kASiteMax + 1) * 2 +
-                          (kActMax + 1) * 2 + 2 * kAIncMax * 2 + kBlock * kKMax * 2 + 64;
-static_assert(kHbLds <= 160 * 1024 - 512, "HB working set must fit the LDS of one CU");
+// Valence resolution: one workgroup of kResWaves wavefronts per replica.
+constexpr int kResWaves = 8;
+constexpr int kResBlock = 64 * kResWaves;
+constexpr int kSeqWin = 1024;      // draw window of the sequential form (one wavefront)
+constexpr int kCompWin = 256;      // draw window of each wavefront in the component form
+static_assert(kResWaves * kCompWin >= kSeqWin, "window regions are shared by the two forms");
+static_assert(kASiteMax <= 64 * 64, "over-subscribed sites must fit a 64 x 64 bit mask");
+// LDS layout of one replica's working set (bytes, in this order):
+//   E[kActMax] double | win[kResWaves * kCompWin] double | omask[kResWaves * 64] u64 |
+//   par, excess [kASiteMax] int | A0, A1, croot [kASiteMax] short |
+//   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag[kActMax] u8
+constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + kResWaves * 64 * 8 + 2 * kASiteMax * 4 +
+                          3 * kASiteMax * 2 + (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
+static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of one CU");
 
 enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
 
@@ -503,8 +509,9 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
 }
 
 // Uniform source of one step's HB resolution, with the replica's counters
-// hoisted into registers.  Draw idx is word idx % 4 of Philox block idx / 4
-// (counter = step, block, tag), or value cur + idx of the caller-fed stream.
+// hoisted into registers.  Philox: draw idx of stream `comp` is word idx % 4 of
+// block (comp << 20) + idx / 4 (counter = step, block, tag).  Stream: value
+// cur + idx of the caller-fed glibc stream (one stream, comp 0).
 struct HbDraws {
     int rng;
     uint32_t c0, c1, k0, k1;
@@ -524,11 +531,12 @@ struct HbDraws {
     // reports the exhausted stream
     TIS_DEV long long avail() const { return rng ? lim - cur : (1ll << 62); }
     // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4); threads tid, tid + nt, ...
-    TIS_DEV void fill(double *ubuf, long long lo, int n, int tid, int nt) const {
+    TIS_DEV void fill(double *ubuf, long long lo, int n, int tid, int nt, int comp) const {
         if (rng == 0) {
             for (int q = tid; q < n / 4; q += nt) {
-                const Philox4 blk = philox4x32_10(c0, c1, (uint32_t)((lo >> 2) + q), kTagHb, k0, k1);
-                for (int w = 0; w < 4; ++w) ubuf[4 * q + w] = u01_closed_open(blk.v[w]);
+                const uint32_t blk = ((uint32_t)comp << 20) + (uint32_t)((lo >> 2) + q);
+                const Philox4 b = philox4x32_10(c0, c1, blk, kTagHb, k0, k1);
+                for (int w = 0; w < 4; ++w) ubuf[4 * q + w] = u01_closed_open(b.v[w]);
             }
         } else {
             for (int i = tid; i < n; i += nt) {
@@ -553,26 +561,31 @@ TIS_DEV int wave_incl_scan(int x) {
     return x;
 }
 
-constexpr int kDrawWin = 1024;     // draws held in LDS by the resolution
-constexpr int kOmaskWords = 64;    // over-subscription bitmask: 64 x 64 active sites
-static_assert(kASiteMax <= 64 * kOmaskWords, "active sites must fit the bitmask");
-static_assert(kDrawWin * 8 + kOmaskWords * 8 <= kASiteMax * 4, "draw window and bitmask alias lab[]");
+TIS_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
-// Valence resolution, the reference's sequential process (fast.cpp:2239-2315),
-// run by one wavefront.  Loop: pick the j-th over-subscribed site in site order
-// (j = ceil(u nA)), shuffle its live candidates (static HB_ATOM order,
-// multiplicity kept) with nK draws, walk them keeping k and moving to i when
-// u < exp((E_i - E_k)/T), switch the pick off and lower the excess of every site
-// it is listed at.  The over-subscribed sites are a bitmask (lane w holds sites
-// 64w..64w+63), so the pick is a wavefront prefix count; all acceptance tests
-// of a pass are evaluated at once (lane t holds pair (i, c), c < i, triangular
-// index t = i(i-1)/2 + c) and the walk reads their bits.  Draws come from an LDS
-// window refilled by the wavefront.  Sets *ovf on a site with more than 64 live
-// candidates or an inconsistent table.
-TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *aoff, const short *alist,
-                          const short *hoff, const short *hinc, unsigned char *flag, const double *E, double T,
-                          double *ubuf, const HbDraws &dr, int n_as, int *iters_out, long long *base_out,
-                          bool *ovf) {
+// The reference's sequential valence resolution (fast.cpp:2239-2315) over a set
+// of over-subscribed sites, run by one wavefront.  Loop: pick the j-th
+// over-subscribed site of the set in site order (j = ceil(u nA)), shuffle its live
+// candidates (static HB_ATOM order, multiplicity kept) with nK draws, walk them
+// keeping k and moving to i when u < exp((E_i - E_k)/T), switch the pick off and
+// lower the excess of every site it is listed at.
+//
+// The set is the bit mask omask (64 words in LDS, word w held by lane w in a
+// register): bit b stands for the over-subscribed site A0[c0 + b], A1 maps a
+// site back to its index in A0 (-1 if it was not over-subscribed).  The pick is a
+// wavefront prefix count; all acceptance tests of a pass are evaluated at once
+// (lane t holds pair (i, c), c < i, triangular index t = i(i-1)/2 + c) and the
+// walk reads their bits.  Draws come from an LDS window of `win` values that the
+// wavefront refills.  Sets *ovf on a site with more than 64 live candidates or an
+// inconsistent table.
+TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA0, const short *A0, const short *A1,
+                          const short *aoff, const short *alist, const short *hoff, const short *hinc,
+                          unsigned char *flag, const double *E, double T, double *ubuf, int win, const HbDraws &dr,
+                          int comp, int *iters_out, long long *base_out, bool *ovf) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     // pairs (i, c) of passes 0..3 handled by this lane
@@ -589,21 +602,21 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *a
     unsigned long long w = omask[lane];
     long long base = 0, win_lo = 0;
     int iters = 0;
+    dr.fill(ubuf, 0, win, lane, 64, comp);
+    wave_lds_sync();
     for (;;) {
         const int cnt = __popcll(w);
         const int incl = wave_incl_scan(cnt);
         const int nA = readlane_i(incl, 63);
         if (nA == 0) break;
         if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
-        if (base + 128 > win_lo + kDrawWin) {
+        if (base + 128 > win_lo + win) {
             win_lo = base & ~3ll;
-            dr.fill(ubuf, win_lo, kDrawWin, lane, 64);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            dr.fill(ubuf, win_lo, win, lane, 64, comp);
+            wave_lds_sync();
         }
         const double *u = ubuf + (base - win_lo);
-        // ---- the j-th over-subscribed site
+        // ---- the j-th over-subscribed site of the set
         const int j = ceil_index(u[0], nA) - 1;
         const unsigned long long hit = __ballot(incl > j);
         if (hit == 0) { *ovf = true; break; }
@@ -612,8 +625,9 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *a
         const unsigned long long wl = readlane_u64(w, L);
         const unsigned long long hb = __ballot(((wl >> lane) & 1ull) && __popcll(wl & lt_mask) == jj);
         if (hb == 0) { *ovf = true; break; }
-        const int site = 64 * L + __builtin_ctzll(hb);
-        if (site >= n_as) { *ovf = true; break; }
+        const int ci = c0 + 64 * L + __builtin_ctzll(hb);
+        if (ci >= nA0) { *ovf = true; break; }
+        const int site = A0[ci];
         // ---- its live candidates, compacted to lanes 0..nK-1 in static order
         const int o0 = aoff[site], len = aoff[site + 1] - o0;
         if (len > 64) { *ovf = true; break; }
@@ -664,16 +678,17 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *a
         }
         const int off = readlane_i(bv, k);
         base += 2 * nK;
-        // ---- switch it off; sites whose excess reaches 0 leave the bitmask
+        // ---- switch it off; sites whose excess reaches 0 leave the mask
         if (lane == 0) flag[off] = FLAG_OFF;
         const int h0 = hoff[off], hn = hoff[off + 1] - h0;
         for (int q = lane; q < hn; q += 64) {
             const int st = hinc[h0 + q];
-            if (atomicSub(&excess[st], 1) == 1) atomicAnd(&omask[st >> 6], ~(1ull << (st & 63)));
+            if (atomicSub(&excess[st], 1) == 1) {
+                const int b = A1[st] - c0;
+                if (b >= 0 && b < 64 * 64) atomicAnd(&omask[b >> 6], ~(1ull << (b & 63)));
+            }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        wave_lds_sync();
         w = omask[lane];
     }
     if (base > dr.avail()) *ovf = true;
@@ -681,164 +696,27 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, const short *a
     *base_out = base;
 }
 
-// Valence resolution by independent components (production, Philox draws).
-//
-// Over-subscribed sites interact only through live candidates they share, and
-// a site whose excess is <= 0 can never be picked again.  Split the
-// over-subscribed sites into connected components of that sharing graph: an
-// iteration of the reference's process picks a uniformly random
-// over-subscribed site, so, conditioned on landing in component C, it is a
-// uniform pick among C's sites, and its update touches only C.  Running each
-// component's own process (same pick / shuffle / acceptance rules) therefore
-// gives the same joint distribution of switched-off candidates as the
-// sequential loop; the components run concurrently, one thread each, and the
-// serial depth drops from the total iteration count to the largest component's.
-// Draws: Philox keyed by (seed, replica), counter (step, component, iteration,
-// draw), so results do not depend on thread timing.  Returns false if a site
-// has more than kKMax live candidates.
-TIS_DEV bool resolve_by_components(long long klok, uint64_t seed, int r, int n_act, int n_as, int *excess, int *lab,
-                                   unsigned char *flag, short *A0, short *A1, short *croot, short *csites,
-                                   const short *aoff, short *coff, const short *alist, const short *hoff,
-                                   const short *hinc, short *tbuf, const double *E, double T, int *iters_out) {
-    __shared__ int scratch[20];
-    __shared__ int sh_changed, sh_rounds, sh_bad;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    // over-subscribed sites in site order; A1 maps an active site to its index or -1
-    int nA0 = 0;
-    for (int s0 = 0; s0 < n_as; s0 += nt) {
-        const int s = s0 + tid;
-        const bool c = s < n_as && excess[s] > 0;
-        int tot;
-        const int rank = block_excl_sum(c, scratch, &tot);
-        if (s < n_as) {
-            A1[s] = (short)(c ? nA0 + rank : -1);
-            if (c) A0[nA0 + rank] = (short)s;
-        }
-        nA0 += tot;
+// Union-find root with path halving; par[] lives in LDS and is hooked
+// concurrently (roots only ever move to smaller indices).
+TIS_DEV int uf_find(volatile int *par, int x) {
+    int p = par[x];
+    while (p != x) {
+        const int g = par[p];
+        if (g != p) par[x] = g;
+        x = p;
+        p = par[x];
     }
-    for (int ci = tid; ci < nA0; ci += nt) lab[ci] = ci;
-    if (tid == 0) { sh_rounds = 0; sh_bad = 0; }
-    __syncthreads();
-    // component labels: minimum index, by propagation over the live candidates.
-    // The barrier at the top of each pass keeps thread 0 from clearing the flag
-    // before every thread has read it for the previous pass's exit test.
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) sh_changed = 0;
-        __syncthreads();
-        for (int a = tid; a < n_act; a += nt) {
-            if (flag[a] != FLAG_LIVE) continue;
-            int m = 1 << 30, n = 0;
-            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = A1[hinc[q]];
-                if (ci >= 0) { m = min(m, lab[ci]); ++n; }
-            }
-            if (n < 2) continue;
-            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = A1[hinc[q]];
-                if (ci >= 0 && lab[ci] > m) { atomicMin(&lab[ci], m); sh_changed = 1; }
-            }
-        }
-        __syncthreads();
-        for (int ci = tid; ci < nA0; ci += nt) {   // pointer jumping (labels only decrease)
-            const int l = lab[ci], ll = lab[l];
-            if (ll < l) atomicMin(&lab[ci], ll);
-        }
-        __syncthreads();
-        if (!sh_changed) break;
+    return x;
+}
+TIS_DEV void uf_union(volatile int *par, int a, int b) {
+    int ra = uf_find(par, a), rb = uf_find(par, b);
+    while (ra != rb) {
+        if (ra < rb) { const int t = ra; ra = rb; rb = t; }   // hook the larger root under the smaller
+        const int old = atomicCAS((int *)&par[ra], ra, rb);
+        if (old == ra) break;
+        ra = uf_find(par, old);
+        rb = uf_find(par, rb);
     }
-    // components in order of their smallest site; members in site order
-    int ncomp = 0;
-    for (int c0 = 0; c0 < nA0; c0 += nt) {
-        const int ci = c0 + tid;
-        const bool root = ci < nA0 && lab[ci] == ci;
-        int tot;
-        const int rank = block_excl_sum(root, scratch, &tot);
-        if (root) croot[ncomp + rank] = (short)ci;
-        ncomp += tot;
-    }
-    __syncthreads();
-    int run = 0;
-    for (int k0 = 0; k0 < ncomp; k0 += nt) {
-        const int k = k0 + tid;
-        int cnt = 0;
-        if (k < ncomp)
-            for (int ci = croot[k]; ci < nA0; ++ci) cnt += lab[ci] == croot[k];
-        int tot;
-        const int off = block_excl_sum(cnt, scratch, &tot);
-        if (k < ncomp) coff[k] = (short)(run + off);
-        run += tot;
-    }
-    if (tid == 0) coff[ncomp] = (short)run;
-    __syncthreads();
-    for (int k = tid; k < ncomp; k += nt) {
-        int w = coff[k];
-        for (int ci = croot[k]; ci < nA0; ++ci)
-            if (lab[ci] == croot[k]) csites[w++] = (short)ci;
-    }
-    __syncthreads();
-    // each component's own process
-    const uint32_t k0 = (uint32_t)seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), k1 = (uint32_t)(seed >> 32);
-    short *tb = tbuf + tid * kKMax;
-    for (int k = tid; k < ncomp; k += nt) {
-        const int q0 = coff[k], q1 = coff[k + 1];
-        int it = 0;
-        for (;;) {
-            int nA = 0;
-            for (int q = q0; q < q1; ++q) nA += excess[A0[csites[q]]] > 0;
-            if (nA == 0) break;
-            // draw i of this iteration: Philox block i/4, word i%4
-            Philox4 blk = philox4x32_10((uint32_t)klok, (uint32_t)(klok >> 32),
-                                        ((uint32_t)k << 20) | ((uint32_t)it << 6), kTagHb, k0, k1);
-            int blk_id = 0;
-            auto draw = [&](int i) {
-                if ((i >> 2) != blk_id) {
-                    blk_id = i >> 2;
-                    blk = philox4x32_10((uint32_t)klok, (uint32_t)(klok >> 32),
-                                        ((uint32_t)k << 20) | ((uint32_t)it << 6) | (uint32_t)blk_id, kTagHb, k0, k1);
-                }
-                return u01_closed_open(blk.v[i & 3]);
-            };
-            int j = ceil_index(draw(0), nA) - 1;
-            int site = -1;
-            for (int q = q0; q < q1; ++q) {
-                const int s = A0[csites[q]];
-                if (excess[s] > 0) {
-                    if (j == 0) { site = s; break; }
-                    --j;
-                }
-            }
-            int nK = 0;
-            for (int q = aoff[site]; q < aoff[site + 1]; ++q) {
-                const int b = alist[q];
-                if (flag[b] != FLAG_LIVE) continue;
-                if (nK < kKMax) tb[nK] = (short)b;
-                ++nK;
-            }
-            if (nK > kKMax || nK == 0) { sh_bad = 1; break; }
-            for (int i = 0; i < nK; ++i) {
-                const int jj = ceil_index(draw(1 + i), nK) - 1;
-                const short t = tb[i];
-                tb[i] = tb[jj];
-                tb[jj] = t;
-            }
-            int kk = tb[0];
-            for (int i = 1; i < nK; ++i) {
-                const int jb = tb[i];
-                if (draw(nK + i) < exp((E[jb] - E[kk]) / T)) kk = jb;
-            }
-            flag[kk] = FLAG_OFF;
-            for (int q = hoff[kk]; q < hoff[kk + 1]; ++q) {
-                const int s = hinc[q];
-                if (A1[s] >= 0) excess[s] -= 1;
-            }
-            if (++it >= (1 << 14)) { sh_bad = 1; break; }
-        }
-        atomicMax(&sh_rounds, it);
-    }
-    __syncthreads();
-    *iters_out = sh_rounds;
-    return sh_bad == 0;
 }
 
 // Energy and distance gate of every live candidate (hydrogen_bonds, fast.cpp:2180-2237):
@@ -900,96 +778,162 @@ __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
 // Valence resolution of one replica per workgroup, on the energies and gates of
 // k_hbpre; writes the outcome of every live candidate.  mode bit 0: a step
 // (advance the stream cursor past the HB draws and build the noise table).
-__global__ void __launch_bounds__(kBlock) k_hbres(Dev D, int mode) {
+//
+// Reference stream: the exact sequential process on wavefront 0.  Philox draws:
+// the over-subscribed sites split into the connected components of the graph
+// whose edges are the live candidates they share (union-find), and each
+// component runs its own copy of the process on one wavefront, with its own
+// Philox stream.  An iteration of the reference's process picks a uniformly
+// random over-subscribed site and its update touches only that site's
+// component, so, conditioned on landing in component C, it is a uniform pick
+// among C's over-subscribed sites: the components' processes give the same joint
+// distribution of switched-off candidates as the single sequential loop, with
+// the serial depth of the largest wavefront's share instead of the total.
+__global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     const int r = blockIdx.x;
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
     extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
-    int *lab = (int *)(E + kActMax);
-    int *excess = lab + kASiteMax;
-    unsigned char *flag = (unsigned char *)(excess + kASiteMax);
-    short *A0 = (short *)(flag + kActMax);
+    double *win = E + kActMax;
+    unsigned long long *omask = (unsigned long long *)(win + kResWaves * kCompWin);
+    int *par = (int *)(omask + kResWaves * 64);
+    int *excess = par + kASiteMax;
+    short *A0 = (short *)(excess + kASiteMax);
     short *A1 = A0 + kASiteMax;
     short *croot = A1 + kASiteMax;
-    short *csites = croot + kASiteMax;
-    short *aoff = csites + kASiteMax;
-    short *coff = aoff + kASiteMax + 1;
-    short *hoff = coff + kASiteMax + 1;
-    short *alist = hoff + kActMax + 1;
+    short *aoff = croot + kASiteMax;
+    short *hoff = aoff + kASiteMax + 4;
+    short *alist = hoff + kActMax + 4;
     short *hinc = alist + kAIncMax;
-    short *tbuf = hinc + kAIncMax;
+    unsigned char *flag = (unsigned char *)(hinc + kAIncMax);
+    __shared__ int scratch[20];
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
     __shared__ int sh_err;
     const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
     if (S.overflow & 2) return;
     if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; }
-    // ---- preload the rebuild tables
-    for (int i = threadIdx.x; i <= n_as; i += blockDim.x) aoff[i] = (short)D.asite_off[P.site0 + r + i];
-    for (int i = threadIdx.x; i < n_ainc; i += blockDim.x) alist[i] = (short)D.asite_list[P.inc0 + i];
-    for (int i = threadIdx.x; i <= n_act; i += blockDim.x) hoff[i] = (short)D.ahb_inc_off[P.hb0 + r + i];
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    // ---- the rebuild tables, energies and gates
+    for (int i = tid; i <= n_as; i += nt) aoff[i] = (short)D.asite_off[P.site0 + r + i];
+    for (int i = tid; i < n_ainc; i += nt) alist[i] = (short)D.asite_list[P.inc0 + i];
+    for (int i = tid; i <= n_act; i += nt) hoff[i] = (short)D.ahb_inc_off[P.hb0 + r + i];
     const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
-    for (int i = threadIdx.x; i < n_hinc; i += blockDim.x) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
-    for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+    for (int i = tid; i < n_hinc; i += nt) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
+    for (int a = tid; a < n_act; a += nt) {
         E[a] = D.act_E[P.hb0 + a];
         flag[a] = D.act_flag[P.hb0 + a];
     }
     __syncthreads();
-    // excess = live incidences - valence, per active site
-    for (int s = threadIdx.x; s < n_as; s += blockDim.x) {
-        int c = 0;
-        for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
-        excess[s] = c - D.site_val[D.asite_id[P.site0 + s]];
+    // ---- excess = live incidences - valence per active site; the
+    // over-subscribed ones in site order (A0) and the inverse map (A1)
+    int nA0 = 0;
+    for (int s0 = 0; s0 < n_as; s0 += nt) {
+        const int s = s0 + tid;
+        int ex = 0;
+        if (s < n_as) {
+            int c = 0;
+            for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
+            ex = c - D.site_val[D.asite_id[P.site0 + s]];
+            excess[s] = ex;
+        }
+        int tot;
+        const int rank = block_excl_sum(ex > 0, scratch, &tot);
+        if (s < n_as) {
+            A1[s] = (short)(ex > 0 ? nA0 + rank : -1);
+            if (ex > 0) A0[nA0 + rank] = (short)s;
+        }
+        nA0 += tot;
     }
     __syncthreads();
-    // ---- valence resolution (fast.cpp:2239-2315)
-    // The component form (resolve_by_components) is switched off: a batched
-    // Philox run with it stopped making progress on the GPU and the cause has not
-    // been shown yet.  Both random sources use the exact sequential process.
-    constexpr bool kUseComponents = false;
-    if (D.rng == 1 || !kUseComponents) {
-        // the exact sequential process, one wavefront; the draw window and the
-        // bitmask live in the LDS of lab[], which only the component form uses
-        double *ubuf = (double *)lab;
-        unsigned long long *omask = (unsigned long long *)(ubuf + kDrawWin);
-        HbDraws dr(D, r, S);
-        dr.fill(ubuf, 0, kDrawWin, threadIdx.x, blockDim.x);
-        for (int q = threadIdx.x; q < kOmaskWords; q += blockDim.x) omask[q] = 0;
-        __syncthreads();
-        for (int s0 = 0; s0 < n_as; s0 += blockDim.x) {
-            const int s = s0 + threadIdx.x;
-            const unsigned long long m = __ballot(s < n_as && excess[s] > 0);
-            if ((threadIdx.x & 63) == 0 && s < n_as) omask[s >> 6] = m;
+    const HbDraws dr(D, r, S);
+    if (D.rng == 1) {
+        // reference stream: the exact sequential process on wavefront 0, over all
+        // over-subscribed sites
+        for (int q = tid; q < 64; q += nt) {
+            const int c = 64 * q;
+            unsigned long long m = 0;
+            for (int b = 0; b < 64 && c + b < nA0; ++b) m |= 1ull << b;
+            omask[q] = m;
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
+        if (wv == 0) {
             int iters = 0;
             long long base = 0;
             bool ovf = false;
-            resolve_wave(excess, omask, aoff, alist, hoff, hinc, flag, E, P.T, ubuf, dr, n_as, &iters, &base, &ovf);
-            if (threadIdx.x == 0) {
+            resolve_wave(excess, omask, 0, nA0, A0, A1, aoff, alist, hoff, hinc, flag, E, P.T, win, kSeqWin, dr, 0,
+                         &iters, &base, &ovf);
+            if (lane == 0) {
                 sh_iters = iters;
                 sh_draws = base;
                 if (ovf) sh_err = 1;
             }
         }
     } else {
-        int iters = 0;
-        const bool ok = resolve_by_components(S.klok, D.seed, r, n_act, n_as, excess, lab, flag, A0, A1, croot, csites,
-                                              aoff, coff, alist, hoff, hinc, tbuf, E, P.T, &iters);
-        if (threadIdx.x == 0) {
-            sh_iters = iters;
-            sh_draws = 0;
-            if (!ok) sh_err = 1;
+        // ---- components of the over-subscribed sites (union-find over A0 indices)
+        volatile int *vpar = par;
+        for (int ci = tid; ci < nA0; ci += nt) par[ci] = ci;
+        __syncthreads();
+        for (int a = tid; a < n_act; a += nt) {
+            if (flag[a] != FLAG_LIVE) continue;
+            int first = -1;
+            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
+                const int ci = A1[hinc[q]];
+                if (ci < 0) continue;
+                if (first < 0) first = ci;
+                else uf_union(vpar, first, ci);
+            }
+        }
+        __syncthreads();
+        for (int ci = tid; ci < nA0; ci += nt) par[ci] = uf_find(vpar, ci);
+        __syncthreads();
+        // roots (smallest index of each component) in order
+        int ncomp = 0;
+        for (int c0 = 0; c0 < nA0; c0 += nt) {
+            const int ci = c0 + tid;
+            const bool root = ci < nA0 && par[ci] == ci;
+            int tot;
+            const int rank = block_excl_sum(root, scratch, &tot);
+            if (root) croot[ncomp + rank] = (short)ci;
+            ncomp += tot;
+        }
+        __syncthreads();
+        // ---- each wavefront runs components wv, wv + kResWaves, ...
+        double *ub = win + wv * kCompWin;
+        unsigned long long *om = omask + wv * 64;
+        int wave_iters = 0;
+        bool ovf = false;
+        for (int k = wv; k < ncomp; k += kResWaves) {
+            const int root = croot[k], c0 = root & ~63;
+            // lane w: members of the component among A0 indices c0 + 64 w .. + 63
+            unsigned long long m = 0;
+            const int base_ci = c0 + 64 * lane;
+            for (int b = 0; b < 64; ++b) {
+                const int ci = base_ci + b;
+                if (ci >= nA0) break;
+                if (par[ci] == root) m |= 1ull << b;
+            }
+            om[lane] = m;
+            wave_lds_sync();
+            int it = 0;
+            long long base = 0;
+            resolve_wave(excess, om, c0, nA0, A0, A1, aoff, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k,
+                         &it, &base, &ovf);
+            wave_iters += it;
+            if (ovf) break;
+        }
+        if (lane == 0) {
+            atomicMax(&sh_iters, wave_iters);
+            if (ovf) sh_err = 1;
         }
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+    for (int a = tid; a < n_act; a += nt) {
         D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         S.hb_iters = sh_iters;
         S.hb_draws = sh_draws;
         if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);

@@ -209,16 +209,15 @@ def test_graph_replays_match_eager_steps(lib, sys1):
     b.close()
 
 
-@pytest.mark.skip(reason="component-parallel HB resolution is switched off: a batched Philox run with it "
-                         "stalled on the GPU and the cause is not yet shown")
 def test_component_resolution_has_the_sequential_distribution(lib, sys1):
-    """Production HB valence resolution (independent components, Philox draws)
-    against the reference's sequential process (oracle, glibc draws) on the same
-    configuration: for every candidate, the frequency with which it is kept over
-    1500 independent resolutions must agree (two-proportion z-score below 5 for
-    every candidate; about 1,000 candidates are tested), and so must the mean of
-    the resolved HB energy (4 standard errors).  The serial depth (rounds) must
-    drop well below the sequential iteration count."""
+    """Production HB valence resolution (independent components on separate
+    wavefronts, Philox draws) against the reference's sequential process (oracle,
+    glibc draws) on the same configuration: for every candidate, the frequency
+    with which it is kept over 1500 independent resolutions must agree
+    (two-proportion z-score below 5 for every candidate; about 1,000 candidates
+    are tested), and so must the mean of the resolved HB energy (4 standard
+    errors).  The serial depth (iterations of the busiest wavefront) must drop
+    well below the sequential iteration count."""
     n = 1500
     eng = T.Engine(lib, [sys1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, seed=11)
     kept_g, e_g = 0, []
@@ -251,7 +250,7 @@ def test_component_resolution_has_the_sequential_distribution(lib, sys1):
     assert contested > 50
     assert np.abs(z).max() < 5.0
     assert abs(e_g.mean() - e_o.mean()) < 4 * se
-    assert rounds < iters / 4
+    assert rounds < iters / 2
 
 
 def test_philox_resolution_is_deterministic(lib, sys1):
