@@ -421,15 +421,15 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 constexpr int kResWaves = 8;
 constexpr int kResBlock = 64 * kResWaves;
 constexpr int kSeqWin = 1024;      // draw window of the sequential form (one wavefront)
-constexpr int kCompWin = 256;      // draw window of each wavefront in the component form
+constexpr int kCompWin = 192;      // draw window of each wavefront in the component form
 static_assert(kResWaves * kCompWin >= kSeqWin, "window regions are shared by the two forms");
 static_assert(kASiteMax <= 64 * 64, "over-subscribed sites must fit a 64 x 64 bit mask");
 // LDS layout of one replica's working set (bytes, in this order):
-//   E[kActMax] double | win[kResWaves * kCompWin] double | omask[kResWaves * 64] u64 |
-//   par, excess [kASiteMax] int | A0, A1, croot [kASiteMax] short |
+//   E[kActMax] double | uwin, lwin [kResWaves * kCompWin] double |
+//   par, excess, apack [kASiteMax] int | A1, croot [kASiteMax] short |
 //   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag[kActMax] u8
-constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + kResWaves * 64 * 8 + 2 * kASiteMax * 4 +
-                          3 * kASiteMax * 2 + (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
+constexpr size_t kHbLds = kActMax * 8 + 2 * kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 2 * kASiteMax * 2 +
+                          (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
 static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of one CU");
 
 enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
@@ -530,13 +530,18 @@ struct HbDraws {
     // draws available (stream mode); draws at or past it read 0.5 and the caller
     // reports the exhausted stream
     TIS_DEV long long avail() const { return rng ? lim - cur : (1ll << 62); }
-    // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4); threads tid, tid + nt, ...
-    TIS_DEV void fill(double *ubuf, long long lo, int n, int tid, int nt, int comp) const {
+    // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4), and lbuf[i] = its
+    // logarithm when lbuf is given; threads tid, tid + nt, ...
+    TIS_DEV void fill(double *ubuf, double *lbuf, long long lo, int n, int tid, int nt, int comp) const {
         if (rng == 0) {
             for (int q = tid; q < n / 4; q += nt) {
                 const uint32_t blk = ((uint32_t)comp << 20) + (uint32_t)((lo >> 2) + q);
                 const Philox4 b = philox4x32_10(c0, c1, blk, kTagHb, k0, k1);
-                for (int w = 0; w < 4; ++w) ubuf[4 * q + w] = u01_closed_open(b.v[w]);
+                for (int w = 0; w < 4; ++w) {
+                    const double u = u01_closed_open(b.v[w]);
+                    ubuf[4 * q + w] = u;
+                    if (lbuf) lbuf[4 * q + w] = log(u);
+                }
             }
         } else {
             for (int i = tid; i < n; i += nt) {
@@ -574,18 +579,21 @@ TIS_DEV void wave_lds_sync() {
 // keeping k and moving to i when u < exp((E_i - E_k)/T), switch the pick off and
 // lower the excess of every site it is listed at.
 //
-// The set is the bit mask omask (64 words in LDS, word w held by lane w in a
-// register): bit b stands for the over-subscribed site A0[c0 + b], A1 maps a
-// site back to its index in A0 (-1 if it was not over-subscribed).  The pick is a
-// wavefront prefix count; all acceptance tests of a pass are evaluated at once
-// (lane t holds pair (i, c), c < i, triangular index t = i(i-1)/2 + c) and the
-// walk reads their bits.  Draws come from an LDS window of `win` values that the
-// wavefront refills.  Sets *ovf on a site with more than 64 live candidates or an
-// inconsistent table.
-TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA0, const short *A0, const short *A1,
-                          const short *aoff, const short *alist, const short *hoff, const short *hinc,
-                          unsigned char *flag, const double *E, double T, double *ubuf, int win, const HbDraws &dr,
-                          int comp, int *iters_out, long long *base_out, bool *ovf) {
+// The set is a bit mask over over-subscribed site indices ci (site order), word w
+// held by lane w in register w0 on entry: bit b stands for index c0 + b, whose
+// candidate list is apack[ci] = offset | length << 16 in alist; A1 maps a site
+// back to its index (-1 if it was not over-subscribed).  The pick is a wavefront
+// prefix count; all acceptance tests of a pass are evaluated at once (lane t
+// holds pair (i, c), c < i, triangular index t = i(i-1)/2 + c) and the walk reads
+// their bits.  Draws come from an LDS window of `win` values that the wavefront
+// refills.  LOGFORM tests T log(u) < E_i - E_c, with the logarithms taken when
+// the window is filled: the same event as u < exp((E_i - E_c)/T) up to rounding.
+// Sets *ovf on a site with more than 64 live candidates or an inconsistent table.
+template <bool LOGFORM>
+TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack, const short *A1,
+                          const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
+                          const double *E, double T, double *ubuf, double *lbuf, int win, const HbDraws &dr, int comp,
+                          int *iters_out, long long *base_out, bool *ovf) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     // pairs (i, c) of passes 0..3 handled by this lane
@@ -599,10 +607,9 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
         pi[p] = i;
         pc[p] = t - i * (i - 1) / 2;
     }
-    unsigned long long w = omask[lane];
     long long base = 0, win_lo = 0;
     int iters = 0;
-    dr.fill(ubuf, 0, win, lane, 64, comp);
+    dr.fill(ubuf, LOGFORM ? lbuf : nullptr, 0, win, lane, 64, comp);
     wave_lds_sync();
     for (;;) {
         const int cnt = __popcll(w);
@@ -612,7 +619,7 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
         if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
         if (base + 128 > win_lo + win) {
             win_lo = base & ~3ll;
-            dr.fill(ubuf, win_lo, win, lane, 64, comp);
+            dr.fill(ubuf, LOGFORM ? lbuf : nullptr, win_lo, win, lane, 64, comp);
             wave_lds_sync();
         }
         const double *u = ubuf + (base - win_lo);
@@ -627,9 +634,8 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
         if (hb == 0) { *ovf = true; break; }
         const int ci = c0 + 64 * L + __builtin_ctzll(hb);
         if (ci >= nA0) { *ovf = true; break; }
-        const int site = A0[ci];
         // ---- its live candidates, compacted to lanes 0..nK-1 in static order
-        const int o0 = aoff[site], len = aoff[site + 1] - o0;
+        const int pk = apack[ci], o0 = pk & 0xffff, len = pk >> 16;
         if (len > 64) { *ovf = true; break; }
         int bond = -1;
         bool keep = false;
@@ -653,6 +659,7 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
         // ---- walk: k moves to i when u_{nK+i} < exp((E_i - E_k)/T)
         int k = 0;
         const int npair = nK * (nK - 1) / 2;
+        const double *lu = lbuf + (base - win_lo);
         if (npair <= 256) {
             unsigned long long m[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -660,8 +667,10 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
                 if (64 * p < npair) {
                     const int i = pi[p] < nK ? pi[p] : 0, c = pi[p] < nK ? pc[p] : 0;
                     const double Ei = shfl_d(Ev, i), Ec = shfl_d(Ev, c);
-                    const bool acc = pi[p] < nK && u[nK + i] < exp((Ei - Ec) / T);
-                    m[p] = __ballot(acc);
+                    bool acc;
+                    if (LOGFORM) acc = T * lu[nK + i] < Ei - Ec;
+                    else acc = u[nK + i] < exp((Ei - Ec) / T);
+                    m[p] = __ballot(pi[p] < nK && acc);
                 }
             }
             for (int i = 1; i < nK; ++i) {
@@ -673,7 +682,8 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
             double Ek = readlane_d(Ev, 0);
             for (int i = 1; i < nK; ++i) {
                 const double Ei = readlane_d(Ev, i);
-                if (u[nK + i] < exp((Ei - Ek) / T)) { k = i; Ek = Ei; }
+                const bool acc = LOGFORM ? T * lu[nK + i] < Ei - Ek : u[nK + i] < exp((Ei - Ek) / T);
+                if (acc) { k = i; Ek = Ei; }
             }
         }
         const int off = readlane_i(bv, k);
@@ -681,15 +691,23 @@ TIS_DEV void resolve_wave(int *excess, unsigned long long *omask, int c0, int nA
         // ---- switch it off; sites whose excess reaches 0 leave the mask
         if (lane == 0) flag[off] = FLAG_OFF;
         const int h0 = hoff[off], hn = hoff[off + 1] - h0;
-        for (int q = lane; q < hn; q += 64) {
-            const int st = hinc[h0 + q];
-            if (atomicSub(&excess[st], 1) == 1) {
-                const int b = A1[st] - c0;
-                if (b >= 0 && b < 64 * 64) atomicAnd(&omask[b >> 6], ~(1ull << (b & 63)));
-            }
+        if (hn > 64) { *ovf = true; break; }
+        int b = -1;
+        bool cleared = false;
+        if (lane < hn) {
+            const int st = hinc[h0 + lane];
+            const int ia = A1[st];
+            cleared = atomicSub(&excess[st], 1) == 1;
+            b = ia - c0;
+        }
+        unsigned long long cm = __ballot(cleared && b >= 0 && b < 64 * 64);
+        while (cm) {
+            const int l = __builtin_ctzll(cm);
+            cm &= cm - 1ull;
+            const int bb = readlane_i(b, l);
+            if (lane == (bb >> 6)) w &= ~(1ull << (bb & 63));
         }
         wave_lds_sync();
-        w = omask[lane];
     }
     if (base > dr.avail()) *ovf = true;
     *iters_out = iters;
@@ -795,12 +813,12 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     const RepParams &P = D.rp[r];
     extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
-    double *win = E + kActMax;
-    unsigned long long *omask = (unsigned long long *)(win + kResWaves * kCompWin);
-    int *par = (int *)(omask + kResWaves * 64);
+    double *uwin = E + kActMax;
+    double *lwin = uwin + kResWaves * kCompWin;
+    int *par = (int *)(lwin + kResWaves * kCompWin);
     int *excess = par + kASiteMax;
-    short *A0 = (short *)(excess + kASiteMax);
-    short *A1 = A0 + kASiteMax;
+    int *apack = excess + kASiteMax;
+    short *A1 = (short *)(apack + kASiteMax);
     short *croot = A1 + kASiteMax;
     short *aoff = croot + kASiteMax;
     short *hoff = aoff + kASiteMax + 4;
@@ -810,10 +828,10 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __shared__ int scratch[20];
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
-    __shared__ int sh_err;
+    __shared__ int sh_err, sh_next;
     const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
     if (S.overflow & 2) return;
-    if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; }
+    if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     // ---- the rebuild tables, energies and gates
     for (int i = tid; i <= n_as; i += nt) aoff[i] = (short)D.asite_off[P.site0 + r + i];
@@ -827,7 +845,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     }
     __syncthreads();
     // ---- excess = live incidences - valence per active site; the
-    // over-subscribed ones in site order (A0) and the inverse map (A1)
+    // over-subscribed ones in site order (index ci: candidate list apack[ci]) and
+    // the inverse map A1
     int nA0 = 0;
     for (int s0 = 0; s0 < n_as; s0 += nt) {
         const int s = s0 + tid;
@@ -842,7 +861,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         const int rank = block_excl_sum(ex > 0, scratch, &tot);
         if (s < n_as) {
             A1[s] = (short)(ex > 0 ? nA0 + rank : -1);
-            if (ex > 0) A0[nA0 + rank] = (short)s;
+            if (ex > 0) apack[nA0 + rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
         }
         nA0 += tot;
     }
@@ -851,19 +870,14 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     if (D.rng == 1) {
         // reference stream: the exact sequential process on wavefront 0, over all
         // over-subscribed sites
-        for (int q = tid; q < 64; q += nt) {
-            const int c = 64 * q;
-            unsigned long long m = 0;
-            for (int b = 0; b < 64 && c + b < nA0; ++b) m |= 1ull << b;
-            omask[q] = m;
-        }
-        __syncthreads();
         if (wv == 0) {
+            const int c = 64 * lane;
+            const unsigned long long w0 = c + 64 <= nA0 ? ~0ull : (c < nA0 ? (1ull << (nA0 - c)) - 1ull : 0ull);
             int iters = 0;
             long long base = 0;
             bool ovf = false;
-            resolve_wave(excess, omask, 0, nA0, A0, A1, aoff, alist, hoff, hinc, flag, E, P.T, win, kSeqWin, dr, 0,
-                         &iters, &base, &ovf);
+            resolve_wave<false>(w0, excess, 0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, uwin, lwin, kSeqWin,
+                                dr, 0, &iters, &base, &ovf);
             if (lane == 0) {
                 sh_iters = iters;
                 sh_draws = base;
@@ -899,29 +913,30 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             ncomp += tot;
         }
         __syncthreads();
-        // ---- each wavefront runs components wv, wv + kResWaves, ...
-        double *ub = win + wv * kCompWin;
-        unsigned long long *om = omask + wv * 64;
+        // ---- each wavefront takes the next component in root order when it is done
+        double *ub = uwin + wv * kCompWin, *lb = lwin + wv * kCompWin;
         int wave_iters = 0;
         bool ovf = false;
-        for (int k = wv; k < ncomp; k += kResWaves) {
-            const int root = croot[k], c0 = root & ~63;
-            // lane w: members of the component among A0 indices c0 + 64 w .. + 63
+        for (int k = wv; k < ncomp;) {
+            const int root = croot[k], cfirst = root >> 6, c0 = cfirst << 6;
+            // member mask: word w from the ballot over indices c0 + 64 w .. + 63
             unsigned long long m = 0;
-            const int base_ci = c0 + 64 * lane;
-            for (int b = 0; b < 64; ++b) {
-                const int ci = base_ci + b;
-                if (ci >= nA0) break;
-                if (par[ci] == root) m |= 1ull << b;
+            const int clast = (nA0 - 1) >> 6;
+#pragma unroll 4
+            for (int c = cfirst; c <= clast; ++c) {
+                const int ci = 64 * c + lane;
+                const unsigned long long bm = __ballot(ci < nA0 && par[ci] == root);
+                if (lane == c - cfirst) m = bm;
             }
-            om[lane] = m;
-            wave_lds_sync();
             int it = 0;
             long long base = 0;
-            resolve_wave(excess, om, c0, nA0, A0, A1, aoff, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k,
-                         &it, &base, &ovf);
+            resolve_wave<true>(m, excess, c0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, ub, lb, kCompWin, dr,
+                               k, &it, &base, &ovf);
             wave_iters += it;
             if (ovf) break;
+            int nk = 0;
+            if (lane == 0) nk = atomicAdd(&sh_next, 1);
+            k = __shfl(nk, 0, 64);
         }
         if (lane == 0) {
             atomicMax(&sh_iters, wave_iters);
