@@ -137,7 +137,7 @@ struct tis_ctx {
         timed("k_list", sl, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, sl>>>(D, 1); });
     }
     void launch_rebuild_hb(hipStream_t sh) {
-        timed("k_hbstatus", sh, [&] { k_hbstatus<<<256, kBlock, 0, sh>>>(D); });
+        timed("k_hbstatus", sh, [&] { k_hbstatus<<<dim3(max_hb_chunks(), n_rep), kStatusBlock, 0, sh>>>(D); });
         timed("k_hbsites", sh, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), sh>>>(D); });
     }
     void launch_rebuild() {
@@ -157,6 +157,11 @@ struct tis_ctx {
         for (auto &st : all)
             if (st.need_rebuild) { st.need_rebuild = 0; any = true; }
         if (any) HIP_TRY(hipMemcpy(D.rs, all.data(), n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+    }
+    int max_hb_chunks() const {
+        int m = 1;
+        for (const RepParams &p : rp) m = std::max(m, (p.nhb + kStatusBlock - 1) / kStatusBlock);
+        return m;
     }
     size_t max_site_lds() const {
         int m = 1;
@@ -462,7 +467,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             }
             inc_base += s.n_hb_incid;
         }
-        if (2 * c->max_site_lds() > 150 * 1024) throw Fail{TIS_EINVAL, "too many HB sites per replica"};
+        if (2 * c->max_site_lds() > 100 * 1024) throw Fail{TIS_EINVAL, "too many HB sites per replica"};
         HIP_TRY(hipFuncSetAttribute((const void *)k_hbsites, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(2 * c->max_site_lds())));
         // bonded / 1-3 exclusions: every bond (i, k) and angle end pair (i, k) has its
@@ -525,6 +530,9 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         for (int v : rank)
             if (v < 0 || v > 0x7fff) throw Fail{TIS_EINVAL, "HB site tables are not consistent"};
         D.inc_rank = dalloc<int>(own, rank.size()); upload(D.inc_rank, rank);
+        D.chunk_cnt = dalloc<int>(own, (size_t)kMaxHbChunks * n_rep);
+        for (const RepParams &p : c->rp)
+            if (p.nhb > kMaxHbChunks * kStatusBlock) throw Fail{TIS_EINVAL, "more than 65536 HB candidates in a replica"};
         D.site_cnt = dalloc<int>(own, sv.size());
         HIP_TRY(hipMemset(D.site_cnt, 0, sv.size() * sizeof(int)));
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();

@@ -103,6 +103,7 @@ struct Dev {
     int *site_val;
     int *inc_rank;                // position of each HB incidence within its site's static list
     int *site_cnt;                // live incidences per site (rebuild scratch)
+    int *chunk_cnt;               // live candidates per status chunk, kMaxHbChunks per replica
     // HB dynamic (rebuild products)
     signed char *hb_status;       // -1 / 0 per candidate
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off

@@ -38,6 +38,21 @@ TIS_DEV double shfl_d(double v, int l) {
 
 TIS_DEV int wave_id() { return threadIdx.x >> 6; }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront: Hillis-Steele within
+// rows of 16 (row_shr 1, 2, 4, 8), then row 0's and rows 0-1's totals carried
+// into the rows above them (row_bcast 15 into rows 1 and 3, row_bcast 31 into
+// rows 2 and 3).  Lanes with no source read the old operand, 0.
+TIS_DEV int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
+
 // Ordered exclusive prefix of a per-thread count across the block; *total gets
 // the block sum.  scratch must hold blockDim/64 + 1 ints.  All threads call.
 TIS_DEV int block_excl_sum(int v, int *scratch, int *total) {
@@ -292,21 +307,31 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 // the list's own FP64 distance test (populate_lists, fast.cpp:1702-1727).  Live
 // candidates are counted at each of their sites; k_hbsites clears the counters
 // after reading them.
-__global__ void __launch_bounds__(kBlock) k_hbstatus(Dev D) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-    for (int r = 0; r < D.n_rep; ++r) {
-        if (!D.rs[r].need_rebuild) continue;
-        const RepParams &P = D.rp[r];
-        for (int k = tid; k < P.nhb; k += nt) {
-            const int g = P.hb0 + k;
-            const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
-            const bool live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
-            D.hb_status[g] = live ? 0 : -1;
-            D.hb_out[g] = live ? 0 : -1;
-            if (!live) continue;
+constexpr int kStatusBlock = 1024;
+constexpr int kMaxHbChunks = 64;   // candidates per replica <= kMaxHbChunks * kStatusBlock
+__global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D) {
+    const int r = blockIdx.y;
+    if (!D.rs[r].need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    if ((int)(blockIdx.x * kStatusBlock) >= P.nhb) return;
+    __shared__ int scratch[20];
+    const int k = blockIdx.x * kStatusBlock + threadIdx.x;
+    bool live = false;
+    if (k < P.nhb) {
+        const int g = P.hb0 + k;
+        const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
+        live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
+        D.hb_status[g] = live ? 0 : -1;
+        D.hb_out[g] = live ? 0 : -1;
+        if (live)
             for (int q = D.hb_inc_off[g]; q < D.hb_inc_off[g + 1]; ++q) atomicAdd(&D.site_cnt[D.hb_inc[q]], 1);
-        }
     }
+    // rank among the chunk's live candidates (candidate order); k_hbsites adds the
+    // chunk offsets
+    int tot;
+    const int rank = block_excl_sum(live, scratch, &tot);
+    if (k < P.nhb) D.act_idx[P.hb0 + k] = live ? rank : -1;
+    if (threadIdx.x == 0) D.chunk_cnt[kMaxHbChunks * r + blockIdx.x] = tot;
 }
 
 // k_hbsites: one workgroup per rebuilt replica.  Active sites (with at least one
@@ -322,17 +347,30 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
     extern __shared__ int smem[];
     int *site_map = smem, *fill = smem + P.nsite;
     __shared__ int scratch[20];
-    // live candidates in candidate order
-    int n_act = 0;
-    for (int k0 = 0; k0 < P.nhb; k0 += blockDim.x) {
-        const int k = k0 + threadIdx.x;
-        const bool live = k < P.nhb && D.hb_status[P.hb0 + k] == 0;
-        int tot;
-        const int a = n_act + block_excl_sum(live, scratch, &tot);
-        if (k < P.nhb) D.act_idx[P.hb0 + k] = live ? a : -1;
-        if (live && a < kActMax) D.act_hb[P.hb0 + a] = P.hb0 + k;
-        n_act += tot;
+    __shared__ int chunk_off[kMaxHbChunks + 1];
+    __shared__ int aoff_l[kASiteMax + 1];   // site segment offsets
+    __shared__ int seg_l[kAIncMax];         // (static rank << 16 | live index) per site segment
+    // live candidates in candidate order: chunk offsets, then chunk-local ranks
+    const int nchunk = (P.nhb + kStatusBlock - 1) / kStatusBlock;
+    if (threadIdx.x < 64) {
+        const int c = threadIdx.x < nchunk ? D.chunk_cnt[kMaxHbChunks * r + threadIdx.x] : 0;
+        const int incl = wave_incl_scan(c);
+        chunk_off[threadIdx.x] = incl - c;
+        if (threadIdx.x == 63) chunk_off[kMaxHbChunks] = incl;
     }
+    __syncthreads();
+    const int n_act = chunk_off[kMaxHbChunks];
+    if (n_act <= kActMax) {
+        for (int k = threadIdx.x; k < P.nhb; k += blockDim.x) {
+            const int g = P.hb0 + k;
+            const int lr = D.act_idx[g];
+            if (lr < 0) continue;
+            const int a = chunk_off[k / kStatusBlock] + lr;
+            D.act_idx[g] = a;
+            D.act_hb[P.hb0 + a] = g;
+        }
+    }
+    __syncthreads();
     if (n_act > kActMax) {
         for (int q = threadIdx.x; q < P.nsite; q += blockDim.x) D.site_cnt[P.site0 + q] = 0;
         if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
@@ -350,8 +388,11 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
             fill[s] = 0;
             if (cnt > 0) {
                 site_map[s] = n_as + rank;
-                D.asite_id[P.site0 + n_as + rank] = P.site0 + s;
-                D.asite_off[P.site0 + r + n_as + rank] = n_inc + off;
+                if (n_as + rank < kASiteMax) {
+                    D.asite_id[P.site0 + n_as + rank] = P.site0 + s;
+                    D.asite_off[P.site0 + r + n_as + rank] = n_inc + off;
+                    aoff_l[n_as + rank] = n_inc + off;
+                }
             } else {
                 site_map[s] = -1;
             }
@@ -359,7 +400,10 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         n_as += tot_s;
         n_inc += tot_i;
     }
-    if (threadIdx.x == 0) D.asite_off[P.site0 + r + n_as] = n_inc;
+    if (threadIdx.x == 0 && n_as <= kASiteMax) {
+        D.asite_off[P.site0 + r + n_as] = n_inc;
+        aoff_l[n_as] = n_inc;
+    }
     __syncthreads();
     if (n_as > kASiteMax || n_inc > kAIncMax) {
         if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
@@ -383,25 +427,26 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
                 const int ai = site_map[D.hb_inc[e] - P.site0];
                 D.ahb_inc[P.inc0 + n_ai + off + q] = ai;
                 const int slot = atomicAdd(&fill[D.hb_inc[e] - P.site0], 1);
-                D.asite_list[P.inc0 + D.asite_off[P.site0 + r + ai] + slot] = (D.inc_rank[e] << 16) | a;
+                seg_l[aoff_l[ai] + slot] = (D.inc_rank[e] << 16) | a;
             }
         }
         n_ai += tot;
     }
     if (threadIdx.x == 0) D.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
     __syncthreads();
-    // order each site's segment by static rank, keep the live index
+    // order each site's segment by static rank (in LDS), keep the live index
     for (int ai = threadIdx.x; ai < n_as; ai += blockDim.x) {
-        int *seg = D.asite_list + P.inc0 + D.asite_off[P.site0 + r + ai];
-        const int len = D.asite_off[P.site0 + r + ai + 1] - D.asite_off[P.site0 + r + ai];
+        int *seg = seg_l + aoff_l[ai];
+        const int len = aoff_l[ai + 1] - aoff_l[ai];
         for (int i = 1; i < len; ++i) {
             const int v = seg[i];
             int j = i - 1;
             while (j >= 0 && seg[j] > v) { seg[j + 1] = seg[j]; --j; }
             seg[j + 1] = v;
         }
-        for (int i = 0; i < len; ++i) seg[i] &= 0xffff;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_inc; i += blockDim.x) D.asite_list[P.inc0 + i] = seg_l[i] & 0xffff;
     if (threadIdx.x == 0) {
         S.n_act = n_act;
         S.n_asite = n_as;
@@ -470,20 +515,23 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
 template <class R>
 TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double *geo, R w, const R psi0[3],
                            R kr, R kth, R kpsi) {
-    const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
-    const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
+    const double4 p[6] = {D.pos[b[0]], D.pos[b[1]], D.pos[b[2]], D.pos[b[3]], D.pos[b[4]], D.pos[b[5]]};
+    // the six beads' forces are summed here, then added once each
+    V3<R> fb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) fb[q] = V3<R>{(R)0, (R)0, (R)0};
     // bond (b1, b4)
     {
-        V3<R> d = diff<R>(p1, p4);
+        V3<R> d = diff<R>(p[1], p[4]);
         R t = bond_coef<R>(d, kr * w, (R)geo[0], nullptr);
-        atomic_add_force(F, b[1], d * (-t));
-        atomic_add_force(F, b[4], d * t);
+        fb[1] = fb[1] + d * (-t);
+        fb[4] = fb[4] + d * t;
     }
     // angles (b0, b1, b4) and (b3, b4, b1) with the near-pi guard of stacking_valence_force
-    const int tri[2][3] = {{b[0], b[1], b[4]}, {b[3], b[4], b[1]}};
-    const double4 tp[2][3] = {{p0, p1, p4}, {p3, p4, p1}};
+    constexpr int tri[2][3] = {{0, 1, 4}, {3, 4, 1}};
+#pragma unroll
     for (int a = 0; a < 2; ++a) {
-        V3<R> ba = diff<R>(tp[a][1], tp[a][0]), bc = diff<R>(tp[a][1], tp[a][2]);
+        V3<R> ba = diff<R>(p[tri[a][1]], p[tri[a][0]]), bc = diff<R>(p[tri[a][1]], p[tri[a][2]]);
         R c;
         V3<R> g0, g2;
         angle_grad<R>(ba, bc, &c, &g0, &g2);
@@ -492,20 +540,23 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
         R th = acos(c);
         R coef = (R)-2 * (kth * w) * ((R)geo[1 + a] - th) / s;
         V3<R> f0 = g0 * coef, f2 = g2 * coef;
-        atomic_add_force(F, tri[a][0], f0);
-        atomic_add_force(F, tri[a][2], f2);
-        atomic_add_force(F, tri[a][1], (f0 + f2) * (R)-1);
+        fb[tri[a][0]] = fb[tri[a][0]] + f0;
+        fb[tri[a][2]] = fb[tri[a][2]] + f2;
+        fb[tri[a][1]] = fb[tri[a][1]] + (f0 + f2) * (R)-1;
     }
     // dihedrals (b0,b1,b4,b3), (b4,b1,b0,b2), (b1,b4,b3,b5)
-    const int quad[3][4] = {{b[0], b[1], b[4], b[3]}, {b[4], b[1], b[0], b[2]}, {b[1], b[4], b[3], b[5]}};
-    const double4 qp[3][4] = {{p0, p1, p4, p3}, {p4, p1, p0, p2}, {p1, p4, p3, p5}};
+    constexpr int quad[3][4] = {{0, 1, 4, 3}, {4, 1, 0, 2}, {1, 4, 3, 5}};
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         V3<R> g[4];
-        R psi = dihedral_grad<R>(diff<R>(qp[a][0], qp[a][1]), diff<R>(qp[a][1], qp[a][2]),
-                                 diff<R>(qp[a][2], qp[a][3]), g);
+        R psi = dihedral_grad<R>(diff<R>(p[quad[a][0]], p[quad[a][1]]), diff<R>(p[quad[a][1]], p[quad[a][2]]),
+                                 diff<R>(p[quad[a][2]], p[quad[a][3]]), g);
         R coef = (R)2 * (kpsi * w) * (psi0[a] - psi);
-        for (int c = 0; c < 4; ++c) atomic_add_force(F, quad[a][c], g[c] * coef);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fb[quad[a][c]] = fb[quad[a][c]] + g[c] * coef;
     }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) atomic_add_force(F, b[q], fb[q]);
 }
 
 // Uniform source of one step's HB resolution, with the replica's counters
@@ -551,20 +602,6 @@ struct HbDraws {
         }
     }
 };
-
-// Inclusive prefix sum over the 64 lanes of a wavefront: Hillis-Steele within
-// rows of 16 (row_shr 1, 2, 4, 8), then row 0's and rows 0-1's totals carried
-// into the rows above them (row_bcast 15 into rows 1 and 3, row_bcast 31 into
-// rows 2 and 3).  Lanes with no source read the old operand, 0.
-TIS_DEV int wave_incl_scan(int x) {
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
-    return x;
-}
 
 TIS_DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
