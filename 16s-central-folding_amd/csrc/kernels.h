@@ -470,10 +470,10 @@ constexpr int kCompWin = 192;      // draw window of each wavefront in the compo
 static_assert(kResWaves * kCompWin >= kSeqWin, "window regions are shared by the two forms");
 static_assert(kASiteMax <= 64 * 64, "over-subscribed sites must fit a 64 x 64 bit mask");
 // LDS layout of one replica's working set (bytes, in this order):
-//   E[kActMax] double | uwin, lwin [kResWaves * kCompWin] double |
+//   E[kActMax] double | uwin[kResWaves * kCompWin] double |
 //   par, excess, apack [kASiteMax] int | A1, croot [kASiteMax] short |
 //   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag[kActMax] u8
-constexpr size_t kHbLds = kActMax * 8 + 2 * kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 2 * kASiteMax * 2 +
+constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 2 * kASiteMax * 2 +
                           (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
 static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of one CU");
 
@@ -581,18 +581,13 @@ struct HbDraws {
     // draws available (stream mode); draws at or past it read 0.5 and the caller
     // reports the exhausted stream
     TIS_DEV long long avail() const { return rng ? lim - cur : (1ll << 62); }
-    // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4), and lbuf[i] = its
-    // logarithm when lbuf is given; threads tid, tid + nt, ...
-    TIS_DEV void fill(double *ubuf, double *lbuf, long long lo, int n, int tid, int nt, int comp) const {
+    // ubuf[i] = draw lo + i for i < n (lo and n multiples of 4); threads tid, tid + nt, ...
+    TIS_DEV void fill(double *ubuf, long long lo, int n, int tid, int nt, int comp) const {
         if (rng == 0) {
             for (int q = tid; q < n / 4; q += nt) {
                 const uint32_t blk = ((uint32_t)comp << 20) + (uint32_t)((lo >> 2) + q);
                 const Philox4 b = philox4x32_10(c0, c1, blk, kTagHb, k0, k1);
-                for (int w = 0; w < 4; ++w) {
-                    const double u = u01_closed_open(b.v[w]);
-                    ubuf[4 * q + w] = u;
-                    if (lbuf) lbuf[4 * q + w] = log(u);
-                }
+                for (int w = 0; w < 4; ++w) ubuf[4 * q + w] = u01_closed_open(b.v[w]);
             }
         } else {
             for (int i = tid; i < n; i += nt) {
@@ -602,6 +597,18 @@ struct HbDraws {
         }
     }
 };
+
+// u < exp(dE / T), the reference's acceptance test (fast.cpp:2300-2306), decided
+// by an FP32 exponential when u lies outside +-1e-4 (relative) of it, which that
+// exponential's error (<= 1e-5 relative for |dE/T| < 88) cannot cross, and by the
+// FP64 expression itself otherwise.
+TIS_DEV bool hb_accept(double u, double dE, double T, double invT) {
+    const float e = __expf((float)(dE * invT));
+    const double ed = (double)e;
+    if (u < ed * (1.0 - 1e-4)) return true;
+    if (u > ed * (1.0 + 1e-4)) return false;
+    return u < exp(dE / T);
+}
 
 TIS_DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -623,13 +630,11 @@ TIS_DEV void wave_lds_sync() {
 // prefix count; all acceptance tests of a pass are evaluated at once (lane t
 // holds pair (i, c), c < i, triangular index t = i(i-1)/2 + c) and the walk reads
 // their bits.  Draws come from an LDS window of `win` values that the wavefront
-// refills.  LOGFORM tests T log(u) < E_i - E_c, with the logarithms taken when
-// the window is filled: the same event as u < exp((E_i - E_c)/T) up to rounding.
-// Sets *ovf on a site with more than 64 live candidates or an inconsistent table.
-template <bool LOGFORM>
+// refills.  Sets *ovf on a site with more than 64 live candidates or an
+// inconsistent table.
 TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack, const short *A1,
                           const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
-                          const double *E, double T, double *ubuf, double *lbuf, int win, const HbDraws &dr, int comp,
+                          const double *E, double T, double *ubuf, int win, const HbDraws &dr, int comp,
                           int *iters_out, long long *base_out, bool *ovf) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -646,7 +651,8 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
     }
     long long base = 0, win_lo = 0;
     int iters = 0;
-    dr.fill(ubuf, LOGFORM ? lbuf : nullptr, 0, win, lane, 64, comp);
+    const double invT = 1.0 / T;
+    dr.fill(ubuf, 0, win, lane, 64, comp);
     wave_lds_sync();
     for (;;) {
         const int cnt = __popcll(w);
@@ -656,7 +662,7 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
         if (base + 128 > win_lo + win) {
             win_lo = base & ~3ll;
-            dr.fill(ubuf, LOGFORM ? lbuf : nullptr, win_lo, win, lane, 64, comp);
+            dr.fill(ubuf, win_lo, win, lane, 64, comp);
             wave_lds_sync();
         }
         const double *u = ubuf + (base - win_lo);
@@ -696,7 +702,6 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         // ---- walk: k moves to i when u_{nK+i} < exp((E_i - E_k)/T)
         int k = 0;
         const int npair = nK * (nK - 1) / 2;
-        const double *lu = lbuf + (base - win_lo);
         if (npair <= 256) {
             unsigned long long m[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -704,10 +709,8 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
                 if (64 * p < npair) {
                     const int i = pi[p] < nK ? pi[p] : 0, c = pi[p] < nK ? pc[p] : 0;
                     const double Ei = shfl_d(Ev, i), Ec = shfl_d(Ev, c);
-                    bool acc;
-                    if (LOGFORM) acc = T * lu[nK + i] < Ei - Ec;
-                    else acc = u[nK + i] < exp((Ei - Ec) / T);
-                    m[p] = __ballot(pi[p] < nK && acc);
+                    const bool acc = pi[p] < nK && hb_accept(u[nK + i], Ei - Ec, T, invT);
+                    m[p] = __ballot(acc);
                 }
             }
             for (int i = 1; i < nK; ++i) {
@@ -719,8 +722,7 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
             double Ek = readlane_d(Ev, 0);
             for (int i = 1; i < nK; ++i) {
                 const double Ei = readlane_d(Ev, i);
-                const bool acc = LOGFORM ? T * lu[nK + i] < Ei - Ek : u[nK + i] < exp((Ei - Ek) / T);
-                if (acc) { k = i; Ek = Ei; }
+                if (hb_accept(u[nK + i], Ei - Ek, T, invT)) { k = i; Ek = Ei; }
             }
         }
         const int off = readlane_i(bv, k);
@@ -851,8 +853,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
     double *uwin = E + kActMax;
-    double *lwin = uwin + kResWaves * kCompWin;
-    int *par = (int *)(lwin + kResWaves * kCompWin);
+    int *par = (int *)(uwin + kResWaves * kCompWin);
     int *excess = par + kASiteMax;
     int *apack = excess + kASiteMax;
     short *A1 = (short *)(apack + kASiteMax);
@@ -913,8 +914,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             int iters = 0;
             long long base = 0;
             bool ovf = false;
-            resolve_wave<false>(w0, excess, 0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, uwin, lwin, kSeqWin,
-                                dr, 0, &iters, &base, &ovf);
+            resolve_wave(w0, excess, 0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, uwin, kSeqWin, dr, 0, &iters,
+                         &base, &ovf);
             if (lane == 0) {
                 sh_iters = iters;
                 sh_draws = base;
@@ -951,7 +952,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         }
         __syncthreads();
         // ---- each wavefront takes the next component in root order when it is done
-        double *ub = uwin + wv * kCompWin, *lb = lwin + wv * kCompWin;
+        double *ub = uwin + wv * kCompWin;
         int wave_iters = 0;
         bool ovf = false;
         for (int k = wv; k < ncomp;) {
@@ -967,8 +968,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
             int it = 0;
             long long base = 0;
-            resolve_wave<true>(m, excess, c0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, ub, lb, kCompWin, dr,
-                               k, &it, &base, &ovf);
+            resolve_wave(m, excess, c0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
+                         &base, &ovf);
             wave_iters += it;
             if (ovf) break;
             int nk = 0;
