@@ -463,7 +463,7 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 
 // ------------------------------------------------------------ k_hb
 // Valence resolution: one workgroup of kResWaves wavefronts per replica.
-constexpr int kResWaves = 8;
+constexpr int kResWaves = 16;
 constexpr int kResBlock = 64 * kResWaves;
 constexpr int kSeqWin = 1024;      // draw window of the sequential form (one wavefront)
 constexpr int kCompWin = 192;      // draw window of each wavefront in the component form
