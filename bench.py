@@ -15,6 +15,8 @@ kernel from HIP-event timings, and the FP64 CPU oracle timed on this host.
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
 import time
 
@@ -81,6 +83,76 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
     return tot
 
 
+def batched_params(n_rep=16):
+    return [(0.5e-3 * 40.0 ** (k / 15.0), 350.0 + 100.0 * k / 15.0, k + 1) for k in range(n_rep)]
+
+
+def cpu_baseline_reference(steps_each):
+    """The reference's own step sequence (its translation unit compiled by
+    oracle/Makefile into oracle/_ref/ref_harness) timed on one host core for
+    `steps_each` steps of each of the 16 replica configurations; replica-steps/s.
+    None when that binary was not built."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+    if not os.access(exe, os.X_OK):
+        return None
+    data = os.path.join(REPO, "16s-central-folding_amd", "data", "16SCD")
+    tot_steps, tot_s = 0, 0.0
+    for c_mg, box, seed in batched_params():
+        cmd = [exe, "-D0.850", "-E3.975", "-d6.5", "-T0.596", f"-c{c_mg:.10f}", "-f0.050", f"-s{seed}",
+               f"-L{box:.6f}", f"-N{steps_each}", "-P"]
+        out = subprocess.run(cmd, cwd=data, capture_output=True, text=True, timeout=600, check=True).stdout
+        m = re.search(r"steps (\d+) seconds ([0-9.eE+-]+)", out)
+        if not m:
+            raise RuntimeError("unexpected reference harness output: " + out[-200:])
+        tot_steps += int(m.group(1))
+        tot_s += float(m.group(2))
+        log(f"reference CPU: c_Mg {c_mg * 1e3:.2f} mM box {box:.1f} A, {m.group(1)} steps in {float(m.group(2)):.2f} s")
+    return tot_steps / tot_s
+
+
+def step_bytes(eng, handles):
+    """Algorithmic bytes of one batched step, summed over replicas:
+    B_step = 148 N + 8 P_SR + 0.8 P_LR + 56 H + 52 S + 16 n_bond + 20 n_angle
+             + (40 N + 8 (P_SR + P_LR)) / R_rebuild
+    with P the pair counts, H the live HB candidates, S the stacks and R_rebuild
+    the mean steps between list rebuilds of the replica."""
+    tot = 0.0
+    for r, h in enumerate(handles):
+        e = eng.energies(r)
+        d = h.d
+        n = d["n_particles"]
+        psr, plr = e["n_sr_pairs"], e["n_lr_pairs"]
+        rb = max(1.0, (e["klok"] - 1) / max(1, e["n_rebuilds"]))
+        tot += (148 * n + 8 * psr + 0.8 * plr + 56 * e["n_hb_active"] + 52 * (d["n_st3"] + d["n_st2"])
+                + 16 * d["n_bonds"] + 20 * d["n_angles"] + (40 * n + 8 * (psr + plr)) / rb)
+    return tot
+
+
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB, five times: run under rocprofv3 --pmc to
+    relate FETCH_SIZE / WRITE_SIZE to bytes moved."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 5}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy of known size), or None."""
+    path = os.path.join(REPO, "profiles", "pmc_traffic_batched16_16SCD.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else k["hbm_bytes"]
+
+
 def cpu_baseline(systems, steps_each):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_ctypes as O
@@ -110,9 +182,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=250)
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--plain-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=500, help="oracle steps per replica for cpu_baseline")
+    ap.add_argument("--cpu-steps", type=int, default=200, help="CPU steps per replica for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches (for counter collection)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X only: replicas are batched inside one launch on one GPU; "
               "multi-GPU runs (replica sharding over RCCL) are not part of this build.", file=sys.stderr)
@@ -121,8 +199,8 @@ def main():
     systems = batched_systems(lib)
     log(f"built {len(systems)} replicas")
     handles = [T.SystemHandle(d) for d in systems]
-    eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024, use_graph=1,
-                   graph_steps=50)
+    eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
+                   use_graph=0 if args.no_graph else 1, graph_steps=50)
     eng.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -143,25 +221,33 @@ def main():
         bytes_launch = algorithmic_bytes(eng, handles, dominant)
         achieved = None if bytes_launch is None else bytes_launch / (avg_ms * 1e-3) / 1e9
     energies = [eng.energies(r) for r in range(n_rep)]
+    b_step = step_bytes(eng, handles)
     eng.close()
     # plain FP64 one-replica-per-launch path (5 mM, 350 A, 3,659 particles)
     d1, _ = T.build_16scd(lib, T=0.596, c_mg=0.005, c_k=0.050, seed=1, box=350.0)
-    h1 = T.SystemHandle(d1)
-    e1 = T.Engine(lib, [h1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, use_graph=0)
-    log("plain FP64 single-replica leg")
-    e1.step(max(50, args.warmup // 5))
-    t0 = time.perf_counter()
-    e1.step(args.plain_steps)
-    plain = args.plain_steps / (time.perf_counter() - t0)
-    e1.close()
+    plain = None
+    if args.plain_steps > 0:
+        h1 = T.SystemHandle(d1)
+        e1 = T.Engine(lib, [h1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, use_graph=0)
+        log("plain FP64 single-replica leg")
+        e1.step(max(50, args.warmup // 5))
+        t0 = time.perf_counter()
+        e1.step(args.plain_steps)
+        plain = args.plain_steps / (time.perf_counter() - t0)
+        e1.close()
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg")
-        sample = systems
-        v = cpu_baseline(sample, args.cpu_steps)
-        cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 CPU oracle (reference cell lists, glibc-stream noise), "
-                         f"{args.cpu_steps} steps of each of the {len(sample)} replicas, single thread"}
+        v = cpu_baseline_reference(args.cpu_steps)
+        if v is not None:
+            cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "reference",
+                   "sample": f"the reference's step loop (fast.cpp:6380-6386, g++ -O3, one MPI rank), "
+                             f"{args.cpu_steps} steps of each of the 16 replica configurations, one core"}
+        else:
+            v = cpu_baseline(systems, args.cpu_steps)
+            cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
+                   "sample": f"FP64 CPU oracle (reference cell lists, glibc-stream noise), "
+                             f"{args.cpu_steps} steps of each of the 16 replicas, single thread"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -180,13 +266,18 @@ def main():
                    "replicas": n_rep, "particles": int(sum(d["n_particles"] for d in systems)),
                    "precision": "mixed", "rng": "philox", "graph_steps": 50},
         "ns_per_day_aggregate": value * FS_PER_STEP * 86400 * 1e-6,
-        "plain_fp64_single": {"value": plain, "unit": "steps/s", "particles": d1["n_particles"],
-                              "ns_per_day": plain * FS_PER_STEP * 86400 * 1e-6},
-        "batched_vs_plain": value / plain,
+        "plain_fp64_single": None if plain is None else {
+            "value": plain, "unit": "steps/s", "particles": d1["n_particles"],
+            "ns_per_day": plain * FS_PER_STEP * 86400 * 1e-6},
+        "batched_vs_plain": None if plain is None else value / plain,
         "roofline": None if dominant is None else {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dominant),
             "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
+        "roofline_step": {"bound": "hbm", "bytes_per_step": b_step, "achieved": b_step * (value / n_rep) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": b_step * (value / n_rep) / 1e9 / HBM_PEAK_GBS},
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": [e["n_hb_active"] for e in energies],
         "cpu_baseline": cpu,

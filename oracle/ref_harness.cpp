@@ -16,6 +16,7 @@
 #include <string.h>
 #include <math.h>
 #include <mpi.h>
+#include <ctime>
 #include <vector>
 
 static long long g_rand_calls = 0;
@@ -74,7 +75,7 @@ void dump_csr(const char *name, int n_rows, const int *row_n, int *const *rows) 
 int main(int argc, char **argv) {
     double box = 350.0;
     long nsteps = 0, every = 1;
-    int noise_off = 0;
+    int noise_off = 0, timing = 0;
     unsigned rng_terms = 12345u, rng_traj = 777u;
     const char *out = "ref_dump.bin";
     int seed = 1;
@@ -96,6 +97,7 @@ int main(int argc, char **argv) {
             case 'R': rng_terms = (unsigned)atol(v); break;
             case 'Q': rng_traj = (unsigned)atol(v); break;
             case 'O': out = v; break;
+            case 'P': timing = 1; break;
             default: fprintf(stderr, "unknown flag %s\n", argv[a]); return 2;
         }
     }
@@ -131,6 +133,25 @@ int main(int argc, char **argv) {
     list_content[1] = (int *)calloc(cap, sizeof(int));
     populate_lists();
     PROGRESS = 0;
+
+    // -P: time nsteps of the reference's own step sequence (fast.cpp:6380-6386)
+    // and print "steps <n> seconds <s>"; nothing is dumped
+    if (timing) {
+        srand(rng_traj);
+        timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        for (klok = 1; klok <= nsteps; ++klok) {
+            deterministic_forces_MPI();
+            full_forces();
+            move_rigid_units();
+            check_shifts();
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        printf("steps %ld seconds %.9f particles %d\n", nsteps,
+               (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec), NT_atom);
+        MPI_Finalize();
+        return 0;
+    }
 
     g_out = fopen(out, "wb");
     if (!g_out) { perror(out); return 1; }
