@@ -93,6 +93,7 @@ TIS_HD double u01_open_closed(uint32_t r) { return ((double)r + 1.0) * 2.3283064
 // Stream tags of the counter's fourth word.
 constexpr uint32_t kTagNoise = 0x4E4F4953u;  // "NOIS"
 constexpr uint32_t kTagHb = 0x48425253u;     // "HBRS"
+constexpr uint32_t kTagSwap = 0x52454D44u;   // "REMD"
 
 // ------------------------------------------------------------------ forces
 // Gradient pieces are written into g[] (one V3 per bead) and scaled by the

@@ -82,6 +82,9 @@ struct tis_ctx {
     std::vector<std::string> knames;
     std::vector<double> ksum_ms;
     std::vector<long long> kcount;
+    // replica exchange
+    long long steps_done = 0;         // steps of tis_step since creation (exchange cadence)
+    long long exchange_attempts = 0;
 
     ~tis_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -183,6 +186,47 @@ struct tis_ctx {
         });
     }
 
+    // Energies of every replica at its current configuration, with the pair
+    // forces left untouched (eval mode 2); multi-body forces land in f_mb and are
+    // cleared by the caller.  with_hb: include the HB terms (with their valence
+    // resolution, which does not advance any stream).
+    void launch_energies(bool with_hb) {
+        k_zero_energies<<<grid(n_rep), kBlock, 0, s_main>>>(D);
+        if (with_hb) launch_hb(s_main, 0);
+        const int nmb = n_rep + nb_bonded, npr = grid(kPairLanes * D.n_tot);
+        const unsigned mb = TIS_TERM_STACK | TIS_TERM_BACKBONE, pr = TIS_TERM_SR | TIS_TERM_LR;
+        if (D.precision == TIS_PREC_FP64) {
+            k_force<double, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, 1);
+            k_force<double, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, 2);
+        } else {
+            k_force<float, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, 1);
+            k_force<float, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, 2);
+        }
+    }
+    // One replica-exchange attempt, all on the device (the sequence documented
+    // at tis_exchange_now in tis_gpu.h).
+    void launch_exchange() {
+        const int parity = (int)(exchange_attempts & 1);
+        int max_n = 1;
+        for (const RepParams &p : rp) max_n = std::max(max_n, p.n);
+        const dim3 gs(grid(max_n), (n_rep + 1) / 2);
+        launch_rebuild();                        // a trigger pending from the last step
+        launch_energies(true);
+        k_remd_collect<<<grid(n_rep), kBlock, 0, s_main>>>(D, 0);
+        k_remd_mark<<<grid(n_rep), kBlock, 0, s_main>>>(D, parity);
+        k_remd_swap<<<gs, kBlock, 0, s_main>>>(D, parity, 0);
+        launch_rebuild();
+        launch_energies(false);
+        k_remd_collect<<<grid(n_rep), kBlock, 0, s_main>>>(D, 1);
+        k_remd_decide<<<grid(n_rep), kBlock, 0, s_main>>>(D, exchange_attempts);
+        k_remd_swap<<<gs, kBlock, 0, s_main>>>(D, parity, 1);
+        launch_rebuild();
+        k_clear_rebuild<<<grid(n_rep), kBlock, 0, s_main>>>(D);
+        k_zero_fmb<<<grid(D.n_tot), kBlock, 0, s_main>>>(D);
+        HIP_TRY(hipGetLastError());
+        ++exchange_attempts;
+    }
+
     // One Langevin step.  The HB resolution and the multi-body terms run on
     // their own streams, concurrently with the pair gather; the integrator joins all.
     void launch_step() {
@@ -234,6 +278,22 @@ namespace {
 
 constexpr double kSigLJh = 1.5852;
 
+// True when two replica descriptions are the same system: particle types and
+// classes, bonds, angles, stacks and HB candidates (with their sites) equal.
+// Temperatures, boxes, coordinates and T-dependent energies may differ.
+bool same_system(const tis_system &a, const tis_system &b) {
+    auto eq = [](const auto *x, const auto *y, long long n) {
+        return n <= 0 || (x && y && std::memcmp(x, y, sizeof(*x) * (size_t)n) == 0);
+    };
+    return a.n_particles == b.n_particles && a.n_beads == b.n_beads && a.n_bonds == b.n_bonds &&
+           a.n_angles == b.n_angles && a.n_st3 == b.n_st3 && a.n_st2 == b.n_st2 && a.n_hb == b.n_hb &&
+           a.n_hb_sites == b.n_hb_sites && a.n_hb_incid == b.n_hb_incid && eq(a.type, b.type, a.n_particles) &&
+           eq(a.cls, b.cls, a.n_particles) && eq(a.bond_ij, b.bond_ij, 2LL * a.n_bonds) &&
+           eq(a.angle_ijk, b.angle_ijk, 3LL * a.n_angles) && eq(a.st3_beads, b.st3_beads, 6LL * a.n_st3) &&
+           eq(a.st2_beads, b.st2_beads, 7LL * a.n_st2) && eq(a.hb_beads, b.hb_beads, 6LL * a.n_hb) &&
+           eq(a.hb_site_idx, b.hb_site_idx, (long long)a.n_hb_incid);
+}
+
 // Validates one replica description; throws Fail on the first problem found.
 void validate(const tis_system &s, int r) {
     auto bad = [&](const std::string &m) { throw Fail{TIS_EINVAL, "replica " + std::to_string(r) + ": " + m}; };
@@ -283,6 +343,7 @@ extern "C" void tis_default_options(tis_options *o) {
     o->lr_every = 10;
     o->use_graph = 1;
     o->graph_steps = 50;
+    o->exchange_every = 0;
 }
 
 static int fail(const Fail &f) {
@@ -322,6 +383,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         if (o.precision != TIS_PREC_FP64 && o.precision != TIS_PREC_MIXED) throw Fail{TIS_EINVAL, "precision"};
         if (o.rng != TIS_RNG_PHILOX && o.rng != TIS_RNG_STREAM) throw Fail{TIS_EINVAL, "rng"};
         if (!(o.r_sr > 0) || !(o.r_lr >= o.r_sr) || !(o.skin >= 0) || o.lr_every <= 0) throw Fail{TIS_EINVAL, "list settings"};
+        if (o.exchange_every < 0) throw Fail{TIS_EINVAL, "exchange_every"};
         for (int r = 0; r < n_rep; ++r) validate(reps[r], r);
         int dev_count = 0;
         HIP_TRY(hipGetDeviceCount(&dev_count));
@@ -552,6 +614,22 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.stream_base = dalloc<long long>(own, n_rep);
         D.stream_len = dalloc<long long>(own, n_rep);
         c->stream_cap.assign(n_rep, 0);
+        // ---- replica exchange: neighbours describing the same system may swap
+        {
+            std::vector<int> ok(n_rep, 0), walker(n_rep);
+            for (int r = 0; r < n_rep; ++r) walker[r] = r;
+            for (int r = 0; r + 1 < n_rep; ++r) ok[r] = same_system(reps[r], reps[r + 1]) ? 1 : 0;
+            D.remd_ok = dalloc<int>(own, n_rep); upload(D.remd_ok, ok);
+            D.walker = dalloc<int>(own, n_rep); upload(D.walker, walker);
+            D.remd_state = dalloc<int>(own, n_rep);
+            HIP_TRY(hipMemset(D.remd_state, 0, n_rep * sizeof(int)));
+            D.remd_e = dalloc<double>(own, 4 * (size_t)n_rep);
+            HIP_TRY(hipMemset(D.remd_e, 0, 4 * n_rep * sizeof(double)));
+            D.remd_d = dalloc<double>(own, n_rep);
+            HIP_TRY(hipMemset(D.remd_d, 0, n_rep * sizeof(double)));
+            D.remd_stats = dalloc<long long>(own, 2 * (size_t)n_rep);
+            HIP_TRY(hipMemset(D.remd_stats, 0, 2 * n_rep * sizeof(long long)));
+        }
         upload(D.ptype, ptype);
         upload(D.prep, prep);
         upload(D.pos, pos);
@@ -680,42 +758,95 @@ extern "C" int tis_set_step_counter(tis_ctx *c, int rep, long long klok) {
     });
 }
 
+// n steps: event-timed eager launches when profiling, else whole captured
+// graphs of graph_steps and eager launches for the remainder.
+static void run_steps(tis_ctx *c, long long n_steps) {
+    long long done = 0;
+    if (c->profiling) {
+        // bounded batches so that the event pool stays small
+        while (done < n_steps) {
+            const long long m = std::min<long long>(64, n_steps - done);
+            for (long long q = 0; q < m; ++q) c->launch_step();
+            HIP_TRY(hipStreamSynchronize(c->s_main));
+            HIP_TRY(hipStreamSynchronize(c->s_hb));
+            HIP_TRY(hipStreamSynchronize(c->s_mb));
+            c->collect_times();
+            done += m;
+        }
+    }
+    if (c->opt.use_graph && c->opt.graph_steps > 0 && n_steps - done >= c->opt.graph_steps) {
+        if (!c->graph) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
+            HIP_TRY(hipStreamEndCapture(c->s_main, &g));
+            HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+            HIP_TRY(hipGraphDestroy(g));
+            c->graph_steps = c->opt.graph_steps;
+        }
+        while (n_steps - done >= c->graph_steps) {
+            HIP_TRY(hipGraphLaunch(c->graph, c->s_main));
+            done += c->graph_steps;
+        }
+    }
+    for (; done < n_steps; ++done) c->launch_step();
+}
+
 extern "C" int tis_step(tis_ctx *c, long long n_steps) {
     if (!c || n_steps < 0) return fail(Fail{TIS_EINVAL, "tis_step: bad arguments"});
     return guarded([&]() -> int {
-        long long done = 0;
-        if (c->profiling) {
-            // bounded batches so that the event pool stays small
-            while (done < n_steps) {
-                const long long m = std::min<long long>(64, n_steps - done);
-                for (long long q = 0; q < m; ++q) c->launch_step();
-                HIP_TRY(hipStreamSynchronize(c->s_main));
-                HIP_TRY(hipStreamSynchronize(c->s_hb));
-                HIP_TRY(hipStreamSynchronize(c->s_mb));
-                c->collect_times();
-                done += m;
-            }
+        const long long ex = c->opt.exchange_every;
+        long long left = n_steps;
+        while (left > 0) {
+            const long long chunk = ex > 0 ? std::min(left, ex - c->steps_done % ex) : left;
+            run_steps(c, chunk);
+            c->steps_done += chunk;
+            left -= chunk;
+            if (ex > 0 && c->steps_done % ex == 0) c->launch_exchange();
         }
-        if (c->opt.use_graph && c->opt.graph_steps > 0 && n_steps - done >= c->opt.graph_steps) {
-            if (!c->graph) {
-                hipGraph_t g;
-                HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
-                for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
-                HIP_TRY(hipStreamEndCapture(c->s_main, &g));
-                HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-                HIP_TRY(hipGraphDestroy(g));
-                c->graph_steps = c->opt.graph_steps;
-            }
-            while (n_steps - done >= c->graph_steps) {
-                HIP_TRY(hipGraphLaunch(c->graph, c->s_main));
-                done += c->graph_steps;
-            }
-        }
-        for (; done < n_steps; ++done) c->launch_step();
         HIP_TRY(hipGetLastError());
         if (n_steps > 0) c->finish_pending_rebuild();
         HIP_TRY(hipStreamSynchronize(c->s_main));
         c->check_flags();
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_exchange_now(tis_ctx *c) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_exchange_now: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->launch_exchange();
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->check_flags();
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_exchange_stats(tis_ctx *c, long long *attempts, long long *accepts, double *last_d,
+                                      int n_pairs) {
+    if (!c || n_pairs < 0 || n_pairs > std::max(0, c->n_rep - 1))
+        return fail(Fail{TIS_EINVAL, "tis_get_exchange_stats: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        std::vector<long long> st(2 * (size_t)c->n_rep);
+        std::vector<double> d(c->n_rep);
+        HIP_TRY(hipMemcpy(st.data(), c->D.remd_stats, st.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(d.data(), c->D.remd_d, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int q = 0; q < n_pairs; ++q) {
+            if (attempts) attempts[q] = st[2 * q];
+            if (accepts) accepts[q] = st[2 * q + 1];
+            if (last_d) last_d[q] = d[q];
+        }
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_walkers(tis_ctx *c, int *walker, int n_rep) {
+    if (!c || !walker || n_rep != c->n_rep) return fail(Fail{TIS_EINVAL, "tis_get_walkers: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        HIP_TRY(hipMemcpy(walker, c->D.walker, n_rep * sizeof(int), hipMemcpyDeviceToHost));
         return TIS_OK;
     });
 }

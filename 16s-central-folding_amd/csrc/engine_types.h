@@ -121,6 +121,14 @@ struct Dev {
     double *maxwell;              // 3000 per replica (stream mode)
     int32_t *stream;              // raw glibc values, region per replica
     long long *stream_base, *stream_len;
+    // replica exchange
+    double *remd_e;               // 4 per replica: non-HB energy of own configuration, its HB energy,
+                                  // non-HB energy of the partner's configuration, spare
+    double *remd_d;               // per neighbour pair: exponent D of the last attempt
+    long long *remd_stats;        // 2 per neighbour pair: attempts, accepts
+    int *remd_ok;                 // per neighbour pair: exchangeable
+    int *remd_state;              // per neighbour pair: 0 idle, 1 swapped, 2 to restore
+    int *walker;                  // trajectory id held by each replica
 };
 
 }  // namespace tis

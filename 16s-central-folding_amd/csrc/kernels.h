@@ -1246,7 +1246,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             }
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
-        if (valid && sub == 0) D.f_sr[p] = make_double4(gx, gy, gz, 0.0);
+        if (valid && sub == 0 && eval != 2) D.f_sr[p] = make_double4(gx, gy, gz, 0.0);
         if (eval) {
             ew = group_sum(ew);
             ec = group_sum(ec);
@@ -1276,7 +1276,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             if (eval) ec += P.qlb[ab] / sqrt(r2);
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
-        if (lr && valid && sub == 0) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);
+        if (lr && valid && sub == 0 && eval != 2) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);
         if (eval) {
             ec = group_sum(ec);
             if (lr && valid && sub == 0) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
@@ -1397,6 +1397,121 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
     D.old[p] = D.old[D.n_tot + p];
     D.vel[p] = D.vel[D.n_tot + p];
     D.f_mb[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// ------------------------------------------------------------ replica exchange
+// Helpers of the on-device temperature exchange between neighbouring replicas
+// that describe the same system (no counterpart in the reference, which runs
+// its replicas as separate jobs).
+
+__global__ void __launch_bounds__(kBlock) k_zero_energies(Dev D) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < D.n_rep)
+        for (int q = 0; q < 12; ++q) D.rs[r].e[q] = 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_fmb(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < D.n_tot) D.f_mb[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// Energy sums of the evaluation just done: which 0 = the replica's own
+// configuration (non-HB sum and HB sum), 1 = the partner's configuration
+// (non-HB sum).  Every term enters with the sign for which F = -grad U.
+__global__ void __launch_bounds__(kBlock) k_remd_collect(Dev D, int which) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= D.n_rep) return;
+    const double *e = D.rs[r].e;
+    const double nonhb = e[EI_E2_ST] + e[EI_E3_ST] + e[EI_WCA] + e[EI_COUL_SR] + e[EI_COUL_LR] + e[EI_BOND] +
+                         e[EI_ANGLE];
+    if (which == 0) {
+        D.remd_e[4 * r] = nonhb;
+        D.remd_e[4 * r + 1] = e[EI_E2_HB] + e[EI_E3_HB];
+    } else {
+        D.remd_e[4 * r + 2] = nonhb;
+    }
+}
+
+// Marks the exchangeable pairs (r, r + 1) with r % 2 == parity for swapping
+// (state 1); the other pairs are idle (0).
+__global__ void __launch_bounds__(kBlock) k_remd_mark(Dev D, int parity) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r + 1 < D.n_rep) D.remd_state[r] = (D.remd_ok[r] && (r & 1) == parity) ? 1 : 0;
+}
+
+// mode 0: swap the configurations of the pairs in state 1, keeping the old ones
+// in the integrator's second buffer; the configuration moving from slot a to
+// slot b has its velocity and x - x_old scaled by sqrt(T_b / T_a).  mode 1:
+// restore the pairs in state 2.  Grid (particle blocks, pairs q, pair (2q +
+// parity, 2q + parity + 1)); the slots are marked for a list rebuild and their
+// walker ids exchanged.  States are only read here.
+__global__ void __launch_bounds__(kBlock) k_remd_swap(Dev D, int parity, int mode) {
+    const int q = blockIdx.y;
+    const int ra = 2 * q + parity, rb = ra + 1;
+    if (rb >= D.n_rep) return;
+    if (D.remd_state[ra] != (mode == 0 ? 1 : 2)) return;
+    const RepParams &A = D.rp[ra], &B = D.rp[rb];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = D.n_tot;
+    if (i < A.n) {
+        const int pa = A.p0 + i, pb = B.p0 + i;
+        if (mode == 0) {
+            const double4 xa = D.pos[pa], oa = D.old[pa], va = D.vel[pa];
+            const double4 xb = D.pos[pb], ob = D.old[pb], vb = D.vel[pb];
+            D.pos[N + pa] = xa; D.old[N + pa] = oa; D.vel[N + pa] = va;
+            D.pos[N + pb] = xb; D.old[N + pb] = ob; D.vel[N + pb] = vb;
+            const double s_ab = sqrt(B.T / A.T), s_ba = sqrt(A.T / B.T);   // a -> b, b -> a
+            D.pos[pa] = xb;
+            D.old[pa] = make_double4(xb.x - s_ba * (xb.x - ob.x), xb.y - s_ba * (xb.y - ob.y),
+                                     xb.z - s_ba * (xb.z - ob.z), 0.0);
+            D.vel[pa] = make_double4(s_ba * vb.x, s_ba * vb.y, s_ba * vb.z, 0.0);
+            D.pos[pb] = xa;
+            D.old[pb] = make_double4(xa.x - s_ab * (xa.x - oa.x), xa.y - s_ab * (xa.y - oa.y),
+                                     xa.z - s_ab * (xa.z - oa.z), 0.0);
+            D.vel[pb] = make_double4(s_ab * va.x, s_ab * va.y, s_ab * va.z, 0.0);
+        } else {
+            D.pos[pa] = D.pos[N + pa]; D.old[pa] = D.old[N + pa]; D.vel[pa] = D.vel[N + pa];
+            D.pos[pb] = D.pos[N + pb]; D.old[pb] = D.old[N + pb]; D.vel[pb] = D.vel[N + pb];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int w = D.walker[ra];
+        D.walker[ra] = D.walker[rb];
+        D.walker[rb] = w;
+        for (int r : {ra, rb}) {
+            RepState &S = D.rs[r];
+            S.need_rebuild = 1;
+            S.lr_due = 1;
+            S.progress = 0;
+        }
+    }
+}
+
+// Metropolis decision for each swapped pair: D as documented in tis_gpu.h,
+// accepted when D <= 0 or u < exp(-D) with u a Philox draw keyed by (seed,
+// attempt, pair).  Rejected pairs are marked for restoring.
+__global__ void __launch_bounds__(kBlock) k_remd_decide(Dev D, long long attempt) {
+    const int ra = blockIdx.x * blockDim.x + threadIdx.x, rb = ra + 1;
+    if (rb >= D.n_rep || D.remd_state[ra] != 1) return;
+    const double ba = 1.0 / D.rp[ra].T, bb = 1.0 / D.rp[rb].T;
+    const double *ea = D.remd_e + 4 * ra, *eb = D.remd_e + 4 * rb;
+    const double d = ba * (ea[2] - ea[0]) + bb * (eb[2] - eb[0]) + (eb[1] - ea[1]) * (ba - bb);
+    const Philox4 u4 = philox4x32_10((uint32_t)attempt, (uint32_t)(attempt >> 32), (uint32_t)ra, kTagSwap,
+                                     (uint32_t)D.seed, (uint32_t)(D.seed >> 32));
+    const bool acc = d <= 0.0 || u01_closed_open(u4.v[0]) < exp(-d);
+    D.remd_d[ra] = d;
+    D.remd_stats[2 * ra] += 1;
+    if (acc) {
+        D.remd_stats[2 * ra + 1] += 1;
+        D.remd_state[ra] = 0;
+    } else {
+        D.remd_state[ra] = 2;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_rebuild(Dev D) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < D.n_rep) D.rs[r].need_rebuild = 0;
 }
 
 }  // namespace tis

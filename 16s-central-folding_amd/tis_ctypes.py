@@ -51,7 +51,7 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
                 ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
                 ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
-                ("graph_steps", C.c_int)]
+                ("graph_steps", C.c_int), ("exchange_every", C.c_longlong)]
 
 
 class Energies(C.Structure):
@@ -136,6 +136,10 @@ def load_library(path=LIB_PATH):
         lib.tis_set_profiling.argtypes = [C.c_void_p, C.c_int]
         lib.tis_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.tis_philox4x32_10.restype = None
+        lib.tis_exchange_now.argtypes = [C.c_void_p]
+        lib.tis_get_exchange_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp,
+                                               C.c_int]
+        lib.tis_get_walkers.argtypes = [C.c_void_p, _ip, C.c_int]
     return lib
 
 
@@ -312,6 +316,25 @@ class Engine:
         check(self.lib, self.lib.tis_set_uniform_stream(
             self.ctx, rep, self._stream.ctypes.data_as(C.POINTER(C.c_int32)), len(self._stream)),
             "tis_set_uniform_stream")
+
+    def exchange_now(self):
+        check(self.lib, self.lib.tis_exchange_now(self.ctx), "tis_exchange_now")
+
+    def exchange_stats(self):
+        n = max(0, len(self.handles) - 1)
+        att = np.zeros(max(n, 1), dtype=np.int64)
+        acc = np.zeros(max(n, 1), dtype=np.int64)
+        d = np.zeros(max(n, 1))
+        check(self.lib, self.lib.tis_get_exchange_stats(
+            self.ctx, att.ctypes.data_as(C.POINTER(C.c_longlong)), acc.ctypes.data_as(C.POINTER(C.c_longlong)),
+            d.ctypes.data_as(_dp), n), "tis_get_exchange_stats")
+        return att[:n], acc[:n], d[:n]
+
+    def walkers(self):
+        w = np.zeros(len(self.handles), dtype=np.int32)
+        check(self.lib, self.lib.tis_get_walkers(self.ctx, w.ctypes.data_as(_ip), len(self.handles)),
+              "tis_get_walkers")
+        return w
 
     def set_profiling(self, on):
         check(self.lib, self.lib.tis_set_profiling(self.ctx, int(on)), "tis_set_profiling")

@@ -153,6 +153,7 @@ typedef struct tis_options {
     int lr_every;         /* long-range Coulomb refresh cadence, 10 in the reference */
     int use_graph;        /* capture each tis_step chunk in a hipGraph */
     int graph_steps;      /* steps per captured graph */
+    long long exchange_every; /* steps between replica-exchange attempts; 0 = none */
 } tis_options;
 
 /* Fills *opt with the reference's settings (FP64, stream RNG off => Philox, skin 3, ...). */
@@ -177,6 +178,31 @@ int tis_set_step_counter(tis_ctx *ctx, int rep, long long klok);
  * without its output block (deterministic_forces_MPI, full_forces,
  * move_rigid_units, check_shifts). */
 int tis_step(tis_ctx *ctx, long long n_steps);
+
+/* Replica exchange (temperature ladder).  No counterpart in the reference,
+ * whose replicas are separate MPI jobs; this is the on-device swap of the
+ * batched configuration.  Neighbouring replicas r and r + 1 exchange when they
+ * describe the same system (particle types, bonds, angles, stacks and HB
+ * candidates equal; temperatures and set-up draws may differ).  An attempt, every
+ * exchange_every steps of tis_step and on each tis_exchange_now, pairs
+ * (r, r+1) with r even on even attempts and r odd on odd ones; for each pair it
+ *   1. evaluates every replica's energy U_r(x_r) (all terms; the HB term after
+ *      the valence resolution, which does not depend on T),
+ *   2. swaps the pair's configurations (velocities and x - x_old scaled by
+ *      sqrt(T_to / T_from)), rebuilds their lists and evaluates the non-HB
+ *      energies U_r(x_s),
+ *   3. accepts with probability min(1, exp(-D)),
+ *      D = (U_r(x_s) - U_r(x_r)) / T_r + (U_s(x_r) - U_s(x_s)) / T_s
+ *          + (H(x_s) - H(x_r)) (1/T_r - 1/T_s)
+ *      (H = HB energy of each configuration from step 1), with a Philox draw;
+ *   4. restores rejected pairs exactly.
+ * Energies depend on T through lB(T) and the secondary-stack E(T). */
+int tis_exchange_now(tis_ctx *ctx);
+/* Per neighbour pair (r, r + 1), r < n_rep - 1: attempts, accepts and the last D
+ * (0 where no attempt was made); any pointer may be NULL. */
+int tis_get_exchange_stats(tis_ctx *ctx, long long *attempts, long long *accepts, double *last_d, int n_pairs);
+/* Trajectory (walker) id held by each replica slot; initially walker[r] = r. */
+int tis_get_walkers(tis_ctx *ctx, int *walker, int n_rep);
 
 /* Energies of the last force evaluation of replica rep. */
 typedef struct tis_energies {

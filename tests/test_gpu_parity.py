@@ -262,3 +262,102 @@ def test_philox_resolution_is_deterministic(lib, sys1):
     assert np.abs(fa - fb).max() < 1e-12
     a.close()
     b.close()
+
+
+def _replica(lib, T_kcal=0.596, c_mg=0.005, seed=1):
+    d, _ = T.build_16scd(lib, T=T_kcal, c_mg=c_mg, c_k=0.050, seed=seed, box=350.0)
+    return T.SystemHandle(d)
+
+
+def _energy_parts(eng, rep, mask):
+    _, e = eng.eval(mask, rep=rep)
+    nonhb = sum(e[k] for k in ("e2_stack", "e3_stack", "e_wca", "e_coul_sr", "e_coul_lr", "e_bond", "e_angle"))
+    return nonhb, e["e2_hb"] + e["e3_hb"]
+
+
+def test_exchange_exponent_matches_energies_at_two_temperatures(lib):
+    """Two replicas of the 5 mM system at T = 0.596 and 0.640 kcal/mol (lB and the
+    secondary-stack energies depend on T).  The on-device attempt's exponent
+    D = (U_0(x_1) - U_0(x_0))/T_0 + (U_1(x_0) - U_1(x_1))/T_1 + (H(x_1) - H(x_0))(1/T_0 - 1/T_1)
+    must equal the one composed from per-term evaluations of the same states
+    (each per-term energy is itself checked against the FP64 oracle above); an
+    accepted attempt swaps the configurations with velocities scaled by
+    sqrt(T_to / T_from), a rejected one leaves both replicas bit-identical."""
+    temps = (0.596, 0.640)
+    hs = [_replica(lib, t) for t in temps]
+    eng = T.Engine(lib, hs, precision=T.PREC_FP64, seed=3, use_graph=0)
+    eng.step(30)
+    st = [eng.get_state(r) for r in range(2)]
+    kl = [eng.energies(r)["klok"] for r in range(2)]
+    host = T.Engine(lib, hs, precision=T.PREC_FP64, seed=3, use_graph=0)
+    self_e = []
+    for r in range(2):
+        host.set_state(*st[r], klok=kl[r], rep=r)
+        self_e.append(_energy_parts(host, r, T.TERM_ALL))
+    cross = []
+    for r in range(2):
+        host.set_state(*st[1 - r], klok=kl[r], rep=r)
+        cross.append(_energy_parts(host, r, T.TERM_ALL & ~T.TERM_HB)[0])
+    b0, b1 = 1.0 / temps[0], 1.0 / temps[1]
+    d_host = (b0 * (cross[0] - self_e[0][0]) + b1 * (cross[1] - self_e[1][0])
+              + (self_e[1][1] - self_e[0][1]) * (b0 - b1))
+    eng.exchange_now()
+    att, acc, d = eng.exchange_stats()
+    print("D host", d_host, "device", d[0], "accepted", acc[0])
+    assert att[0] == 1
+    assert d[0] == pytest.approx(d_host, rel=1e-9, abs=1e-7)
+    after = [eng.get_state(r) for r in range(2)]
+    if acc[0] == 1:
+        s01 = np.sqrt(temps[0] / temps[1])
+        assert np.abs(after[0][0] - st[1][0]).max() == 0.0
+        assert np.abs(after[0][2] - s01 * st[1][2]).max() < 1e-12
+        assert list(eng.walkers()) == [1, 0]
+    else:
+        for r in range(2):
+            for a, b in zip(after[r], st[r]):
+                assert np.array_equal(a, b)
+        assert list(eng.walkers()) == [0, 1]
+    eng.close()
+    host.close()
+
+
+def test_exchange_between_equal_temperatures_always_swaps(lib):
+    """Same parameters in both slots: the non-HB terms cancel and the HB term has
+    a zero factor, so D is rounding noise and the attempt is accepted; the slots
+    then hold each other's configurations (velocity scale 1)."""
+    h = _replica(lib)
+    eng = T.Engine(lib, [h, h], precision=T.PREC_FP64, seed=9, use_graph=0)
+    eng.step(5)
+    a = eng.get_state(0)
+    other = T.Engine(lib, [h], precision=T.PREC_FP64, seed=21, use_graph=0)
+    other.step(40)
+    b = other.get_state(0)
+    eng.set_state(*b, klok=6, rep=1)
+    eng.exchange_now()
+    att, acc, d = eng.exchange_stats()
+    assert att[0] == 1 and acc[0] == 1 and abs(d[0]) < 1e-6
+    p0, o0, v0 = eng.get_state(0)
+    p1, o1, v1 = eng.get_state(1)
+    assert np.array_equal(p0, b[0]) and np.array_equal(p1, a[0])
+    assert np.array_equal(v0, b[2]) and np.array_equal(v1, a[2])
+    assert np.abs(o0 - b[1]).max() < 1e-12 and np.abs(o1 - a[1]).max() < 1e-12
+    assert list(eng.walkers()) == [1, 0]
+    eng.close()
+    other.close()
+
+
+def test_exchange_cadence_in_steps_and_incompatible_neighbours(lib):
+    """exchange_every = 50: 200 steps make 4 attempts, alternating the pairs
+    (0,1),(2,3) and (1,2); a neighbour with another [Mg] (other particle count)
+    is never paired.  Graph-captured chunks run between the attempts."""
+    hs = [_replica(lib, t) for t in (0.596, 0.605, 0.615, 0.625)] + [_replica(lib, 0.596, c_mg=0.010)]
+    eng = T.Engine(lib, hs, precision=T.PREC_MIXED, seed=4, use_graph=1, graph_steps=25, exchange_every=50)
+    eng.step(200)
+    att, acc, _ = eng.exchange_stats()
+    print("attempts", att, "accepts", acc, "walkers", eng.walkers())
+    assert list(att) == [2, 2, 2, 0]
+    assert sorted(eng.walkers()[:4]) == [0, 1, 2, 3] and eng.walkers()[4] == 4
+    for r in range(5):
+        p, _, v = eng.get_state(r)
+        assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    eng.close()
