@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/tis_gpu.h"
-#include "kernels.h"
+#include "backward.h"
 
 namespace tis_host {
 void set_error(const std::string &msg);
@@ -85,6 +85,13 @@ struct tis_ctx {
     // replica exchange
     long long steps_done = 0;         // steps of tis_step since creation (exchange cadence)
     long long exchange_attempts = 0;
+    bool all_same = false;            // every replica describes the same system
+    // force field as uploaded (for tis_set_force_field) and force-matching buffers
+    std::vector<double> h_hbE, h_st3E;
+    std::vector<double2> h_st2rE;
+    std::vector<std::vector<double>> type_R, type_eps, type_mass, type_q;
+    double4 *fm_target = nullptr, *fm_forces = nullptr;
+    double *fm_loss = nullptr, *fm_ion = nullptr, *fm_ghb = nullptr, *fm_gst3 = nullptr, *fm_gst2 = nullptr;
 
     ~tis_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -196,11 +203,11 @@ struct tis_ctx {
         const int nmb = n_rep + nb_bonded, npr = grid(kPairLanes * D.n_tot);
         const unsigned mb = TIS_TERM_STACK | TIS_TERM_BACKBONE, pr = TIS_TERM_SR | TIS_TERM_LR;
         if (D.precision == TIS_PREC_FP64) {
-            k_force<double, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, 1);
-            k_force<double, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, 2);
+            k_force<double, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, kEvalFull);
+            k_force<double, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
         } else {
-            k_force<float, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, 1);
-            k_force<float, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, 2);
+            k_force<float, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, kEvalFull);
+            k_force<float, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
         }
     }
     // One replica-exchange attempt, all on the device (the sequence documented
@@ -237,14 +244,14 @@ struct tis_ctx {
         launch_hb(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
-            else k_force<float, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, 0);
+            if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
+            else k_force<float, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
         launch_rebuild_lists(s_main);
         timed("k_pairs", s_main, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
-            else k_force<float, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, 0);
+            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
+            else k_force<float, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
@@ -277,6 +284,32 @@ struct tis_ctx {
 namespace {
 
 constexpr double kSigLJh = 1.5852;
+
+// read_maxi_key derived constants (fast.cpp:2895-2944): friction and noise per
+// type (gamma = 0.555 R), and the 9 x 9 pair tables (D_LJ = 3.2 between RNA
+// beads, R_a + R_b otherwise; E_LJ = sqrt(eps_a eps_b); F_LJ = 12 E_LJ / 1.5852^2;
+// Q_C lB).  P.T and P.h must be set.
+void type_tables(RepParams &P, const double *R, const double *eps, const double *q, const double *mass, double lB) {
+    for (int t = 0; t < kNT; ++t) {
+        const double visc = 0.555 * R[t];
+        P.mass[t] = mass[t];
+        P.K1[t] = std::exp(-visc / mass[t] * P.h);
+        P.K2[t] = (1.0 - P.K1[t]) / visc;
+        P.sigma[t] = std::sqrt(2.0 * visc * P.T / P.h);
+    }
+    for (int a = 0; a < kNT; ++a)
+        for (int b = 0; b < kNT; ++b) {
+            const int lo = std::min(a, b), hi = std::max(a, b), ab = a * kNT + b;
+            const double d = (lo <= 5 && hi <= 5) ? 3.2 : R[lo] + R[hi];
+            P.dlj[ab] = d;
+            P.d2lj[ab] = d * d;
+            const double dm = d - 0.4406141861 * kSigLJh;
+            P.d2min[ab] = dm * dm;
+            P.elj[ab] = std::sqrt(eps[lo] * eps[hi]);
+            P.flj[ab] = 12.0 * P.elj[ab] / (kSigLJh * kSigLJh);
+            P.qlb[ab] = q[lo] * q[hi] * lB;
+        }
+}
 
 // True when two replica descriptions are the same system: particle types and
 // classes, bonds, angles, stacks and HB candidates (with their sites) equal.
@@ -418,26 +451,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             P.bond0 = b0; P.nbond = s.n_bonds; P.ang0 = a0; P.nang = s.n_angles;
             P.st30 = s30; P.nst3 = s.n_st3; P.st20 = s20; P.nst2 = s.n_st2;
             P.hb0 = h0; P.nhb = s.n_hb; P.site0 = si0; P.nsite = s.n_hb_sites; P.inc0 = i0; P.ninc = s.n_hb_incid;
-            // read_maxi_key derived constants (fast.cpp:2895-2944)
-            for (int t = 0; t < kNT; ++t) {
-                const double visc = 0.555 * s.type_R[t];
-                P.mass[t] = s.type_mass[t];
-                P.K1[t] = std::exp(-visc / s.type_mass[t] * s.h);
-                P.K2[t] = (1.0 - P.K1[t]) / visc;
-                P.sigma[t] = std::sqrt(2.0 * visc * s.T / s.h);
-            }
-            for (int a = 0; a < kNT; ++a)
-                for (int b = 0; b < kNT; ++b) {
-                    const int lo = std::min(a, b), hi = std::max(a, b), ab = a * kNT + b;
-                    const double d = (lo <= 5 && hi <= 5) ? 3.2 : s.type_R[lo] + s.type_R[hi];
-                    P.dlj[ab] = d;
-                    P.d2lj[ab] = d * d;
-                    const double dm = d - 0.4406141861 * kSigLJh;
-                    P.d2min[ab] = dm * dm;
-                    P.elj[ab] = std::sqrt(s.type_eps[lo] * s.type_eps[hi]);
-                    P.flj[ab] = 12.0 * P.elj[ab] / (kSigLJh * kSigLJh);
-                    P.qlb[ab] = s.type_q[lo] * s.type_q[hi] * s.lB;
-                }
+            type_tables(P, s.type_R, s.type_eps, s.type_q, s.type_mass, s.lB);
             p0 += s.n_particles; b0 += s.n_bonds; a0 += s.n_angles; s30 += s.n_st3; s20 += s.n_st2;
             h0 += s.n_hb; si0 += s.n_hb_sites; i0 += s.n_hb_incid;
             cell0 += P.M[0] * P.M[1] * P.M[2] + 1;
@@ -564,12 +578,21 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.st3_b = dalloc<int>(own, st3b.size()); upload(D.st3_b, st3b);
         D.st3_g = dalloc<double>(own, st3g.size()); upload(D.st3_g, st3g);
         D.st3_E = dalloc<double>(own, st3E.size()); upload(D.st3_E, st3E);
+        c->h_st3E = st3E;
         D.st3_dis = dalloc<int2>(own, st3d.size()); upload(D.st3_dis, st3d);
         D.st2_b = dalloc<int>(own, st2b.size()); upload(D.st2_b, st2b);
         D.st2_rE = dalloc<double2>(own, st2rE.size()); upload(D.st2_rE, st2rE);
+        c->h_st2rE = st2rE;
         D.hb_b = dalloc<int>(own, hbb.size()); upload(D.hb_b, hbb);
         D.hb_g = dalloc<double>(own, hbg.size()); upload(D.hb_g, hbg);
         D.hb_E = dalloc<double>(own, hbE.size()); upload(D.hb_E, hbE);
+        c->h_hbE = hbE;
+        for (int r = 0; r < n_rep; ++r) {
+            c->type_R.emplace_back(reps[r].type_R, reps[r].type_R + kNT);
+            c->type_eps.emplace_back(reps[r].type_eps, reps[r].type_eps + kNT);
+            c->type_mass.emplace_back(reps[r].type_mass, reps[r].type_mass + kNT);
+            c->type_q.emplace_back(reps[r].type_q, reps[r].type_q + kNT);
+        }
         D.hb_dode = dalloc<int>(own, hbd.size()); upload(D.hb_dode, hbd);
         D.hb_inc_off = dalloc<int>(own, hio.size()); upload(D.hb_inc_off, hio);
         D.hb_inc = dalloc<int>(own, hi.size()); upload(D.hb_inc, hi);
@@ -619,6 +642,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             std::vector<int> ok(n_rep, 0), walker(n_rep);
             for (int r = 0; r < n_rep; ++r) walker[r] = r;
             for (int r = 0; r + 1 < n_rep; ++r) ok[r] = same_system(reps[r], reps[r + 1]) ? 1 : 0;
+            c->all_same = true;
+            for (int r = 0; r + 1 < n_rep; ++r) c->all_same = c->all_same && ok[r];
             D.remd_ok = dalloc<int>(own, n_rep); upload(D.remd_ok, ok);
             D.walker = dalloc<int>(own, n_rep); upload(D.walker, walker);
             D.remd_state = dalloc<int>(own, n_rep);
@@ -630,6 +655,16 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             D.remd_stats = dalloc<long long>(own, 2 * (size_t)n_rep);
             HIP_TRY(hipMemset(D.remd_stats, 0, 2 * n_rep * sizeof(long long)));
         }
+        // ---- force matching: tangents, target, outputs
+        D.jv = dalloc<double4>(own, N);
+        D.jf = dalloc<double4>(own, N);
+        c->fm_target = dalloc<double4>(own, N);
+        c->fm_forces = dalloc<double4>(own, N);
+        c->fm_loss = dalloc<double>(own, 1);
+        c->fm_ion = dalloc<double>(own, 3);
+        c->fm_ghb = dalloc<double>(own, std::max<size_t>(1, c->h_hbE.size()));
+        c->fm_gst3 = dalloc<double>(own, std::max<size_t>(1, c->h_st3E.size()));
+        c->fm_gst2 = dalloc<double>(own, std::max<size_t>(1, c->h_st2rE.size()));
         upload(D.ptype, ptype);
         upload(D.prep, prep);
         upload(D.pos, pos);
@@ -812,6 +847,104 @@ extern "C" int tis_step(tis_ctx *c, long long n_steps) {
     });
 }
 
+extern "C" int tis_set_force_field(tis_ctx *c, const double *hb_E, const double *st3_E, const double *st2_E,
+                                   const double *ion_R) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_set_force_field: bad arguments"});
+    if ((hb_E || st3_E || st2_E) && !c->all_same)
+        return fail(Fail{TIS_EINVAL, "tis_set_force_field: per-term parameters need replicas of one system"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        Dev &D = c->D;
+        for (int r = 0; r < c->n_rep; ++r) {
+            const RepParams &P = c->rp[r];
+            if (hb_E) for (int k = 0; k < P.nhb; ++k) c->h_hbE[P.hb0 + k] = hb_E[k];
+            if (st3_E) for (int k = 0; k < P.nst3; ++k) c->h_st3E[P.st30 + k] = st3_E[k];
+            if (st2_E) for (int k = 0; k < P.nst2; ++k) c->h_st2rE[P.st20 + k].y = st2_E[k];
+            if (ion_R) {
+                for (int t = 0; t < 3; ++t) c->type_R[r][6 + t] = ion_R[t];
+                type_tables(c->rp[r], c->type_R[r].data(), c->type_eps[r].data(), c->type_q[r].data(),
+                            c->type_mass[r].data(), P.lB);
+            }
+        }
+        if (hb_E) upload(D.hb_E, c->h_hbE);
+        if (st3_E) upload(D.st3_E, c->h_st3E);
+        if (st2_E) upload(D.st2_rE, c->h_st2rE);
+        if (ion_R) HIP_TRY(hipMemcpy(D.rp, c->rp.data(), c->n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, double *forces, double *grad_pos,
+                               double *grad_hb_E, double *grad_st3_E, double *grad_st2_E, double *grad_ion_R) {
+    if (!c || !target || !loss) return fail(Fail{TIS_EINVAL, "tis_force_match: bad arguments"});
+    if ((grad_hb_E || grad_st3_E || grad_st2_E) && !c->all_same)
+        return fail(Fail{TIS_EINVAL, "tis_force_match: per-term gradients need replicas of one system"});
+    return guarded([&]() -> int {
+        Dev &D = c->D;
+        const int N = D.n_tot;
+        hipStream_t st = c->s_main;
+        HIP_TRY(hipStreamSynchronize(st));
+        std::vector<double4> t4(N);
+        for (int p = 0; p < N; ++p) t4[p] = make_double4(target[3 * p], target[3 * p + 1], target[3 * p + 2], 0.0);
+        HIP_TRY(hipMemcpy(c->fm_target, t4.data(), N * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(c->fm_loss, 0, sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(c->fm_ion, 0, 3 * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(c->fm_ghb, 0, std::max<size_t>(1, c->h_hbE.size()) * sizeof(double), st));
+        // forward: the step's forces with a fresh long-range term
+        k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
+        c->launch_hb(st, 0);
+        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * N);
+        if (D.precision == TIS_PREC_FP64) {
+            k_force<double, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+            k_force<double, 1><<<npr, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+        } else {
+            k_force<float, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+            k_force<float, 1><<<npr, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+        }
+        k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_target, c->fm_forces, c->fm_loss);
+        // backward: force tangents along v = F - target, parameter sums
+        k_jvp_pairs<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_ion);
+        k_jvp_multibody<<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, c->fm_gst3, c->fm_gst2);
+        k_jvp_hb<<<dim3((kActMax + kBlock - 1) / kBlock, c->n_rep), kBlock, 0, st>>>(D, c->fm_ghb);
+        k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));
+        c->check_flags();
+        const double M = 3.0 * N, sc = 2.0 / M;
+        double l = 0;
+        HIP_TRY(hipMemcpy(&l, c->fm_loss, sizeof(double), hipMemcpyDeviceToHost));
+        *loss = l / M;
+        if (forces) {
+            std::vector<double4> f(N);
+            HIP_TRY(hipMemcpy(f.data(), c->fm_forces, N * sizeof(double4), hipMemcpyDeviceToHost));
+            for (int p = 0; p < N; ++p) { forces[3 * p] = f[p].x; forces[3 * p + 1] = f[p].y; forces[3 * p + 2] = f[p].z; }
+        }
+        if (grad_pos) {
+            std::vector<double4> g(N);
+            HIP_TRY(hipMemcpy(g.data(), D.jf, N * sizeof(double4), hipMemcpyDeviceToHost));
+            for (int p = 0; p < N; ++p) {
+                grad_pos[3 * p] = sc * g[p].x; grad_pos[3 * p + 1] = sc * g[p].y; grad_pos[3 * p + 2] = sc * g[p].z;
+            }
+        }
+        auto summed = [&](const double *dev, size_t total, int n_local, int RepParams::*off, double *out) {
+            std::vector<double> h(std::max<size_t>(1, total));
+            HIP_TRY(hipMemcpy(h.data(), dev, std::max<size_t>(1, total) * sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = 0; k < n_local; ++k) out[k] = 0.0;
+            for (int r = 0; r < c->n_rep; ++r)
+                for (int k = 0; k < n_local; ++k) out[k] += sc * h[c->rp[r].*off + k];
+        };
+        if (grad_hb_E) summed(c->fm_ghb, c->h_hbE.size(), c->rp[0].nhb, &RepParams::hb0, grad_hb_E);
+        if (grad_st3_E) summed(c->fm_gst3, c->h_st3E.size(), c->rp[0].nst3, &RepParams::st30, grad_st3_E);
+        if (grad_st2_E) summed(c->fm_gst2, c->h_st2rE.size(), c->rp[0].nst2, &RepParams::st20, grad_st2_E);
+        if (grad_ion_R) {
+            double g3[3];
+            HIP_TRY(hipMemcpy(g3, c->fm_ion, 3 * sizeof(double), hipMemcpyDeviceToHost));
+            for (int t = 0; t < 3; ++t) grad_ion_R[t] = sc * g3[t];
+        }
+        return TIS_OK;
+    });
+}
+
 extern "C" int tis_exchange_now(tis_ctx *c) {
     if (!c) return fail(Fail{TIS_EINVAL, "tis_exchange_now: bad arguments"});
     return guarded([&]() -> int {
@@ -922,11 +1055,11 @@ extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out
         const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
         const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * D.n_tot);
         if (D.precision == TIS_PREC_FP64) {
-            k_force<double, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
-            k_force<double, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+            k_force<double, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
+            k_force<double, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
         } else {
-            k_force<float, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
-            k_force<float, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, 1);
+            k_force<float, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
+            k_force<float, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(c->s_main));

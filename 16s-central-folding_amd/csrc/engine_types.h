@@ -74,6 +74,7 @@ struct Dev {
     // particles (global index)
     double4 *pos, *old, *vel;
     double4 *f_sr, *f_lr, *f_mb;
+    double4 *jv, *jf;             // force matching: position tangent in, force tangent out
     int *ptype;                   // type | cls << 4 | charged << 8
     int *prep;                    // replica of each particle
     // cells and lists

@@ -17,6 +17,7 @@
 //   k_hbsites     (populate_lists :1702-1736)
 #pragma once
 #include "device_math.h"
+#include "dual.h"
 #include "engine_types.h"
 
 namespace tis {
@@ -85,6 +86,23 @@ TIS_DEV void atomic_add_force(double4 *f, int i, V3<R> v) {
     unsafeAtomicAdd(&f[i].y, (double)v.y);
     unsafeAtomicAdd(&f[i].z, (double)v.z);
 }
+// Tangent part of a force with derivatives (the backward pass accumulates d F).
+TIS_DEV void atomic_add_force(double4 *f, int i, V3<Dual> v) {
+    unsafeAtomicAdd(&f[i].x, v.x.d);
+    unsafeAtomicAdd(&f[i].y, v.y.d);
+    unsafeAtomicAdd(&f[i].z, v.z.d);
+}
+
+// Position of particle i as the force routines of precision R see it: the FP64
+// position, with its tangent from D.jv for derivatives.
+template <class R>
+struct PtLoad {
+    TIS_DEV static double4 at(const Dev &D, int i) { return D.pos[i]; }
+};
+template <>
+struct PtLoad<Dual> {
+    TIS_DEV static DualPt at(const Dev &D, int i) { return {D.pos[i], D.jv[i]}; }
+};
 
 TIS_DEV double block_sum_double(double v, double *scratch) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -484,8 +502,9 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
     (void)d_unused;
     const int *b = D.hb_b + 6 * g;
     const double *geo = D.hb_g + 6 * g;
-    const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
-    const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
+    using Pt = typename PtOf<R>::type;
+    const Pt p0 = PtLoad<R>::at(D, b[0]), p1 = PtLoad<R>::at(D, b[1]), p2 = PtLoad<R>::at(D, b[2]);
+    const Pt p3 = PtLoad<R>::at(D, b[3]), p4 = PtLoad<R>::at(D, b[4]), p5 = PtLoad<R>::at(D, b[5]);
     V3<R> v14 = diff<R>(p1, p4);
     R rr = sqrt(dot(v14, v14));
     R c1, c2;
@@ -512,10 +531,14 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
 // Forces of the six-bead Gaussian/Lorentzian-type term: with w the coefficient
 // (-E for HB, -E/(1+Q) for stacks), each harmonic piece k (y - y0)^2 is applied
 // with k scaled by w (stacking_*_force calls, fast.cpp:2043-2055, 2333-2345).
+// With vdotf, also returns the sum over the six beads of (force value) . (position
+// tangent), the derivative of v . F along a factor that scales the whole term.
 template <class R>
 TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double *geo, R w, const R psi0[3],
-                           R kr, R kth, R kpsi) {
-    const double4 p[6] = {D.pos[b[0]], D.pos[b[1]], D.pos[b[2]], D.pos[b[3]], D.pos[b[4]], D.pos[b[5]]};
+                           R kr, R kth, R kpsi, double *vdotf = nullptr) {
+    using Pt = typename PtOf<R>::type;
+    const Pt p[6] = {PtLoad<R>::at(D, b[0]), PtLoad<R>::at(D, b[1]), PtLoad<R>::at(D, b[2]),
+                     PtLoad<R>::at(D, b[3]), PtLoad<R>::at(D, b[4]), PtLoad<R>::at(D, b[5])};
     // the six beads' forces are summed here, then added once each
     V3<R> fb[6];
 #pragma unroll
@@ -557,6 +580,12 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) atomic_add_force(F, b[q], fb[q]);
+    if (vdotf) {
+        double sdot = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sdot += value_dot_tangent(fb[q], p[q]);
+        *vdotf = sdot;
+    }
 }
 
 // Uniform source of one step's HB resolution, with the replica's counters
@@ -1042,15 +1071,140 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     }
 }
 
+// Tertiary stack g (stacking_interactions, fast.cpp:1911-2141): E / (1 + Q6)
+// with Q6 the weighted squared deviations of one distance, two angles and three
+// dihedrals; returns the energy and adds the forces (see sextet_forces for vdotf).
+template <class R>
+TIS_DEV R tertiary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
+    using Pt = typename PtOf<R>::type;
+    const int *b = D.st3_b + 6 * g;
+    const double *geo = D.st3_g + 6 * g;
+    const Pt p0 = PtLoad<R>::at(D, b[0]), p1 = PtLoad<R>::at(D, b[1]), p2 = PtLoad<R>::at(D, b[2]);
+    const Pt p3 = PtLoad<R>::at(D, b[3]), p4 = PtLoad<R>::at(D, b[4]), p5 = PtLoad<R>::at(D, b[5]);
+    V3<R> v14 = diff<R>(p1, p4);
+    R dr = sqrt(dot(v14, v14)) - (R)geo[0];
+    R c1, c2;
+    V3<R> ga, gb;
+    angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
+    angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
+    R th1 = acos(c1), th2 = acos(c2);
+    R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
+    R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
+    R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
+    R psi0[3] = {wrap_target<R>(ps, (R)geo[3]), wrap_target<R>(ps1, (R)geo[4]), wrap_target<R>(ps2, (R)geo[5])};
+    R q = (R)1 + (R)5 * dr * dr;
+    q += (R)1.5 * (th1 - (R)geo[1]) * (th1 - (R)geo[1]);
+    q += (R)1.5 * (th2 - (R)geo[2]) * (th2 - (R)geo[2]);
+    q += (R)0.15 * (ps - psi0[0]) * (ps - psi0[0]);
+    q += (R)0.15 * (ps1 - psi0[1]) * (ps1 - psi0[1]);
+    q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
+    const R e = (R)D.st3_E[g] / q;
+    sextet_forces<R>(D, F, b, geo, -e / q, psi0, (R)5, (R)1.5, (R)0.15, vdotf);
+    return e;
+}
+
+// Secondary stack g (stacking_interactions, fast.cpp:2060-2141): E / (1 + Q3)
+// over the B-B distance and two backbone dihedrals; returns the energy and adds
+// the forces (vdotf as in sextet_forces).
+template <class R>
+TIS_DEV R secondary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
+    using Pt = typename PtOf<R>::type;
+    const int *b = D.st2_b + 8 * g;   // P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2
+    const double2 rE = D.st2_rE[g];
+    const Pt q0 = PtLoad<R>::at(D, b[0]), q1 = PtLoad<R>::at(D, b[1]), q2 = PtLoad<R>::at(D, b[2]);
+    const Pt q3 = PtLoad<R>::at(D, b[3]), q4 = PtLoad<R>::at(D, b[4]), q5 = PtLoad<R>::at(D, b[5]);
+    const Pt q6 = PtLoad<R>::at(D, b[6]);
+    V3<R> v25 = diff<R>(q2, q5);
+    R rr = sqrt(dot(v25, v25));
+    V3<R> g1[4], g2[4];
+    R ps1 = dihedral_grad<R>(diff<R>(q0, q1), diff<R>(q1, q3), diff<R>(q3, q4), g1);
+    R ps2 = dihedral_grad<R>(diff<R>(q6, q4), diff<R>(q4, q3), diff<R>(q3, q1), g2);
+    R t1 = wrap_target<R>(ps1, (R)-2.58684), t2 = wrap_target<R>(ps2, (R)3.07135);
+    R q = (R)1 + (R)1.4 * (rr - (R)rE.x) * (rr - (R)rE.x);
+    q += (R)4 * (ps1 - t1) * (ps1 - t1);
+    q += (R)4 * (ps2 - t2) * (ps2 - t2);
+    const R e = (R)rE.y / q;
+    const R w = -e / q;
+    const R tb = (R)2 * ((R)1.4 * w) * ((R)rE.x - rr) / rr;
+    double sdot = 0.0;
+    const V3<R> f2 = v25 * (-tb), f5 = v25 * tb;
+    atomic_add_force(F, b[2], f2);
+    atomic_add_force(F, b[5], f5);
+    sdot += value_dot_tangent(f2, q2) + value_dot_tangent(f5, q5);
+    const R c1 = (R)2 * ((R)4 * w) * (t1 - ps1), c2 = (R)2 * ((R)4 * w) * (t2 - ps2);
+    const int i1[4] = {b[0], b[1], b[3], b[4]}, i2[4] = {b[6], b[4], b[3], b[1]};
+    const Pt p1[4] = {q0, q1, q3, q4}, p2[4] = {q6, q4, q3, q1};
+    for (int c = 0; c < 4; ++c) {
+        const V3<R> fa = g1[c] * c1, fb = g2[c] * c2;
+        atomic_add_force(F, i1[c], fa);
+        atomic_add_force(F, i2[c], fb);
+        sdot += value_dot_tangent(fa, p1[c]) + value_dot_tangent(fb, p2[c]);
+    }
+    if (vdotf) *vdotf = sdot;
+    return e;
+}
+
+// Bond or angle g of the concatenated bonded list (polymer_constraints,
+// fast.cpp:2354-2520: bead_bond_force k (r - r0)^2, bead_valence_force
+// k (theta - theta0)^2): adds the forces, returns the energy in *eb or *ea and the
+// end beads (i, k) whose WCA the reference removes; false past the list's end.
+template <class R>
+TIS_DEV bool bonded_term(const Dev &D, double4 *F, int g, int nbond, int nang, int *i_out, int *k_out, double *eb,
+                         double *ea) {
+    if (g < nbond) {
+        const int2 ij = D.bond_ij[g];
+        const double2 kr = D.bond_kr[g];
+        V3<R> d = diff<R>(PtLoad<R>::at(D, ij.x), PtLoad<R>::at(D, ij.y));
+        R rr;
+        R t = bond_coef<R>(d, (R)kr.x, (R)kr.y, &rr);
+        atomic_add_force(F, ij.x, d * (-t));
+        atomic_add_force(F, ij.y, d * t);
+        *eb = kr.x * ((double)rr - kr.y) * ((double)rr - kr.y);
+        *i_out = ij.x;
+        *k_out = ij.y;
+        return true;
+    }
+    if (g < nbond + nang) {
+        const int4 t3 = D.ang_ijk[g - nbond];
+        const double2 kt = D.ang_kt[g - nbond];
+        const int i = t3.x, j = t3.y, k = t3.z;
+        const auto pj = PtLoad<R>::at(D, j);
+        V3<R> ba = diff<R>(pj, PtLoad<R>::at(D, i)), bc = diff<R>(pj, PtLoad<R>::at(D, k));
+        R c;
+        V3<R> g0, g2;
+        angle_grad<R>(ba, bc, &c, &g0, &g2);
+        R sn = sqrt((R)1 - c * c), th = acos(c);
+        R coef = (R)-2 * (R)kt.x * ((R)kt.y - th) / sn;
+        V3<R> f0 = g0 * coef, f2 = g2 * coef;
+        atomic_add_force(F, i, f0);
+        atomic_add_force(F, k, f2);
+        atomic_add_force(F, j, (f0 + f2) * (R)-1);
+        *ea = kt.x * ((double)th - kt.y) * ((double)th - kt.y);
+        *i_out = i;
+        *k_out = k;
+        return true;
+    }
+    return false;
+}
+
 // ------------------------------------------------------------ k_force
 constexpr int kPairLanes = 4;   // lanes per particle in the pair gather
 // ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
 // backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
 // that the register-heavy stacking code does not lower the occupancy of the
 // latency-bound gather; they run concurrently on two streams.
+// eval: kEvalStep (a step: no energies, bonded / 1-3 WCA left out, LR only when
+// due), kEvalFull (per-term evaluation with the reference's split: bonded WCA
+// added in the pair sum and removed in the bonded terms, energies, LR always),
+// kEvalEnergy (as kEvalFull without writing pair forces), kEvalForce (the step's
+// forces with a fresh LR term, no energies).
+enum { kEvalStep = 0, kEvalFull = 1, kEvalEnergy = 2, kEvalForce = 3 };
 template <class R, int ROLE>
 __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned mask, int eval) {
     double4 *F = D.f_mb;
+    const bool want_e = eval == kEvalFull || eval == kEvalEnergy;
+    const bool split = want_e;
+    const bool write_f = eval != kEvalEnergy;
     if (ROLE == 0 && (int)blockIdx.x < D.n_rep) {
         if (!(mask & 1)) return;
         const int r = blockIdx.x;
@@ -1076,61 +1230,13 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         for (int k = threadIdx.x; k < P.nst3 + P.nst2; k += blockDim.x) {
             if (k < P.nst3) {
                 const int g = P.st30 + k;
-                const int *b = D.st3_b + 6 * g;
-                const double *geo = D.st3_g + 6 * g;
-                const double4 p0 = D.pos[b[0]], p1 = D.pos[b[1]], p2 = D.pos[b[2]];
-                const double4 p3 = D.pos[b[3]], p4 = D.pos[b[4]], p5 = D.pos[b[5]];
-                V3<R> v14 = diff<R>(p1, p4);
-                R dr = sqrt(dot(v14, v14)) - (R)geo[0];
-                R c1, c2;
-                V3<R> ga, gb;
-                angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
-                angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
-                R th1 = acos(c1), th2 = acos(c2);
-                R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
-                R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
-                R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
-                R psi0[3] = {wrap_target<R>(ps, (R)geo[3]), wrap_target<R>(ps1, (R)geo[4]),
-                             wrap_target<R>(ps2, (R)geo[5])};
-                R q = (R)1 + (R)5 * dr * dr;
-                q += (R)1.5 * (th1 - (R)geo[1]) * (th1 - (R)geo[1]);
-                q += (R)1.5 * (th2 - (R)geo[2]) * (th2 - (R)geo[2]);
-                q += (R)0.15 * (ps - psi0[0]) * (ps - psi0[0]);
-                q += (R)0.15 * (ps1 - psi0[1]) * (ps1 - psi0[1]);
-                q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
-                const R e = (R)D.st3_E[g] / q;
-                D.st_energy[g] = (double)e;
-                e3 += (double)e;
-                sextet_forces<R>(D, F, b, geo, -e / q, psi0, (R)5, (R)1.5, (R)0.15);
+                const double e = (double)tertiary_stack<R>(D, F, g, nullptr);
+                D.st_energy[g] = e;
+                e3 += e;
             } else {
                 const int m = k - P.nst3;
                 if (off[m]) continue;
-                const int g = P.st20 + m;
-                const int *b = D.st2_b + 8 * g;   // P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2
-                const double2 rE = D.st2_rE[g];
-                const double4 q0 = D.pos[b[0]], q1 = D.pos[b[1]], q2 = D.pos[b[2]], q3 = D.pos[b[3]];
-                const double4 q4 = D.pos[b[4]], q5 = D.pos[b[5]], q6 = D.pos[b[6]];
-                V3<R> v25 = diff<R>(q2, q5);
-                R rr = sqrt(dot(v25, v25));
-                V3<R> g1[4], g2[4];
-                R ps1 = dihedral_grad<R>(diff<R>(q0, q1), diff<R>(q1, q3), diff<R>(q3, q4), g1);
-                R ps2 = dihedral_grad<R>(diff<R>(q6, q4), diff<R>(q4, q3), diff<R>(q3, q1), g2);
-                R t1 = wrap_target<R>(ps1, (R)-2.58684), t2 = wrap_target<R>(ps2, (R)3.07135);
-                R q = (R)1 + (R)1.4 * (rr - (R)rE.x) * (rr - (R)rE.x);
-                q += (R)4 * (ps1 - t1) * (ps1 - t1);
-                q += (R)4 * (ps2 - t2) * (ps2 - t2);
-                const R e = (R)rE.y / q;
-                e2 += (double)e;
-                const R w = -e / q;
-                const R tb = (R)2 * ((R)1.4 * w) * ((R)rE.x - rr) / rr;
-                atomic_add_force(F, b[2], v25 * (-tb));
-                atomic_add_force(F, b[5], v25 * tb);
-                const R c1 = (R)2 * ((R)4 * w) * (t1 - ps1), c2 = (R)2 * ((R)4 * w) * (t2 - ps2);
-                const int i1[4] = {b[0], b[1], b[3], b[4]}, i2[4] = {b[6], b[4], b[3], b[1]};
-                for (int c = 0; c < 4; ++c) {
-                    atomic_add_force(F, i1[c], g1[c] * c1);
-                    atomic_add_force(F, i2[c], g2[c] * c2);
-                }
+                e2 += (double)secondary_stack<R>(D, F, P.st20 + m, nullptr);
             }
         }
         double s2 = block_sum_double(e2, red);
@@ -1147,41 +1253,13 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         const int nang = D.rp[D.n_rep - 1].ang0 + D.rp[D.n_rep - 1].nang;
         int i, k;
         double eb = 0, ea = 0;
-        if (g < nbond) {
-            const int2 ij = D.bond_ij[g];
-            const double2 kr = D.bond_kr[g];
-            i = ij.x; k = ij.y;
-            V3<R> d = diff<R>(D.pos[i], D.pos[k]);
-            R rr;
-            R t = bond_coef<R>(d, (R)kr.x, (R)kr.y, &rr);
-            atomic_add_force(F, i, d * (-t));
-            atomic_add_force(F, k, d * t);
-            if (eval) eb = kr.x * ((double)rr - kr.y) * ((double)rr - kr.y);
-        } else if (g < nbond + nang) {
-            const int4 t3 = D.ang_ijk[g - nbond];
-            const double2 kt = D.ang_kt[g - nbond];
-            i = t3.x; k = t3.z;
-            const int j = t3.y;
-            V3<R> ba = diff<R>(D.pos[j], D.pos[i]), bc = diff<R>(D.pos[j], D.pos[k]);
-            R c;
-            V3<R> g0, g2;
-            angle_grad<R>(ba, bc, &c, &g0, &g2);
-            R s = sqrt((R)1 - c * c), th = acos(c);
-            R coef = (R)-2 * (R)kt.x * ((R)kt.y - th) / s;
-            V3<R> f0 = g0 * coef, f2 = g2 * coef;
-            atomic_add_force(F, i, f0);
-            atomic_add_force(F, k, f2);
-            atomic_add_force(F, j, (f0 + f2) * (R)-1);
-            if (eval) ea = kt.x * ((double)th - kt.y) * ((double)th - kt.y);
-        } else {
-            return;
-        }
+        if (!bonded_term<R>(D, F, g, nbond, nang, &i, &k, &eb, &ea)) return;
         // remove the WCA force of the bonded / 1-3 pair (reduce_forces, fast.cpp:880-897).
         // The step path leaves that pair's WCA out of the pair sum instead, which is
         // the same force without the cancellation of two terms that diverge as the
         // shifted WCA distance r + 1.5852 - D_LJ goes to 0; a pair listed by several
         // terms is still removed once per extra term.
-        if (!eval && !D.bsub[g]) return;   // energies are only summed in evaluations
+        if (!split && !D.bsub[g]) return;   // energies are only summed in evaluations
         const int r = D.prep[i];
         const RepParams &P = D.rp[r];
         double3 dd;
@@ -1191,7 +1269,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         V3<R> v = {(R)dd.x * f, (R)dd.y * f, (R)dd.z * f};
         atomic_add_force(F, i, v);
         atomic_add_force(F, k, v * (R)-1);
-        if (eval) {
+        if (want_e) {
             RepState &S = D.rs[r];
             unsafeAtomicAdd(&S.e[EI_BOND], eb);
             unsafeAtomicAdd(&S.e[EI_ANGLE], ea);
@@ -1233,21 +1311,21 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             // a bonded / 1-3 pair carries no net WCA force (the reference adds it
             // here and removes it in the bonded terms); only evaluations, which
             // report the reference's split by term, compute it
-            R f = (bonded && !eval) ? (R)0 : wca_factor<R>(r2, ab, P);
+            R f = (bonded && !split) ? (R)0 : wca_factor<R>(r2, ab, P);
             const bool both = chg_i && ((ptj >> 8) & 1) && r2 <= kCoulCut2;
             if (both) {
                 const R rr = sqrt((R)r2);
                 f += (R)P.qlb[ab] / (rr * (R)r2);
             }
             fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
-            if (eval) {
+            if (want_e) {
                 ew += wca_energy(r2, ab, P);
                 if (both) ec += P.qlb[ab] / sqrt(r2);
             }
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
-        if (valid && sub == 0 && eval != 2) D.f_sr[p] = make_double4(gx, gy, gz, 0.0);
-        if (eval) {
+        if (valid && sub == 0 && write_f) D.f_sr[p] = make_double4(gx, gy, gz, 0.0);
+        if (want_e) {
             ew = group_sum(ew);
             ec = group_sum(ec);
             if (valid && sub == 0) {
@@ -1258,7 +1336,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     }
     // the LR refresh is decided per replica; a group never spans two particles,
     // but a wavefront may span two replicas, so the test stays per lane
-    const bool lr = (mask & 16) && (eval || S.lr_due);
+    const bool lr = (mask & 16) && (eval != kEvalStep || S.lr_due);
     if (__ballot(lr)) {
         R fx = 0, fy = 0, fz = 0;
         double ec = 0;
@@ -1273,11 +1351,11 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             const R rr = sqrt((R)r2);
             const R f = (R)P.qlb[ab] / (rr * (R)r2);
             fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
-            if (eval) ec += P.qlb[ab] / sqrt(r2);
+            if (want_e) ec += P.qlb[ab] / sqrt(r2);
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
-        if (lr && valid && sub == 0 && eval != 2) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);
-        if (eval) {
+        if (lr && valid && sub == 0 && write_f) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);
+        if (want_e) {
             ec = group_sum(ec);
             if (lr && valid && sub == 0) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
         }

@@ -140,6 +140,8 @@ def load_library(path=LIB_PATH):
         lib.tis_get_exchange_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _dp,
                                                C.c_int]
         lib.tis_get_walkers.argtypes = [C.c_void_p, _ip, C.c_int]
+        lib.tis_force_match.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.tis_set_force_field.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     return lib
 
 
@@ -335,6 +337,38 @@ class Engine:
         check(self.lib, self.lib.tis_get_walkers(self.ctx, w.ctypes.data_as(_ip), len(self.handles)),
               "tis_get_walkers")
         return w
+
+    def n_total(self):
+        return sum(h.d["n_particles"] for h in self.handles)
+
+    def force_match(self, target, want_grads=True):
+        """Loss, forces and gradients of tis_force_match (target: (N_total, 3))."""
+        d0 = self.handles[0].d
+        n = self.n_total()
+        tgt = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        loss = np.zeros(1)
+        f = np.zeros(3 * n)
+        out = {"forces": f}
+        ptrs = [None] * 5
+        if want_grads:
+            out.update(pos=np.zeros(3 * n), hb_E=np.zeros(max(1, d0["n_hb"])), st3_E=np.zeros(max(1, d0["n_st3"])),
+                       st2_E=np.zeros(max(1, d0["n_st2"])), ion_R=np.zeros(3))
+            ptrs = [out[k].ctypes.data_as(_dp) for k in ("pos", "hb_E", "st3_E", "st2_E", "ion_R")]
+        check(self.lib, self.lib.tis_force_match(self.ctx, tgt.ctypes.data_as(_dp), loss.ctypes.data_as(_dp),
+                                                 f.ctypes.data_as(_dp), *ptrs), "tis_force_match")
+        out["loss"] = float(loss[0])
+        out["forces"] = f.reshape(-1, 3)
+        if want_grads:
+            out["pos"] = out["pos"].reshape(-1, 3)
+            out["hb_E"] = out["hb_E"][:d0["n_hb"]]
+            out["st3_E"] = out["st3_E"][:d0["n_st3"]]
+            out["st2_E"] = out["st2_E"][:d0["n_st2"]]
+        return out
+
+    def set_force_field(self, hb_E=None, st3_E=None, st2_E=None, ion_R=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (hb_E, st3_E, st2_E, ion_R)]
+        ptrs = [None if a is None else a.ctypes.data_as(_dp) for a in arrs]
+        check(self.lib, self.lib.tis_set_force_field(self.ctx, *ptrs), "tis_set_force_field")
 
     def set_profiling(self, on):
         check(self.lib, self.lib.tis_set_profiling(self.ctx, int(on)), "tis_set_profiling")

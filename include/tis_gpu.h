@@ -204,6 +204,34 @@ int tis_get_exchange_stats(tis_ctx *ctx, long long *attempts, long long *accepts
 /* Trajectory (walker) id held by each replica slot; initially walker[r] = r. */
 int tis_get_walkers(tis_ctx *ctx, int *walker, int n_rep);
 
+/* Force matching (no counterpart in the reference; the differentiable path of
+ * the batched engine).  With F the forces a step uses (HB after the valence
+ * resolution, whose outcome is held fixed for the derivatives; bonded and 1-3
+ * pairs without WCA; the long-range term evaluated afresh) at every replica's
+ * current state and target the caller's forces (3 * N_total doubles, replicas
+ * concatenated),
+ *   loss = (1/M) sum |F - target|^2,  M = 3 * N_total,
+ * and, for each non-NULL output:
+ *   forces     F (3 * N_total);
+ *   grad_pos   dloss/dx (3 * N_total) = (2/M) J (F - target), J = dF/dx (a
+ *              Hessian-vector product, by forward-mode derivatives of the force
+ *              kernels);
+ *   grad_hb_E  dloss/d(hb_E[k]) for the n_hb candidates (summed over replicas);
+ *   grad_st3_E, grad_st2_E  dloss/d(E) of the tertiary / secondary stacks (the
+ *              stacking free energy enters as E = -dG + ..., so dloss/d(dG) = -this);
+ *   grad_ion_R dloss/d(R) for the LJ radii of Mg, Cl, K (through D_LJ = R_a + R_b).
+ * Per-term gradients need replicas that describe one system (the snapshots of a
+ * force-matching set).  Each replica's lists and HB tables are those of its
+ * current state (tis_set_state rebuilds them). */
+int tis_force_match(tis_ctx *ctx, const double *target, double *loss, double *forces, double *grad_pos,
+                    double *grad_hb_E, double *grad_st3_E, double *grad_st2_E, double *grad_ion_R);
+/* Replaces force-field parameters in every replica: HB strengths (n_hb), tertiary
+ * and secondary stack energies (n_st3, n_st2) and the LJ radii of Mg, Cl, K (which
+ * also set their friction, gamma = 0.555 R, as read_maxi_key does, fast.cpp:2897).
+ * NULL leaves a set unchanged; per-term sets need replicas of one system. */
+int tis_set_force_field(tis_ctx *ctx, const double *hb_E, const double *st3_E, const double *st2_E,
+                        const double *ion_R);
+
 /* Energies of the last force evaluation of replica rep. */
 typedef struct tis_energies {
     double e2_hb, e3_hb, e2_stack, e3_stack;   /* as in the Energies file, fast.cpp:6312 */
