@@ -9,6 +9,7 @@
 
 namespace tis_host {
 tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws);
+tis_system *build_synthetic(const std::string &dir, const tis_synthetic_args &args);
 
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
@@ -32,6 +33,31 @@ extern "C" int tis_build_16scd(const char *data_dir, const tis_build_args *args,
         return TIS_EIO;
     } catch (const std::bad_alloc &) {
         tis_host::set_error("tis_build_16scd: out of host memory");
+        return TIS_ENOMEM;
+    } catch (const std::exception &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    }
+}
+
+extern "C" int tis_build_synthetic(const char *data_dir, const tis_synthetic_args *args, tis_system **out) {
+    if (!args || !out) { tis_host::set_error("tis_build_synthetic: null argument"); return TIS_EINVAL; }
+    *out = nullptr;
+    const tis_build_args &b = args->build;
+    if (args->n_nt < 8 || !(args->radius > 0) || !(args->stack_frac >= 0 && args->stack_frac <= 1) ||
+        args->n_hb < 0 || !(b.T > 0) || b.c_mg < 0 || b.c_k < 0 || !(b.box >= 5 * args->radius)) {
+        tis_host::set_error("tis_build_synthetic: need n_nt >= 8, radius > 0, 0 <= stack_frac <= 1, "
+                            "n_hb >= 0, T > 0, concentrations >= 0 and box >= 5 * radius");
+        return TIS_EINVAL;
+    }
+    try {
+        *out = tis_host::build_synthetic(data_dir ? data_dir : ".", *args);
+        return TIS_OK;
+    } catch (const tis_host::IoError &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    } catch (const std::bad_alloc &) {
+        tis_host::set_error("tis_build_synthetic: out of host memory");
         return TIS_ENOMEM;
     } catch (const std::exception &e) {
         tis_host::set_error(e.what());

@@ -18,6 +18,7 @@
 // Internally the tables are 1-based like the reference's (slot 0 unused), so
 // the unit arithmetic (2*RES-1, atom_key[i+1][0], ...) reads the same; they are
 // converted to the 0-based tis_system at the end.
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -27,6 +28,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/tis_gpu.h"
@@ -203,6 +205,7 @@ struct Builder {
     std::vector<int> st_excess;
     double box = 350.0, volume = 0, half[3]{};
     int n_ion[3]{};
+    int kh = 500;                 // recentring bead (1-based), fast.cpp:6163
 
     Vec3 P(int k) const { return {x[k], y[k], z[k]}; }
 
@@ -644,19 +647,27 @@ struct Builder {
 // Backbone bond/angle lists in the order polymer_constraints visits them
 // (fast.cpp:2360-2519): base bond + two B-S-P angles per nucleoside, then
 // S(n)-P(n+1) bonds with P-S-P angles, then P-S bonds with S-P-S angles.
+// Base bond and B-S-P angles per base: r0, theta0 towards P(n+1), theta0 towards P(n).
+struct BaseParam { char k; double r0, a1, a2; };
+const BaseParam kBase[4] = {{'A', 4.8515, 1.9259, 1.7029},
+                            {'C', 4.2738, 1.9655, 1.5803},
+                            {'G', 4.9659, 1.9150, 1.7690},
+                            {'U', 4.2733, 1.9663, 1.5735}};
+const BaseParam *base_param(char k) {
+    for (const BaseParam &q : kBase)
+        if (q.k == k) return &q;
+    return nullptr;
+}
+// Sugar-phosphate backbone: S(n)-P(n+1) and P(n)-S(n) bond lengths, the
+// P-S-P angle at S and the S-P-S angle at P.
+constexpr double kBondSP = 3.8157, kBondPS = 4.6010, kAngleS = 1.4440, kAngleP = 1.5256;
+
 void backbone_terms(const Builder &b, std::vector<int> &bij, std::vector<double> &bkr,
                     std::vector<int> &aijk, std::vector<double> &akt) {
-    struct BaseParam { char k; double r0, a1, a2; };
-    static const BaseParam kBase[4] = {{'A', 4.8515, 1.9259, 1.7029},
-                                       {'C', 4.2738, 1.9655, 1.5803},
-                                       {'G', 4.9659, 1.9150, 1.7690},
-                                       {'U', 4.2733, 1.9663, 1.5735}};
     int n = b.n_units;
     for (int e = 1; e <= n; e += 2) {
         int i = b.key0[e], j = b.key1[e];
-        const BaseParam *p = nullptr;
-        for (const BaseParam &q : kBase)
-            if (q.k == b.kind[e]) p = &q;
+        const BaseParam *p = base_param(b.kind[e]);
         if (!p) continue;
         bij.insert(bij.end(), {i - 1, j - 1});
         bkr.insert(bkr.end(), {10.0, p->r0});
@@ -674,19 +685,19 @@ void backbone_terms(const Builder &b, std::vector<int> &bij, std::vector<double>
     for (int e = 2; e <= n; e += 2) {
         int i = b.key0[e], j = b.key0[e - 1];
         bij.insert(bij.end(), {i - 1, j - 1});
-        bkr.insert(bkr.end(), {64.0, 3.8157});
+        bkr.insert(bkr.end(), {64.0, kBondSP});
         if (e > 2) {
             int k = b.key0[e - 2];
             aijk.insert(aijk.end(), {i - 1, j - 1, k - 1});
-            akt.insert(akt.end(), {20.0, 1.4440});
+            akt.insert(akt.end(), {20.0, kAngleS});
         }
     }
     for (int e = 3; e <= n; e += 2) {
         int i = b.key0[e], j = b.key0[e - 1], k = b.key0[e - 2];
         bij.insert(bij.end(), {i - 1, j - 1});
-        bkr.insert(bkr.end(), {23.0, 4.6010});
+        bkr.insert(bkr.end(), {23.0, kBondPS});
         aijk.insert(aijk.end(), {i - 1, j - 1, k - 1});
-        akt.insert(akt.end(), {20.0, 1.5256});
+        akt.insert(akt.end(), {20.0, kAngleP});
     }
 }
 
@@ -710,7 +721,7 @@ tis_system *export_system(const Builder &b) {
     s->T = b.args.T;
     s->lB = 332.0637090 * (0.03273600947 * b.args.T - 0.00669625750);
     s->h = 0.05;
-    s->recentre = 499;
+    s->recentre = b.kh - 1;
     for (int t = 0; t < TIS_NTYPES; ++t) {
         s->type_mass[t] = b.mass[t + 1]; s->type_R[t] = b.rlj[t + 1];
         s->type_eps[t] = b.elj[t + 1]; s->type_q[t] = b.qc[t + 1];
@@ -801,6 +812,358 @@ tis_system *export_system(const Builder &b) {
     return s;
 }
 
+// ------------------------------------------------------------ synthetic chains
+//
+// The reference builds only 16SCD, from its PDB and hand-written HB/stack
+// lists with hard-coded residue offsets (fast.cpp:3137-4492).  A synthetic
+// chain of any length gets the same tables from a self-avoiding walk instead:
+// units and beads in the reference's order (S1 B1 P2 S2 B2 ...), every bond
+// and bond angle at the equilibrium value of its backbone term, secondary
+// stacks on a random subset of adjacent nucleotide pairs (same parameter
+// table as initialize_unprocessed_stacks), and HB candidates between bases
+// the walk brings into contact, their native geometry measured on the built
+// chain the way initialize_unprocessed_bonds measures it on the PDB.
+
+Vec3 add(const Vec3 &a, const Vec3 &b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+Vec3 sub(const Vec3 &a, const Vec3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+Vec3 scale(const Vec3 &a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double dot(const Vec3 &a, const Vec3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3 crossv(const Vec3 &a, const Vec3 &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+Vec3 unit(const Vec3 &a) { return scale(a, 1.0 / std::sqrt(dot(a, a))); }
+
+// Closest approach the walk allows between beads that are neither bonded nor
+// 1-3 partners (the RNA-RNA WCA repels below D_LJ = 3.2 A).
+constexpr double kClash = 3.5;
+
+class ChainGrower {
+  public:
+    ChainGrower(Builder &b, double radius, double clash)
+        : b_(b), radius_(radius), clash_(clash), clash2_(clash * clash) {}
+
+    // Sequence, units and beads, then positions.  Step n places S(n) (n > 1),
+    // P(n+1) (n < N) and B(n); a step that finds no free position undoes
+    // itself and the walk backs up, further each time it fails again before
+    // reaching new ground.
+    void grow(int n_nt) {
+        static const char kSeq[4] = {'A', 'C', 'G', 'U'};
+        std::vector<char> seq(n_nt + 1);
+        for (int n = 1; n <= n_nt; ++n) seq[n] = kSeq[std::min(3, (int)(b_.uniform() * 4.0))];
+        b_.kind.assign(1, 0);
+        for (int n = 1; n <= n_nt; ++n) {
+            if (n > 1) b_.kind.push_back('P');
+            b_.kind.push_back(seq[n]);
+        }
+        b_.n_units = (int)b_.kind.size() - 1;
+        b_.key0.assign(b_.n_units + 1, 0);
+        b_.key1.assign(b_.n_units + 1, 0);
+        for (int u = 1; u <= b_.n_units; ++u) {
+            if (b_.kind[u] == 'P') {
+                b_.key0[u] = b_.new_particle(0, 1);
+                b_.key1[u] = -1;
+            } else {
+                char k = b_.kind[u];
+                b_.key0[u] = b_.new_particle(1, 2);
+                b_.key1[u] = b_.new_particle(2, k == 'A' ? 3 : k == 'G' ? 4 : k == 'C' ? 5 : 6);
+            }
+        }
+        b_.n_beads = (int)b_.x.size() - 1;
+        n_nt_ = n_nt;
+
+        put(S(1), {0, 0, 0});
+        std::vector<size_t> mark(n_nt + 2, 0);
+        budget_ = 2000LL * n_nt;
+        int n = 1, best = 0, depth = 1;
+        while (n <= n_nt) {
+            mark[n] = placed_.size();
+            if (step(n)) {
+                if (n > best) { best = n; depth = 1; }
+                ++n;
+                continue;
+            }
+            undo_to(mark[n]);
+            if (budget_ < 0)
+                throw IoError("synthetic chain: the walk found no free position after nucleotide " +
+                              std::to_string(best) + " (radius too small)");
+            int back = std::min(depth, n - 1);
+            n -= back;
+            undo_to(mark[n]);
+            depth = std::min(2 * depth, 64);
+        }
+    }
+
+  private:
+    Builder &b_;
+    double radius_, clash_, clash2_;
+    int n_nt_ = 0;
+    long long budget_ = 0;        // trial positions left before the walk gives up
+    std::unordered_map<long long, std::vector<int>> grid_;
+    std::vector<int> placed_;
+
+    int S(int n) const { return b_.key0[2 * n - 1]; }
+    int B(int n) const { return b_.key1[2 * n - 1]; }
+    int Pn(int n) const { return b_.key0[2 * n - 2]; }   // n >= 2
+
+    // The base's position follows from S(n) and the two phosphates, so a
+    // blocked base is retried with new dihedrals for S(n) and P(n+1).
+    bool step(int n) {
+        for (int t = 0; t < 32; ++t) {
+            size_t m = placed_.size();
+            if (n > 1 && !place_backbone(S(n), n > 2 ? Pn(n - 1) : -1, S(n - 1), Pn(n), kBondPS, kAngleP,
+                                         {Pn(n), S(n - 1)}))
+                return false;
+            if (n < n_nt_ && !place_backbone(Pn(n + 1), n > 1 ? S(n - 1) : -1, n > 1 ? Pn(n) : -1, S(n),
+                                             kBondSP, kAngleS, {S(n), n > 1 ? Pn(n) : -1})) {
+                undo_to(m);
+                continue;
+            }
+            if (place_base(n, S(n), B(n), n > 1 ? Pn(n) : -1, n < n_nt_ ? Pn(n + 1) : -1)) return true;
+            undo_to(m);
+        }
+        return false;
+    }
+
+    long long cell_key(int cx, int cy, int cz) const {
+        const long long o = 1 << 20;
+        return ((cx + o) << 42) | ((cy + o) << 21) | (cz + o);
+    }
+    int cell_of(double v) const { return (int)std::floor(v / clash_); }
+    long long key_of(int k) const { return cell_key(cell_of(b_.x[k]), cell_of(b_.y[k]), cell_of(b_.z[k])); }
+    void put(int k, const Vec3 &p) {
+        b_.x[k] = p.x; b_.y[k] = p.y; b_.z[k] = p.z;
+        grid_[key_of(k)].push_back(k);
+        placed_.push_back(k);
+    }
+    void undo_to(size_t mark) {
+        while (placed_.size() > mark) {
+            int k = placed_.back();
+            placed_.pop_back();
+            std::vector<int> &c = grid_[key_of(k)];
+            c.erase(std::find(c.begin(), c.end(), k));
+        }
+    }
+    // True when p is closer than the clash distance to a placed bead other
+    // than the listed bonded / 1-3 partners.
+    bool clashes(const Vec3 &p, std::initializer_list<int> skip) {
+        if (--budget_ < 0) return true;
+        int cx = cell_of(p.x), cy = cell_of(p.y), cz = cell_of(p.z);
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    auto it = grid_.find(cell_key(cx + dx, cy + dy, cz + dz));
+                    if (it == grid_.end()) continue;
+                    for (int j : it->second) {
+                        bool skipped = false;
+                        for (int s : skip) skipped |= s == j;
+                        if (skipped) continue;
+                        Vec3 d = sub(p, b_.P(j));
+                        if (dot(d, d) < clash2_) return true;
+                    }
+                }
+        return false;
+    }
+    Vec3 random_unit() {
+        double z = 2.0 * b_.uniform() - 1.0, ph = 2.0 * kPi * b_.uniform();
+        double s = std::sqrt(std::max(0.0, 1.0 - z * z));
+        return {s * std::cos(ph), s * std::sin(ph), z};
+    }
+    // Orthonormal pair perpendicular to the unit vector u.
+    void frame(const Vec3 &u, Vec3 &e1, Vec3 &e2) {
+        Vec3 t = std::fabs(u.x) < 0.9 ? Vec3{1, 0, 0} : Vec3{0, 1, 0};
+        e1 = unit(crossv(u, t));
+        e2 = crossv(u, e1);
+    }
+    bool inside(const Vec3 &p, double r) const { return dot(p, p) <= r * r; }
+
+    // Bead k at distance r from c, with angle theta at c towards bd and a
+    // random dihedral about (bd, c); a is the bead before bd, or -1 at the
+    // chain start (then the dihedral reference is a random direction).
+    // 64 random dihedrals; the last 16 may leave the sphere by up to 25%.
+    bool place_backbone(int k, int a, int bd, int c, double r, double theta,
+                        std::initializer_list<int> skip) {
+        Vec3 C = b_.P(c);
+        for (int t = 0; t < 64; ++t) {
+            Vec3 p;
+            if (bd < 0) {
+                p = add(C, scale(random_unit(), r));
+            } else {
+                Vec3 Bv = b_.P(bd);
+                Vec3 bc = unit(sub(C, Bv));
+                Vec3 ref = a >= 0 ? sub(Bv, b_.P(a)) : random_unit();
+                Vec3 nrm = crossv(ref, bc);
+                if (dot(nrm, nrm) < 1e-12) continue;
+                nrm = unit(nrm);
+                Vec3 m = crossv(nrm, bc);
+                double ph = 2.0 * kPi * b_.uniform();
+                Vec3 d = add(scale(bc, -r * std::cos(theta)),
+                             add(scale(m, r * std::sin(theta) * std::cos(ph)),
+                                 scale(nrm, r * std::sin(theta) * std::sin(ph))));
+                p = add(C, d);
+            }
+            if (!inside(p, t < 48 ? radius_ : 1.25 * radius_) || clashes(p, skip)) continue;
+            put(k, p);
+            return true;
+        }
+        return false;
+    }
+
+    // Base bead of nucleotide n: bond to S at the base's r0 with both B-S-P
+    // angles at their equilibrium values where both phosphates exist (two
+    // mirror solutions), else on the cone of the one angle that exists.
+    bool place_base(int n, int s, int bk, int p_prev, int p_next) {
+        const BaseParam *bp = base_param(b_.kind[2 * n - 1]);
+        Vec3 Sv = b_.P(s);
+        if (p_prev >= 0 && p_next >= 0) {
+            Vec3 u1 = unit(sub(b_.P(p_prev), Sv)), u2 = unit(sub(b_.P(p_next), Sv));
+            double c = dot(u1, u2), c1 = std::cos(bp->a2), c2 = std::cos(bp->a1);
+            double den = 1.0 - c * c;
+            if (!(den > 1e-9)) return false;
+            double al = (c1 - c * c2) / den, be = (c2 - c * c1) / den;
+            Vec3 inplane = add(scale(u1, al), scale(u2, be));
+            Vec3 nrm = crossv(u1, u2);
+            double g2 = (1.0 - dot(inplane, inplane)) / dot(nrm, nrm);
+            if (g2 < 0) return false;
+            double g = std::sqrt(g2);
+            bool flip = b_.uniform() < 0.5;
+            for (int q = 0; q < 2; ++q) {
+                Vec3 dir = add(inplane, scale(nrm, (flip != (q == 1)) ? -g : g));
+                Vec3 p = add(Sv, scale(dir, bp->r0));
+                if (inside(p, 1.25 * radius_) && !clashes(p, {s, p_prev, p_next})) { put(bk, p); return true; }
+            }
+            return false;
+        }
+        // Chain ends: on the cone about the one phosphate direction.
+        int axis_bead = p_next >= 0 ? p_next : p_prev;
+        double ang = p_next >= 0 ? bp->a1 : bp->a2;
+        Vec3 u = unit(sub(b_.P(axis_bead), Sv)), e1, e2;
+        frame(u, e1, e2);
+        for (int t = 0; t < 64; ++t) {
+            double ps = 2.0 * kPi * b_.uniform();
+            Vec3 dir = add(scale(u, std::cos(ang)),
+                           add(scale(e1, std::sin(ang) * std::cos(ps)), scale(e2, std::sin(ang) * std::sin(ps))));
+            Vec3 p = add(Sv, scale(dir, bp->r0));
+            if (!inside(p, 1.25 * radius_) || clashes(p, {s, p_prev, p_next})) continue;
+            put(bk, p);
+            return true;
+        }
+        return false;
+    }
+};
+
+// Secondary stacks on a random subset of adjacent pairs of units (i, i+2),
+// i = 3, 5, ... as initialize_unprocessed_stacks enumerates them
+// (fast.cpp:4399-4429); ST_EXCESS as there, with no tertiary stacks.
+void synthetic_stacks(Builder &b, double frac) {
+    b.st_excess.assign(b.n_units + 1, 0);
+    for (int i = 3; i < b.n_units - 2; i += 2) {
+        if (!(b.uniform() < frac)) continue;
+        const SecStackParam *p = nullptr;
+        for (const SecStackParam &q : kSecStack)
+            if (q.a == b.kind[i] && q.b == b.kind[i + 2]) p = &q;
+        Builder::St2 s{};
+        s.unit = i;
+        s.r0 = p->r0;
+        s.E = -p->dG + b.args.ss_D / p->c + p->s * (b.args.T - p->Tm);
+        b.st2.push_back(s);
+        b.st_excess[i] += 1;
+        b.st_excess[i + 1] += 1;
+    }
+    for (int f = 1; f <= b.n_units; ++f) b.st_excess[f] -= 1;
+}
+
+// HB candidates between nucleotides i < j (j >= i + 4, both with a 3'
+// phosphate) whose base beads lie r_lo..r_hi apart: pairs drawn in random
+// order, each nucleotide in at most two candidates, up to n_hb.  A candidate
+// joins 2 atom-level bonds (3 for G-C) between distinct base sites of the two
+// nucleotides, E = -(bonds) hs_D kHbUnit as summed in
+// initialize_unprocessed_bonds (fast.cpp:3881-3890); tertiary class when the
+// nucleotides are more than 50 apart in sequence.
+void synthetic_hbonds(Builder &b, int n_nt, int n_hb, double r_lo, double r_hi) {
+    auto S = [&](int n) { return b.key0[2 * n - 1]; };
+    auto Bb = [&](int n) { return b.key1[2 * n - 1]; };
+    auto P3 = [&](int n) { return b.key0[2 * n]; };
+    std::unordered_map<long long, std::vector<int>> cells;
+    auto key = [](int cx, int cy, int cz) {
+        const long long o = 1 << 20;
+        return ((cx + o) << 42) | ((cy + o) << 21) | (cz + o);
+    };
+    auto cidx = [&](double v) { return (int)std::floor(v / r_hi); };
+    for (int n = 1; n < n_nt; ++n) {
+        Vec3 p = b.P(Bb(n));
+        cells[key(cidx(p.x), cidx(p.y), cidx(p.z))].push_back(n);
+    }
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 1; i < n_nt; ++i) {
+        Vec3 p = b.P(Bb(i));
+        int cx = cidx(p.x), cy = cidx(p.y), cz = cidx(p.z);
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    auto it = cells.find(key(cx + dx, cy + dy, cz + dz));
+                    if (it == cells.end()) continue;
+                    for (int j : it->second) {
+                        if (j < i + 4) continue;
+                        double r = dist(p, b.P(Bb(j)));
+                        if (r >= r_lo && r <= r_hi) pairs.emplace_back(i, j);
+                    }
+                }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    for (int q = (int)pairs.size() - 1; q > 0; --q) {
+        int r = std::min(q, (int)(b.uniform() * (q + 1)));
+        std::swap(pairs[q], pairs[r]);
+    }
+    std::vector<int> uses(n_nt + 1, 0);
+    std::vector<std::pair<int, int>> take;
+    for (const auto &pr : pairs) {
+        if ((int)take.size() >= n_hb) break;
+        if (uses[pr.first] >= 2 || uses[pr.second] >= 2) continue;
+        ++uses[pr.first];
+        ++uses[pr.second];
+        take.push_back(pr);
+    }
+    std::sort(take.begin(), take.end());
+    auto pick_slots = [&](int unit_idx, int m, int *out) {
+        // base sites are slots 2..ns of the nucleotide (slot 0 O2', 1 O4')
+        std::vector<int> slots;
+        for (int q = 2; q < (int)b.unit_sites[unit_idx].size(); ++q) slots.push_back(q);
+        for (int q = 0; q < m; ++q) {
+            int r = q + std::min((int)slots.size() - 1 - q, (int)(b.uniform() * (slots.size() - q)));
+            std::swap(slots[q], slots[r]);
+            out[q] = slots[q];
+        }
+    };
+    for (const auto &pr : take) {
+        int i = pr.first, j = pr.second;
+        char ki = b.kind[2 * i - 1], kj = b.kind[2 * j - 1];
+        int m = ((ki == 'G' && kj == 'C') || (ki == 'C' && kj == 'G')) ? 3 : 2;
+        Builder::Hb h{};
+        int bb[6] = {S(i), Bb(i), P3(i), S(j), Bb(j), P3(j)};
+        std::memcpy(h.b, bb, sizeof bb);
+        h.g[0] = dist(b.P(bb[1]), b.P(bb[4]));
+        h.g[1] = angle(b.P(bb[0]), b.P(bb[1]), b.P(bb[4]));
+        h.g[2] = angle(b.P(bb[3]), b.P(bb[4]), b.P(bb[1]));
+        h.g[3] = dihedral(b.P(bb[0]), b.P(bb[1]), b.P(bb[4]), b.P(bb[3]));
+        h.g[4] = dihedral(b.P(bb[4]), b.P(bb[1]), b.P(bb[0]), b.P(bb[2]));
+        h.g[5] = dihedral(b.P(bb[1]), b.P(bb[4]), b.P(bb[3]), b.P(bb[5]));
+        h.E = -m * b.args.hs_D * kHbUnit;
+        h.dode = j - i > 50 ? 1 : 0;
+        int si[3], sj[3];
+        pick_slots(2 * i - 1, m, si);
+        pick_slots(2 * j - 1, m, sj);
+        int k = (int)b.hb.size();
+        for (int q = 0; q < m; ++q) {
+            int s1 = b.unit_sites[2 * i - 1][si[q]], s2 = b.unit_sites[2 * j - 1][sj[q]];
+            h.sites.push_back(s1);
+            h.sites.push_back(s2);
+            b.site_hbs[s1].push_back(k);
+            b.site_hbs[s2].push_back(k);
+        }
+        b.hb.push_back(h);
+    }
+}
+
 }  // namespace
 
 tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws) {
@@ -814,10 +1177,32 @@ tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long
     b.init_hb_sites();
     b.read_hb_candidates(d + "/16SCD_16SCD_unprocessed_bonds.dat");
     b.read_stacks(d + "/16SCD_16SCD_unprocessed_stacks.dat");
-    if (b.n_beads < 500) throw IoError("chain shorter than the recentring bead 500");
-    b.set_box(500);
+    if (b.n_beads < b.kh) throw IoError("chain shorter than the recentring bead 500");
+    b.set_box(b.kh);
     b.place_ions();
     if (draws) *draws = b.rng.draws();
+    return export_system(b);
+}
+
+tis_system *build_synthetic(const std::string &dir, const tis_synthetic_args &a) {
+    Builder b;
+    b.args = a.build;
+    b.box = a.build.box;
+    b.rng.reseed((unsigned)(a.build.T * 1000 * a.build.seed));
+    b.read_maxi_key((dir.empty() ? std::string(".") : dir) + "/MAXIKEY_Mg_Cl_K.dat");
+    ChainGrower grower(b, a.radius, kClash);
+    grower.grow(a.n_nt);
+    b.init_hb_sites();
+    synthetic_stacks(b, a.stack_frac);
+    synthetic_hbonds(b, a.n_nt, a.n_hb, 5.0, 7.0);
+    // velocities unit by unit, S before B, as read_PDB draws them (fast.cpp:3625-3627)
+    for (int u = 1; u <= b.n_units; ++u) {
+        b.draw_velocity(b.key0[u]);
+        if (b.kind[u] != 'P') b.draw_velocity(b.key1[u]);
+    }
+    b.kh = (b.n_beads + 1) / 2;
+    b.set_box(b.kh);
+    b.place_ions();
     return export_system(b);
 }
 

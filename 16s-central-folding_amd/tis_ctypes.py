@@ -47,6 +47,11 @@ class BuildArgs(C.Structure):
                 ("seed", C.c_int), ("box", C.c_double)]
 
 
+class SyntheticArgs(C.Structure):
+    _fields_ = [("n_nt", C.c_int), ("stack_frac", C.c_double), ("n_hb", C.c_int),
+                ("radius", C.c_double), ("build", BuildArgs)]
+
+
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
                 ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
@@ -110,6 +115,7 @@ def load_library(path=LIB_PATH):
     lib.tis_last_error.restype = C.c_char_p
     lib.tis_build_16scd.argtypes = [C.c_char_p, C.POINTER(BuildArgs), C.POINTER(C.POINTER(TisSystem)),
                                     C.POINTER(C.c_longlong)]
+    lib.tis_build_synthetic.argtypes = [C.c_char_p, C.POINTER(SyntheticArgs), C.POINTER(C.POINTER(TisSystem))]
     lib.tis_system_free.argtypes = [C.POINTER(TisSystem)]
     lib.tis_system_free.restype = None
     lib.tis_glibc_rand_fill.argtypes = [C.c_uint, C.c_longlong, C.POINTER(C.c_int32), C.c_longlong]
@@ -210,6 +216,18 @@ def build_16scd(lib, T=0.615, c_mg=0.005, c_k=0.050, seed=1, box=350.0,
     finally:
         lib.tis_system_free(out)
     return d, draws.value
+
+
+def build_synthetic(lib, n_nt=1500, stack_frac=0.6, n_hb=400, radius=75.0, T=0.596, c_mg=0.010,
+                    c_k=0.100, seed=1, box=600.0, ss_D=0.850, hs_D=3.975, st_D=6.5, data_dir=DATA_16SCD):
+    """Runs tis_build_synthetic (a random-sequence self-avoiding chain) and returns the system dict."""
+    args = SyntheticArgs(n_nt, stack_frac, n_hb, radius, BuildArgs(ss_D, hs_D, st_D, T, c_mg, c_k, seed, box))
+    out = C.POINTER(TisSystem)()
+    check(lib, lib.tis_build_synthetic(data_dir.encode(), C.byref(args), C.byref(out)), "tis_build_synthetic")
+    try:
+        return system_to_dict(out)
+    finally:
+        lib.tis_system_free(out)
 
 
 def glibc_stream(lib, seed, n, skip=0):

@@ -5,12 +5,20 @@ system, replica k in a box of 350 + 100 k / 15 A with c_Mg = 0.5 * 40^(k/15) mM
 MgCl2 and 50 mM KCl (3,308 .. 9,469 particles, 86,655 in all), T = 0.596
 kcal/mol (300 K), seeds 1..16, FP32 pair and multi-body forces with FP64
 positions, velocities and sums, counter-based Philox noise, one hipGraph per
-50 steps.  Replicas are independent (no exchange moves yet).
+50 steps.  The replicas differ in ion count, so no exchange moves are
+attempted between them (an exchange needs two copies of one system).
 
 Printed: one JSON line.  value = aggregate replica-steps per second of the
 batch.  Beside it: the plain FP64 one-replica-per-launch path on the 3,659
 particle 5 mM system (no graph, FP64 forces), the roofline of the dominant
 kernel from HIP-event timings, and the FP64 CPU oracle timed on this host.
+
+--workload chain1500 measures the large synthetic chain instead: one replica
+of a seeded 1,500-nt random-sequence chain (self-avoiding walk in a sphere of
+75 A, secondary stacks on ~60% of adjacent pairs, 400 HB candidates) in a
+600 A box with 10 mM MgCl2 + 100 mM KCl (32,914 particles), same precision,
+noise and graph settings; its line reports steps/s, ns/day, the whole-step
+HBM rate and, from the committed counter summary, the VALU issue fraction.
 """
 import argparse
 import json
@@ -171,6 +179,87 @@ def cpu_baseline(systems, steps_each):
     return total_steps / total_t
 
 
+def measured_valu():
+    """Per-kernel VALU issue fraction on the chain workload from the committed
+    counter summary (SQ_INSTS_VALU per dispatch over the dispatch's duration
+    times 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per cycle), or None."""
+    path = os.path.join(REPO, "profiles", "pmc_valu_chain1500.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def run_chain(args, lib):
+    d = T.build_synthetic(lib, n_nt=1500, stack_frac=0.6, n_hb=400, radius=75.0, T=0.596,
+                          c_mg=0.010, c_k=0.100, seed=1, box=600.0)
+    log(f"built the synthetic chain: {d['n_particles']} particles")
+    h = T.SystemHandle(d)
+    eng = T.Engine(lib, [h], precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
+                   use_graph=0 if args.no_graph else 1, graph_steps=50)
+    eng.step(args.warmup)
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    dt = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {dt:.2f} s")
+    value = args.steps / dt
+    kt, dominant, avg_ms, bytes_launch, achieved = {}, None, None, None, None
+    if args.profile_steps > 0:
+        eng.set_profiling(True)
+        eng.step(args.profile_steps)
+        kt = eng.kernel_times()
+        eng.set_profiling(False)
+        dominant = max(kt, key=lambda k: kt[k][0] * kt[k][1])
+        _, avg_ms = kt[dominant]
+        bytes_launch = algorithmic_bytes(eng, [h], dominant)
+        achieved = None if bytes_launch is None else bytes_launch / (avg_ms * 1e-3) / 1e9
+    e = eng.energies()
+    b_step = step_bytes(eng, [h])
+    eng.close()
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline leg (oracle)")
+        v = cpu_baseline([d], args.cpu_steps)
+        cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 CPU oracle (reference cell lists and force routines, glibc-stream noise), "
+                         f"{args.cpu_steps} steps of the same chain system, single thread; the reference's "
+                         f"own set-up reads only the 16SCD inputs, so its step loop cannot run this system"}
+    valu = measured_valu()
+    out = {
+        "metric": "Langevin steps/s, large synthetic chain + ions, single replica",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic: seeded random-sequence self-avoiding chain, ions placed by the reference's seeded rule",
+        "config": {"workload": "1,500-nt synthetic chain, 600 A box, 10 mM MgCl2 + 100 mM KCl, T=0.596, "
+                               "stacks on ~60% of adjacent pairs, 400 HB candidates",
+                   "replicas": 1, "particles": d["n_particles"], "precision": "mixed", "rng": "philox",
+                   "graph_steps": 50},
+        "ns_per_day": value * FS_PER_STEP * 86400 * 1e-6,
+        "roofline": None if dominant is None else {
+            "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+            "traffic": None, "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
+        "roofline_step": {"bound": "hbm", "bytes_per_step": b_step, "achieved": b_step * value / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
+        "valu_issue_frac": None if valu is None else {k: v["valu_frac"] for k, v in valu.items()},
+        "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
+        "hb_active": e["n_hb_active"],
+        "rebuilds": e["n_rebuilds"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -178,6 +267,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["batched16", "chain1500"], default="batched16")
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=250)
     ap.add_argument("--profile-steps", type=int, default=300)
@@ -196,6 +286,9 @@ def main():
               "multi-GPU runs (replica sharding over RCCL) are not part of this build.", file=sys.stderr)
         sys.exit(2)
     lib = T.load_library()
+    if args.workload == "chain1500":
+        run_chain(args, lib)
+        return
     systems = batched_systems(lib)
     log(f"built {len(systems)} replicas")
     handles = [T.SystemHandle(d) for d in systems]

@@ -126,6 +126,31 @@ typedef struct tis_build_args {
  */
 int tis_build_16scd(const char *data_dir, const tis_build_args *args,
                     tis_system **out, long long *rand_draws_out);
+
+/* A seeded random-sequence chain of any length. */
+typedef struct tis_synthetic_args {
+    int n_nt;            /* nucleotides (>= 8); no 5' phosphate, as in 16SCD */
+    double stack_frac;   /* probability that an adjacent nucleotide pair carries a secondary stack */
+    int n_hb;            /* HB candidates wanted (fewer if the walk makes fewer base contacts) */
+    double radius;       /* radius (A) of the sphere the walk grows in; box >= 5 * radius */
+    tis_build_args build;/* ss_D hs_D st_D T c_mg c_k seed box, meaning as for tis_build_16scd */
+} tis_synthetic_args;
+
+/*
+ * Builds a synthetic chain in place of the 16SCD input files: the reference's
+ * set-up (read_PDB, initialize_HB_atoms, initialize_unprocessed_bonds/stacks,
+ * fast.cpp:3137-4492) is hard-wired to 16SCD's residue numbering
+ * (fast.cpp:3783-3795) and stack-face exceptions (fast.cpp:4458-4491); this
+ * generalises it to n_nt nucleotides.  The chain is a self-avoiding walk grown
+ * inside a sphere (bonds and bond angles at the backbone terms' equilibrium
+ * values, no two other beads closer than 3.5 A), with secondary stacks on a
+ * random stack_frac of adjacent pairs and HB candidates between bases that the
+ * walk puts 5-7 A apart (each nucleotide in at most two).  Then, as for
+ * 16SCD: velocities, set_box on the middle bead, ions by add_crowder.  Only
+ * MAXIKEY_Mg_Cl_K.dat is read from data_dir.  All draws come from the glibc
+ * stream seeded with srand((unsigned)(T*1000*seed)).
+ */
+int tis_build_synthetic(const char *data_dir, const tis_synthetic_args *args, tis_system **out);
 void tis_system_free(tis_system *sys);
 
 /* The glibc rand() stream (TYPE_3 additive generator) after srand(seed), values
