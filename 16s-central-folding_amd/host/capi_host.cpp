@@ -10,6 +10,7 @@
 namespace tis_host {
 tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws);
 tis_system *build_synthetic(const std::string &dir, const tis_synthetic_args &args);
+tis_system *build_chain(const tis_chain_files &f, const tis_build_args &args, long long *draws);
 
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
@@ -33,6 +34,33 @@ extern "C" int tis_build_16scd(const char *data_dir, const tis_build_args *args,
         return TIS_EIO;
     } catch (const std::bad_alloc &) {
         tis_host::set_error("tis_build_16scd: out of host memory");
+        return TIS_ENOMEM;
+    } catch (const std::exception &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    }
+}
+
+extern "C" int tis_build_chain(const tis_chain_files *files, const tis_build_args *args, tis_system **out,
+                               long long *rand_draws_out) {
+    if (!files || !args || !out || !files->pdb || !files->bonds || !files->stacks || !files->maxikey ||
+        (files->n_double_face && !files->double_face_res) || files->n_double_face < 0) {
+        tis_host::set_error("tis_build_chain: null or inconsistent argument");
+        return TIS_EINVAL;
+    }
+    *out = nullptr;
+    if (!(args->T > 0) || args->c_mg < 0 || args->c_k < 0 || !(args->box >= 0)) {
+        tis_host::set_error("tis_build_chain: T must be > 0 and concentrations >= 0");
+        return TIS_EINVAL;
+    }
+    try {
+        *out = tis_host::build_chain(*files, *args, rand_draws_out);
+        return TIS_OK;
+    } catch (const tis_host::IoError &e) {
+        tis_host::set_error(e.what());
+        return TIS_EIO;
+    } catch (const std::bad_alloc &) {
+        tis_host::set_error("tis_build_chain: out of host memory");
         return TIS_ENOMEM;
     } catch (const std::exception &e) {
         tis_host::set_error(e.what());

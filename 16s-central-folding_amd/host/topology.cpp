@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -74,7 +75,7 @@ class TokenFile {
 
 // One PDB record as the reference sees it: tokens 1..9 of a line.
 struct PdbRecord {
-    std::string name, resname;
+    std::string name, resname, resnum;
     std::string xs, ys, zs;
 };
 
@@ -88,7 +89,7 @@ std::vector<PdbRecord> read_pdb_records(const std::string &path) {
         if (got == 0) break;
         if (got < 9) throw IoError("truncated PDB record in " + path);
         tf.skip_line();
-        recs.push_back({t[2], t[3], t[6], t[7], t[8]});
+        recs.push_back({t[2], t[3], t[5], t[6], t[7], t[8]});
     }
     return recs;
 }
@@ -167,16 +168,10 @@ const CanonicalHb kCanonical[10] = {
     {"CAN52", {6.4922, 1.3569, 2.0579, 3.1147, 1.5278, -1.4796}},
 };
 
-// Residues whose 5' face takes part in two tertiary stacks (hard-coded for
-// 16SCD in fast.cpp:4458-4491): ST_EXCESS of face 2*(res-off)-2 drops by 2.
-const int kDoubleFaceRes[15][2] = {
-    {583, 561}, {587, 561}, {658, 561}, {687, 561}, {700, 561}, {701, 561}, {721, 561},
-    {727, 561}, {760, 561}, {782, 561}, {802, 561}, {814, 561}, {881, 567}, {892, 567},
-    {898, 567}};
-
-// 16SCD residue numbering -> nucleotide index: the chain skips 842-847
-// (fast.cpp:3783-3795, 4295-4305).
-int residue_index(int res) { return res < 842 ? res - 561 : res - 567; }
+// 16SCD residues (PDB numbering) whose 5' stacking face, unit 2*n-2 of
+// nucleotide n, has ST_EXCESS lowered by 2 instead of 1: the list the
+// reference hard-codes in fast.cpp:4458-4491.
+const int k16scdDoubleFace[15] = {583, 587, 658, 687, 700, 701, 721, 727, 760, 782, 802, 814, 881, 892, 898};
 
 struct Builder {
     tis_build_args args{};
@@ -206,6 +201,24 @@ struct Builder {
     double box = 350.0, volume = 0, half[3]{};
     int n_ion[3]{};
     int kh = 500;                 // recentring bead (1-based), fast.cpp:6163
+    // PDB residue number -> nucleotide index (1-based) in chain order.  The
+    // reference hard-codes 16SCD's: res - 561, or res - 567 from 842 on
+    // (fast.cpp:3783-3795, 4295-4305), so a number inside the 842-847 gap
+    // takes the offset of the segment after the gap.  The same rule in general:
+    // a listed number maps to its PDB position, a number missing from the PDB
+    // to the offset of the next number the PDB lists (of the last one past
+    // the end), and the result must be a nucleotide of the chain.
+    std::map<int, int> res_index;
+
+    int residue(int res) const {
+        if (res_index.empty()) throw IoError("no residues read from the PDB");
+        auto it = res_index.lower_bound(res);
+        if (it == res_index.end()) it = std::prev(res_index.end());
+        int idx = res - (it->first - it->second);
+        if (idx < 1 || idx > (int)res_index.size())
+            throw IoError("residue " + std::to_string(res) + " lies outside the chain");
+        return idx;
+    }
 
     Vec3 P(int k) const { return {x[k], y[k], z[k]}; }
 
@@ -275,6 +288,9 @@ struct Builder {
                     k = r.resname[0];
                 if (!k) throw IoError("unknown nucleotide " + r.resname);
                 kind.push_back(k);
+                int nt = (int)res_index.size() + 1;
+                if (!res_index.emplace(std::atoi(r.resnum.c_str()), nt).second)
+                    throw IoError("residue " + r.resnum + " appears twice in the PDB");
             }
         }
         n_units = (int)kind.size() - 1;
@@ -369,14 +385,25 @@ struct Builder {
         site_hbs.assign(valence.size(), {});
     }
 
+    // Beads of unit u; an interaction that needs a unit past either chain end
+    // (the 5' phosphate of nucleotide 1, the 3' units of the last one) is an
+    // input error.
+    int bead0(int u) const {
+        if (u < 1 || u > n_units)
+            throw IoError("an HB or stack needs unit " + std::to_string(u) + ", outside the chain's 1.." +
+                          std::to_string(n_units));
+        return key0[u];
+    }
+    int bead1(int u) const { return bead0(u), key1[u]; }
+
     // Bead triple (k?0, k?1, k?2) of an HB atom by its chemical class (fast.cpp:3809-3863).
     void hb_triple(const std::string &atom, int res, int out[3]) const {
         if (atom == "OP1" || atom == "OP2") {
-            out[0] = key0[2 * res - 1]; out[1] = key0[2 * res - 2]; out[2] = key0[2 * res];
+            out[0] = bead0(2 * res - 1); out[1] = bead0(2 * res - 2); out[2] = bead0(2 * res);
         } else if (atom == "O2'" || atom == "O4'") {
-            out[0] = key0[2 * res]; out[1] = key0[2 * res - 1]; out[2] = key0[2 * res + 1];
+            out[0] = bead0(2 * res); out[1] = bead0(2 * res - 1); out[2] = bead0(2 * res + 1);
         } else {
-            out[0] = key0[2 * res - 1]; out[1] = key1[2 * res - 1]; out[2] = key0[2 * res];
+            out[0] = bead0(2 * res - 1); out[1] = bead1(2 * res - 1); out[2] = bead0(2 * res);
         }
     }
     int hb_site(const std::string &atom, int res) const {
@@ -390,8 +417,8 @@ struct Builder {
         std::string code, r1s, r2s, a1, a2;
         while (tf.next(code)) {
             if (!tf.next(r1s) || !tf.next(r2s) || !tf.next(a1)) throw IoError("short HB line in " + path);
-            int res1 = residue_index(std::atoi(r1s.c_str()));
-            int res2 = residue_index(std::atoi(r2s.c_str()));
+            int res1 = residue(std::atoi(r1s.c_str()));
+            int res2 = residue(std::atoi(r2s.c_str()));
             int first_of_line = (int)hb.size();   // merging only inside one line
             const CanonicalHb *can = nullptr;
             for (const CanonicalHb &c : kCanonical)
@@ -437,19 +464,19 @@ struct Builder {
     }
 
     // initialize_unprocessed_stacks (fast.cpp:4232-4492).
-    void read_stacks(const std::string &path) {
+    void read_stacks(const std::string &path, const std::vector<int> &double_face_res) {
         TokenFile tf(path);
         st_excess.assign(n_units + 1, 0);
         std::string s1, s2;
         while (tf.next(s1)) {
             if (!tf.next(s2)) throw IoError("odd token count in " + path);
-            int res1 = residue_index(std::atoi(s1.c_str() + 1));
-            int res2 = residue_index(std::atoi(s2.c_str() + 1));
+            int res1 = residue(std::atoi(s1.c_str() + 1));
+            int res2 = residue(std::atoi(s2.c_str() + 1));
             St3 s{};
             s.unit_i = 2 * res1 - 1;
             s.unit_j = 2 * res2 - 1;
             int i = s.unit_i, j = s.unit_j;
-            int b[6] = {key0[i], key1[i], key0[i + 1], key0[j], key1[j], key0[j + 1]};
+            int b[6] = {bead0(i), bead1(i), bead0(i + 1), bead0(j), bead1(j), bead0(j + 1)};
             std::memcpy(s.b, b, sizeof b);
             s.g[0] = dist(P(b[1]), P(b[4]));
             s.g[1] = angle(P(b[0]), P(b[1]), P(b[4]));
@@ -477,12 +504,12 @@ struct Builder {
             st_excess[i] += 1;
             st_excess[i + 1] += 1;
         }
-        for (int f = 1; f <= n_units; ++f) {
-            int drop = 1;
-            for (const auto &d : kDoubleFaceRes)
-                if (f == 2 * (d[0] - d[1]) - 2) drop = 2;
-            st_excess[f] -= drop;
+        std::vector<int> drop(n_units + 1, 1);
+        for (int r : double_face_res) {
+            int f = 2 * residue(r) - 2;
+            if (f >= 1) drop[f] = 2;
         }
+        for (int f = 1; f <= n_units; ++f) st_excess[f] -= drop[f];
     }
 
     // set_box (fast.cpp:4542-5051): rotate the chain onto its principal axes
@@ -1166,22 +1193,34 @@ void synthetic_hbonds(Builder &b, int n_nt, int n_hb, double r_lo, double r_hi) 
 
 }  // namespace
 
-tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws) {
+tis_system *build_chain(const tis_chain_files &f, const tis_build_args &args, long long *draws) {
     Builder b;
     b.args = args;
     b.box = args.box > 0 ? args.box : 350.0;
+    b.kh = f.recentre_bead > 0 ? f.recentre_bead : 500;
     b.rng.reseed((unsigned)(args.T * 1000 * args.seed));
-    std::string d = dir.empty() ? std::string(".") : dir;
-    b.read_maxi_key(d + "/MAXIKEY_Mg_Cl_K.dat");
-    b.read_pdb(d + "/16SCD.pdb");
+    b.read_maxi_key(f.maxikey);
+    b.read_pdb(f.pdb);
     b.init_hb_sites();
-    b.read_hb_candidates(d + "/16SCD_16SCD_unprocessed_bonds.dat");
-    b.read_stacks(d + "/16SCD_16SCD_unprocessed_stacks.dat");
-    if (b.n_beads < b.kh) throw IoError("chain shorter than the recentring bead 500");
+    b.read_hb_candidates(f.bonds);
+    std::vector<int> faces;
+    if (f.double_face_res) faces.assign(f.double_face_res, f.double_face_res + f.n_double_face);
+    b.read_stacks(f.stacks, faces);
+    if (b.n_beads < b.kh)
+        throw IoError("chain of " + std::to_string(b.n_beads) + " beads is shorter than the recentring bead " +
+                      std::to_string(b.kh));
     b.set_box(b.kh);
     b.place_ions();
     if (draws) *draws = b.rng.draws();
     return export_system(b);
+}
+
+tis_system *build_16scd(const std::string &dir, const tis_build_args &args, long long *draws) {
+    std::string d = dir.empty() ? std::string(".") : dir;
+    std::string pdb = d + "/16SCD.pdb", bonds = d + "/16SCD_16SCD_unprocessed_bonds.dat",
+                stacks = d + "/16SCD_16SCD_unprocessed_stacks.dat", maxi = d + "/MAXIKEY_Mg_Cl_K.dat";
+    tis_chain_files f{pdb.c_str(), bonds.c_str(), stacks.c_str(), maxi.c_str(), k16scdDoubleFace, 15, 500};
+    return build_chain(f, args, draws);
 }
 
 tis_system *build_synthetic(const std::string &dir, const tis_synthetic_args &a) {

@@ -47,6 +47,11 @@ class BuildArgs(C.Structure):
                 ("seed", C.c_int), ("box", C.c_double)]
 
 
+class ChainFiles(C.Structure):
+    _fields_ = [("pdb", C.c_char_p), ("bonds", C.c_char_p), ("stacks", C.c_char_p), ("maxikey", C.c_char_p),
+                ("double_face_res", _ip), ("n_double_face", C.c_int), ("recentre_bead", C.c_int)]
+
+
 class SyntheticArgs(C.Structure):
     _fields_ = [("n_nt", C.c_int), ("stack_frac", C.c_double), ("n_hb", C.c_int),
                 ("radius", C.c_double), ("build", BuildArgs)]
@@ -114,6 +119,8 @@ def load_library(path=LIB_PATH):
     lib = C.CDLL(path)
     lib.tis_last_error.restype = C.c_char_p
     lib.tis_build_16scd.argtypes = [C.c_char_p, C.POINTER(BuildArgs), C.POINTER(C.POINTER(TisSystem)),
+                                    C.POINTER(C.c_longlong)]
+    lib.tis_build_chain.argtypes = [C.POINTER(ChainFiles), C.POINTER(BuildArgs), C.POINTER(C.POINTER(TisSystem)),
                                     C.POINTER(C.c_longlong)]
     lib.tis_build_synthetic.argtypes = [C.c_char_p, C.POINTER(SyntheticArgs), C.POINTER(C.POINTER(TisSystem))]
     lib.tis_system_free.argtypes = [C.POINTER(TisSystem)]
@@ -211,6 +218,23 @@ def build_16scd(lib, T=0.615, c_mg=0.005, c_k=0.050, seed=1, box=350.0,
     draws = C.c_longlong(0)
     check(lib, lib.tis_build_16scd(data_dir.encode(), C.byref(args), C.byref(out), C.byref(draws)),
           "tis_build_16scd")
+    try:
+        d = system_to_dict(out)
+    finally:
+        lib.tis_system_free(out)
+    return d, draws.value
+
+
+def build_chain(lib, pdb, bonds, stacks, maxikey, double_face_res=(), recentre_bead=500, T=0.615, c_mg=0.005,
+                c_k=0.050, seed=1, box=350.0, ss_D=0.850, hs_D=3.975, st_D=6.5):
+    """Runs tis_build_chain on files in the reference's formats; returns (dict, rand draws used)."""
+    faces = np.ascontiguousarray(double_face_res, dtype=np.int32)
+    files = ChainFiles(pdb.encode(), bonds.encode(), stacks.encode(), maxikey.encode(),
+                       faces.ctypes.data_as(_ip) if len(faces) else None, len(faces), recentre_bead)
+    args = BuildArgs(ss_D, hs_D, st_D, T, c_mg, c_k, seed, box)
+    out = C.POINTER(TisSystem)()
+    draws = C.c_longlong(0)
+    check(lib, lib.tis_build_chain(C.byref(files), C.byref(args), C.byref(out), C.byref(draws)), "tis_build_chain")
     try:
         d = system_to_dict(out)
     finally:

@@ -127,6 +127,31 @@ typedef struct tis_build_args {
 int tis_build_16scd(const char *data_dir, const tis_build_args *args,
                     tis_system **out, long long *rand_draws_out);
 
+/* Input files of any chain in the reference's formats. */
+typedef struct tis_chain_files {
+    const char *pdb;            /* atoms as read_PDB reads them (tokens 1-9 of ATOM lines) */
+    const char *bonds;          /* HB candidate lines, <name>_<name>_unprocessed_bonds.dat */
+    const char *stacks;         /* tertiary stack pairs, <name>_<name>_unprocessed_stacks.dat */
+    const char *maxikey;        /* MAXIKEY_Mg_Cl_K.dat */
+    const int *double_face_res; /* PDB residue numbers whose 5' stacking face has ST_EXCESS
+                                   lowered by 2 (may be NULL) */
+    int n_double_face;
+    int recentre_bead;          /* 1-based bead kept at the box centre; <= 0 means 500 */
+} tis_chain_files;
+
+/*
+ * Builds any chain from files in the reference's formats.  The reference's
+ * set-up is 16SCD-only: residue numbers map to nucleotides through hard-coded
+ * offsets (fast.cpp:3783-3795, 4295-4305) and 15 stack faces are hard-coded
+ * exceptions (fast.cpp:4458-4491).  Here residue numbers map to nucleotides in
+ * the order the PDB lists them (a number used twice, or a file naming a residue
+ * the PDB lacks, is TIS_EIO) and the exceptions are an argument.  Everything
+ * else, including the rand() stream, is tis_build_16scd's; that function is
+ * this one on the 16SCD files with the reference's 15 residues and bead 500.
+ */
+int tis_build_chain(const tis_chain_files *files, const tis_build_args *args, tis_system **out,
+                    long long *rand_draws_out);
+
 /* A seeded random-sequence chain of any length. */
 typedef struct tis_synthetic_args {
     int n_nt;            /* nucleotides (>= 8); no 5' phosphate, as in 16SCD */
