@@ -13,6 +13,7 @@
 
 #include "../../include/tis_gpu.h"
 #include "backward.h"
+#include "observe.h"
 
 namespace tis_host {
 void set_error(const std::string &msg);
@@ -86,6 +87,9 @@ struct tis_ctx {
     long long steps_done = 0;         // steps of tis_step since creation (exchange cadence)
     long long exchange_attempts = 0;
     bool all_same = false;            // every replica describes the same system
+    // observables: sample cadence in steps of tis_step (0 = only tis_observe_now), contact cut
+    long long obs_every = 0;
+    double obs_cut = 7.2684;
     // force field as uploaded (for tis_set_force_field) and force-matching buffers
     std::vector<double> h_hbE, h_st3E;
     std::vector<double2> h_st2rE;
@@ -232,6 +236,31 @@ struct tis_ctx {
         k_zero_fmb<<<grid(D.n_tot), kBlock, 0, s_main>>>(D);
         HIP_TRY(hipGetLastError());
         ++exchange_attempts;
+    }
+
+    // One sample of the observables of every replica at its current
+    // configuration: a pending list rebuild is done now (the next step would
+    // do the same one; its trigger is cleared), then every term's energy is
+    // evaluated with the HB valence resolution (no stream advances, pair forces
+    // untouched, multi-body forces cleared), then contacts, shape, energies and
+    // HB / stack outcomes are added to the accumulators.
+    void launch_observe() {
+        launch_rebuild();
+        launch_energies(true);
+        k_clear_rebuild<<<grid(n_rep), kBlock, 0, s_main>>>(D);
+        k_zero_fmb<<<grid(D.n_tot), kBlock, 0, s_main>>>(D);
+        k_obs_contacts<<<grid(D.n_tot), kBlock, 0, s_main>>>(D, obs_cut * obs_cut);
+        k_obs_reduce<<<n_rep, kBlock, 0, s_main>>>(D);
+        HIP_TRY(hipGetLastError());
+    }
+    void reset_observables() {
+        int nhb = 0, nst3 = 0;
+        for (const RepParams &p : rp) { nhb += p.nhb; nst3 += p.nst3; }
+        HIP_TRY(hipMemsetAsync(D.obs_rep, 0, (size_t)kObsRep * n_rep * sizeof(double), s_main));
+        HIP_TRY(hipMemsetAsync(D.obs_p, 0, (size_t)D.n_tot * sizeof(double4), s_main));
+        HIP_TRY(hipMemsetAsync(D.obs_hb, 0, std::max(nhb, 1) * sizeof(double), s_main));
+        HIP_TRY(hipMemsetAsync(D.obs_st3, 0, std::max(nst3, 1) * sizeof(double), s_main));
+        HIP_TRY(hipMemsetAsync(D.obs_cnt, 0, 4 * (size_t)n_rep * sizeof(int), s_main));
     }
 
     // One Langevin step.  The HB resolution and the multi-body terms run on
@@ -655,6 +684,12 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             D.remd_stats = dalloc<long long>(own, 2 * (size_t)n_rep);
             HIP_TRY(hipMemset(D.remd_stats, 0, 2 * n_rep * sizeof(long long)));
         }
+        // ---- observables (zeroed by dalloc)
+        D.obs_rep = dalloc<double>(own, (size_t)kObsRep * n_rep);
+        D.obs_p = dalloc<double4>(own, N);
+        D.obs_hb = dalloc<double>(own, std::max<size_t>(1, h0));
+        D.obs_st3 = dalloc<double>(own, std::max<size_t>(1, s30));
+        D.obs_cnt = dalloc<int>(own, 4 * (size_t)n_rep);
         // ---- force matching: tangents, target, outputs
         D.jv = dalloc<double4>(own, N);
         D.jf = dalloc<double4>(own, N);
@@ -830,14 +865,17 @@ static void run_steps(tis_ctx *c, long long n_steps) {
 extern "C" int tis_step(tis_ctx *c, long long n_steps) {
     if (!c || n_steps < 0) return fail(Fail{TIS_EINVAL, "tis_step: bad arguments"});
     return guarded([&]() -> int {
-        const long long ex = c->opt.exchange_every;
+        const long long ex = c->opt.exchange_every, ob = c->obs_every;
         long long left = n_steps;
         while (left > 0) {
-            const long long chunk = ex > 0 ? std::min(left, ex - c->steps_done % ex) : left;
+            long long chunk = left;
+            if (ex > 0) chunk = std::min(chunk, ex - c->steps_done % ex);
+            if (ob > 0) chunk = std::min(chunk, ob - c->steps_done % ob);
             run_steps(c, chunk);
             c->steps_done += chunk;
             left -= chunk;
             if (ex > 0 && c->steps_done % ex == 0) c->launch_exchange();
+            if (ob > 0 && c->steps_done % ob == 0) c->launch_observe();
         }
         HIP_TRY(hipGetLastError());
         if (n_steps > 0) c->finish_pending_rebuild();
@@ -980,6 +1018,70 @@ extern "C" int tis_get_walkers(tis_ctx *c, int *walker, int n_rep) {
     return guarded([&]() -> int {
         HIP_TRY(hipStreamSynchronize(c->s_main));
         HIP_TRY(hipMemcpy(walker, c->D.walker, n_rep * sizeof(int), hipMemcpyDeviceToHost));
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_observe(tis_ctx *c, long long every, double cut) {
+    if (!c || every < 0 || !(cut > 0)) return fail(Fail{TIS_EINVAL, "tis_observe: bad arguments"});
+    const double lim = c->opt.r_sr - c->opt.skin - 1.0;
+    if (cut > lim)
+        return fail(Fail{TIS_EINVAL, "tis_observe: contact cut above r_sr - skin - 1 = " + std::to_string(lim) +
+                                     " A (pairs beyond it may be missing from the short-range lists)"});
+    return guarded([&]() -> int {
+        c->obs_every = every;
+        c->obs_cut = cut;
+        c->reset_observables();
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_observe_now(tis_ctx *c) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_observe_now: bad arguments"});
+    return guarded([&]() -> int {
+        c->launch_observe();
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        c->check_flags();
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_observables(tis_ctx *c, int rep, tis_observables *o, double *per_bead, int n_beads,
+                                   double *hb_kept, int n_hb, double *st3_mean, int n_st3) {
+    if (!c || rep < 0 || rep >= c->n_rep || !o) return fail(Fail{TIS_EINVAL, "tis_get_observables: bad arguments"});
+    const RepParams &P = c->rp[rep];
+    if ((per_bead && n_beads != P.nbeads) || (hb_kept && n_hb != P.nhb) || (st3_mean && n_st3 != P.nst3))
+        return fail(Fail{TIS_EINVAL, "tis_get_observables: array sizes differ from the replica's"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        double a[kObsRep];
+        HIP_TRY(hipMemcpy(a, c->D.obs_rep + (size_t)kObsRep * rep, sizeof a, hipMemcpyDeviceToHost));
+        const double ns = a[0], w = ns > 0 ? 1.0 / ns : 0.0;
+        std::memset(o, 0, sizeof *o);
+        o->n_samples = (long long)ns;
+        o->cut = c->obs_cut;
+        o->rg = a[1] * w; o->rg2 = a[2] * w; o->asphericity = a[3] * w; o->shape = a[4] * w;
+        o->e2_hb = a[5] * w; o->e3_hb = a[6] * w; o->e2_stack = a[7] * w; o->e3_stack = a[8] * w;
+        o->e_wca = a[9] * w; o->e_coul_sr = a[10] * w; o->e_coul_lr = a[11] * w;
+        o->e_bond = a[12] * w; o->e_angle = a[13] * w;
+        o->bound_mg = a[14] * w; o->bound_cl = a[15] * w; o->bound_k = a[16] * w;
+        if (per_bead) {
+            std::vector<double4> v(P.nbeads);
+            HIP_TRY(hipMemcpy(v.data(), c->D.obs_p + P.p0, P.nbeads * sizeof(double4), hipMemcpyDeviceToHost));
+            for (int k = 0; k < P.nbeads; ++k) {
+                per_bead[4 * k + 0] = v[k].x * w; per_bead[4 * k + 1] = v[k].y * w;
+                per_bead[4 * k + 2] = v[k].z * w; per_bead[4 * k + 3] = v[k].w * w;
+            }
+        }
+        if (hb_kept && P.nhb > 0) {
+            HIP_TRY(hipMemcpy(hb_kept, c->D.obs_hb + P.hb0, P.nhb * sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = 0; k < P.nhb; ++k) hb_kept[k] *= w;
+        }
+        if (st3_mean && P.nst3 > 0) {
+            HIP_TRY(hipMemcpy(st3_mean, c->D.obs_st3 + P.st30, P.nst3 * sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = 0; k < P.nst3; ++k) st3_mean[k] *= w;
+        }
         return TIS_OK;
     });
 }

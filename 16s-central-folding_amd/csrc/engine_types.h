@@ -130,6 +130,11 @@ struct Dev {
     int *remd_ok;                 // per neighbour pair: exchangeable
     int *remd_state;              // per neighbour pair: 0 idle, 1 swapped, 2 to restore
     int *walker;                  // trajectory id held by each replica
+    // equilibrium observables (accumulated per sample; layout in observe.h)
+    double *obs_rep;
+    double4 *obs_p;
+    double *obs_hb, *obs_st3;
+    int *obs_cnt;
 };
 
 }  // namespace tis

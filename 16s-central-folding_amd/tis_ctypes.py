@@ -76,6 +76,27 @@ class Energies(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Observables(C.Structure):
+    _fields_ = [("n_samples", C.c_longlong), ("cut", C.c_double), ("rg", C.c_double), ("rg2", C.c_double),
+                ("asphericity", C.c_double), ("shape", C.c_double), ("e2_hb", C.c_double),
+                ("e3_hb", C.c_double), ("e2_stack", C.c_double), ("e3_stack", C.c_double),
+                ("e_wca", C.c_double), ("e_coul_sr", C.c_double), ("e_coul_lr", C.c_double),
+                ("e_bond", C.c_double), ("e_angle", C.c_double), ("bound_mg", C.c_double),
+                ("bound_cl", C.c_double), ("bound_k", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+R_KT = 7.2684   # P-ion contact distance of the reference's analysis (dcd_traj_SaltCount_lB.py:9)
+
+
+def contact_concentration(mean_count, cut=R_KT):
+    """Mean ion count within `cut` of a phosphate as a local concentration in
+    mol/L (analysis/salt_nt/integrate_num_ions.py:51-55)."""
+    return np.asarray(mean_count) / (4.0 / 3.0 * np.pi * cut ** 3 * 6.02214 / 1e4)
+
+
 PREC_FP64, PREC_MIXED = 0, 1
 RNG_PHILOX, RNG_STREAM = 0, 1
 TERM_STACK, TERM_HB, TERM_BACKBONE, TERM_SR, TERM_LR, TERM_ALL = 1, 2, 4, 8, 16, 31
@@ -155,6 +176,10 @@ def load_library(path=LIB_PATH):
         lib.tis_get_walkers.argtypes = [C.c_void_p, _ip, C.c_int]
         lib.tis_force_match.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.tis_set_force_field.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        lib.tis_observe.argtypes = [C.c_void_p, C.c_longlong, C.c_double]
+        lib.tis_observe_now.argtypes = [C.c_void_p]
+        lib.tis_get_observables.argtypes = [C.c_void_p, C.c_int, C.POINTER(Observables), _dp, C.c_int, _dp,
+                                            C.c_int, _dp, C.c_int]
     return lib
 
 
@@ -411,6 +436,29 @@ class Engine:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (hb_E, st3_E, st2_E, ion_R)]
         ptrs = [None if a is None else a.ctypes.data_as(_dp) for a in arrs]
         check(self.lib, self.lib.tis_set_force_field(self.ctx, *ptrs), "tis_set_force_field")
+
+    def observe(self, every, cut=R_KT):
+        """Clear the observable accumulators; sample every `every` steps (0: only observe_now)."""
+        check(self.lib, self.lib.tis_observe(self.ctx, every, cut), "tis_observe")
+
+    def observe_now(self):
+        check(self.lib, self.lib.tis_observe_now(self.ctx), "tis_observe_now")
+
+    def observables(self, rep=0):
+        """Means over the samples: a dict of the scalars plus per_bead (n_beads, 4):
+        Mg, Cl, K counts within the cut of each P bead and P(Mg count >= 1);
+        hb_kept (n_hb); st3_mean (n_st3)."""
+        d = self.handles[rep].d
+        o = Observables()
+        pb = np.zeros(4 * d["n_beads"])
+        hb = np.zeros(max(1, d["n_hb"]))
+        st = np.zeros(max(1, d["n_st3"]))
+        check(self.lib, self.lib.tis_get_observables(
+            self.ctx, rep, C.byref(o), pb.ctypes.data_as(_dp), d["n_beads"], hb.ctypes.data_as(_dp), d["n_hb"],
+            st.ctypes.data_as(_dp), d["n_st3"]), "tis_get_observables")
+        out = o.as_dict()
+        out.update(per_bead=pb.reshape(-1, 4), hb_kept=hb[:d["n_hb"]], st3_mean=st[:d["n_st3"]])
+        return out
 
     def set_profiling(self, on):
         check(self.lib, self.lib.tis_set_profiling(self.ctx, int(on)), "tis_set_profiling")

@@ -298,6 +298,47 @@ int tis_get_energies(tis_ctx *ctx, int rep, tis_energies *e);
 /* Per-stack energies st_energy[1..s3_N] (the Contacts file, fast.cpp:6290). */
 int tis_get_stack_energies(tis_ctx *ctx, int rep, double *out, int n);
 
+/* Equilibrium observables, accumulated on the device.  The reference writes
+ * frames (Coordinates / Energies / Contacts, fast.cpp:6276-6316) and computes
+ * these offline from them; here every sample adds them in place:
+ *   - per phosphate bead, the Mg, Cl and K ions within `cut` (minimum image;
+ *     analysis/mg_correlation/mg_bound.F90:97-126) and whether at least one Mg
+ *     is bound (count_mg_bound.py:8-15);
+ *   - ions of each kind within `cut` of any phosphate (count_bound_total,
+ *     analysis/salt_nt/dcd_traj_SaltCount_lB.py:52-62, where cut = r_kT = 7.2684 A);
+ *   - chain shape from the gyration tensor of the beads: Rg, asphericity D and
+ *     shape parameter S (analysis/shape/dcd_traj_shape.F90:193-226; Rg as
+ *     radius_of_gyration, fast.cpp:2844-2876);
+ *   - every energy term, and per HB candidate whether the valence resolution
+ *     kept it, per tertiary stack its energy (the Contacts columns).
+ * A sample evaluates every term at the current configuration (a pending list
+ * rebuild is done first, as the next step would do; the trajectory does not
+ * change). */
+typedef struct tis_observables {
+    long long n_samples;
+    double cut;                                /* P-ion contact distance (A) */
+    double rg, rg2, asphericity, shape;        /* means of Rg, Rg^2, D, S */
+    double e2_hb, e3_hb, e2_stack, e3_stack;   /* means of the energy terms */
+    double e_wca, e_coul_sr, e_coul_lr, e_bond, e_angle;
+    double bound_mg, bound_cl, bound_k;        /* mean ions within cut of any P */
+} tis_observables;
+/* Clears the accumulators and sets the cadence: one sample every `every` steps
+ * of tis_step (counted like exchange_every; 0 = samples only on tis_observe_now)
+ * with P-ion contact distance `cut` (A; at most r_sr - skin - 1, so that every
+ * pair within it is in the short-range lists). */
+int tis_observe(tis_ctx *ctx, long long every, double cut);
+/* One sample of every replica now. */
+int tis_observe_now(tis_ctx *ctx);
+/* Means over the samples of replica rep.  Optional arrays (NULL to skip):
+ *   per_bead  4 * n_beads: for each phosphate bead the mean Mg, Cl, K counts
+ *             within cut and the fraction of samples with >= 1 Mg (0 elsewhere);
+ *             mean count / (4/3 pi cut^3 * 6.02214e-4) is the local
+ *             concentration in mol/L (analysis/salt_nt/integrate_num_ions.py:51-55)
+ *   hb_kept   n_hb: fraction of samples in which each candidate was kept
+ *   st3_mean  n_st3: mean energy of each tertiary stack */
+int tis_get_observables(tis_ctx *ctx, int rep, tis_observables *o, double *per_bead, int n_beads,
+                        double *hb_kept, int n_hb, double *st3_mean, int n_st3);
+
 /* Parity hook: forces of the selected terms at the current positions, without
  * moving anything.  HB uses the replica's uniform source (stream or Philox). */
 enum { TIS_TERM_STACK = 1, TIS_TERM_HB = 2, TIS_TERM_BACKBONE = 4,
