@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -110,6 +111,11 @@ struct tis_ctx {
     }
 
     int grid(int n) const { return (n + kBlock - 1) / kBlock; }
+    // launch shape of the pair gather and the list build (tunable, results unchanged
+    // up to the summation order of a particle's pair forces)
+    int pair_block = kBlock, list_waves = kListWaves;
+    int pair_grid() const { return (int)(((long long)D.pair_lanes * D.n_tot + pair_block - 1) / pair_block); }
+    int list_grid() const { return list_waves * 64 / kBlock; }
 
     hipEvent_t next_event() {
         if (event_next == event_pool.size()) {
@@ -148,7 +154,7 @@ struct tis_ctx {
     // the resolution's site tables on `sh` (they depend on positions only).
     void launch_rebuild_lists(hipStream_t sl) {
         timed("k_bin", sl, [&] { k_bin<<<n_rep, 1024, 0, sl>>>(D); });
-        timed("k_list", sl, [&] { k_list<<<kListWaves * 64 / kBlock, kBlock, 0, sl>>>(D, 1); });
+        timed("k_list", sl, [&] { k_list<<<list_grid(), kBlock, 0, sl>>>(D, 1); });
     }
     void launch_rebuild_hb(hipStream_t sh) {
         timed("k_hbstatus", sh, [&] { k_hbstatus<<<dim3(max_hb_chunks(), n_rep), kStatusBlock, 0, sh>>>(D); });
@@ -204,14 +210,14 @@ struct tis_ctx {
     void launch_energies(bool with_hb) {
         k_zero_energies<<<grid(n_rep), kBlock, 0, s_main>>>(D);
         if (with_hb) launch_hb(s_main, 0);
-        const int nmb = n_rep + nb_bonded, npr = grid(kPairLanes * D.n_tot);
+        const int nmb = n_rep + nb_bonded, npr = pair_grid();
         const unsigned mb = TIS_TERM_STACK | TIS_TERM_BACKBONE, pr = TIS_TERM_SR | TIS_TERM_LR;
         if (D.precision == TIS_PREC_FP64) {
             k_force<double, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, kEvalFull);
-            k_force<double, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
+            k_force<double, 1><<<npr, pair_block, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
         } else {
             k_force<float, 0><<<nmb, kBlock, 0, s_main>>>(D, nb_bonded, mb, kEvalFull);
-            k_force<float, 1><<<npr, kBlock, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
+            k_force<float, 1><<<npr, pair_block, 0, s_main>>>(D, nb_bonded, pr, kEvalEnergy);
         }
     }
     // One replica-exchange attempt, all on the device (the sequence documented
@@ -279,8 +285,8 @@ struct tis_ctx {
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
         launch_rebuild_lists(s_main);
         timed("k_pairs", s_main, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
-            else k_force<float, 1><<<grid(kPairLanes * D.n_tot), kBlock, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
+            if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<pair_grid(), pair_block, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
+            else k_force<float, 1><<<pair_grid(), pair_block, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
@@ -393,6 +399,17 @@ void validate(const tis_system &s, int r) {
 
 }  // namespace
 
+// Launch-shape options: lanes per particle a power of two up to 64, pair block a
+// multiple of 64 up to the kernels' launch bound (and of the lanes), list grid
+// 64 .. 65536 wavefronts.
+static void check_launch_options(const tis_options &o) {
+    const int l = o.pair_lanes, b = o.pair_block, w = o.list_waves;
+    if (l < 1 || l > 64 || (l & (l - 1))) throw Fail{TIS_EINVAL, "pair_lanes must be a power of two, 1..64"};
+    if (b < 64 || b > kBlock || b % 64) throw Fail{TIS_EINVAL, "pair_block must be a multiple of 64, at most 256"};
+    if (w < 64 || w > 65536 || (w * 64) % kBlock) throw Fail{TIS_EINVAL, "list_waves must be 64..65536, a multiple of 4"};
+    if (o.graph_steps < 0) throw Fail{TIS_EINVAL, "graph_steps"};
+}
+
 extern "C" void tis_default_options(tis_options *o) {
     if (!o) return;
     o->precision = TIS_PREC_FP64;
@@ -406,6 +423,9 @@ extern "C" void tis_default_options(tis_options *o) {
     o->use_graph = 1;
     o->graph_steps = 50;
     o->exchange_every = 0;
+    o->pair_lanes = kPairLanes;
+    o->pair_block = kBlock;
+    o->list_waves = kListWaves;
 }
 
 static int fail(const Fail &f) {
@@ -446,6 +466,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         if (o.rng != TIS_RNG_PHILOX && o.rng != TIS_RNG_STREAM) throw Fail{TIS_EINVAL, "rng"};
         if (!(o.r_sr > 0) || !(o.r_lr >= o.r_sr) || !(o.skin >= 0) || o.lr_every <= 0) throw Fail{TIS_EINVAL, "list settings"};
         if (o.exchange_every < 0) throw Fail{TIS_EINVAL, "exchange_every"};
+        check_launch_options(o);
         for (int r = 0; r < n_rep; ++r) validate(reps[r], r);
         int dev_count = 0;
         HIP_TRY(hipGetDeviceCount(&dev_count));
@@ -489,6 +510,9 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         Dev &D = c->D;
         D.n_rep = n_rep; D.n_tot = N; D.precision = o.precision; D.rng = o.rng; D.seed = o.seed;
         D.noise_scale = o.noise_scale;
+        D.pair_lanes = o.pair_lanes;
+        c->pair_block = o.pair_block;
+        c->list_waves = o.list_waves;
         auto &own = c->owned;
         D.rp = dalloc<RepParams>(own, n_rep);
         D.rs = dalloc<RepState>(own, n_rep);
@@ -716,7 +740,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         upload(D.rs, rs);
         HIP_TRY(hipMemcpy(D.rp, c->rp.data(), n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
         k_bin<<<n_rep, 1024, 0, c->s_main>>>(D);
-        k_list<<<kListWaves * 64 / kBlock, kBlock, 0, c->s_main>>>(D, 0);
+        k_list<<<c->list_grid(), kBlock, 0, c->s_main>>>(D, 0);
         HIP_TRY(hipStreamSynchronize(c->s_main));
         // fixed per-particle slots: twice the largest count at set-up plus margin
         // (compaction under added salt raises the LR counts of the RNA core)
@@ -885,6 +909,74 @@ extern "C" int tis_step(tis_ctx *c, long long n_steps) {
     });
 }
 
+// ---- autotuner: launch shapes timed on a scratch context
+static void apply_launch(tis_ctx *c, const tis_options &o) {
+    check_launch_options(o);
+    c->opt.pair_lanes = o.pair_lanes;
+    c->opt.pair_block = o.pair_block;
+    c->opt.list_waves = o.list_waves;
+    c->opt.graph_steps = o.graph_steps;
+    c->D.pair_lanes = o.pair_lanes;
+    c->pair_block = o.pair_block;
+    c->list_waves = o.list_waves;
+    if (c->graph) {
+        HIP_TRY(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+}
+
+extern "C" int tis_autotune(const tis_system *reps, int n_rep, tis_options *opt, int trial_steps, tis_tuning *report) {
+    if (!reps || n_rep <= 0 || !opt || trial_steps <= 0) return fail(Fail{TIS_EINVAL, "tis_autotune: bad arguments"});
+    tis_ctx *c = nullptr;
+    const int rc = tis_create(reps, n_rep, opt, &c);
+    if (rc != TIS_OK) return rc;
+    const int out = guarded([&]() -> int {
+        int trials = 0;
+        auto measure = [&](const tis_options &t) {
+            apply_launch(c, t);
+            if (tis_step(c, trial_steps / 4 + 1) != TIS_OK) throw Fail{TIS_EDEVICE, tis_last_error()};
+            double best = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                const auto t0 = std::chrono::steady_clock::now();
+                if (tis_step(c, trial_steps) != TIS_OK) throw Fail{TIS_EDEVICE, tis_last_error()};
+                const std::chrono::duration<double, std::milli> dt = std::chrono::steady_clock::now() - t0;
+                best = std::min(best, dt.count() / trial_steps);
+            }
+            ++trials;
+            return best;
+        };
+        tis_options best = *opt;
+        const double start = measure(best);
+        double best_ms = start;
+        // coordinate descent; a candidate must win by 1 % to replace the incumbent
+        auto sweep = [&](int tis_options::*field, std::initializer_list<int> cands) {
+            for (int v : cands) {
+                if (v == best.*field) continue;
+                tis_options t = best;
+                t.*field = v;
+                const double ms = measure(t);
+                if (ms < 0.99 * best_ms) { best = t; best_ms = ms; }
+            }
+        };
+        if (best.use_graph) sweep(&tis_options::graph_steps, {25, 50, 100, 200});
+        sweep(&tis_options::pair_lanes, {2, 4, 8, 16});
+        sweep(&tis_options::pair_block, {64, 128, 256});
+        sweep(&tis_options::list_waves, {1024, 2048, 4096, 8192, 16384});
+        opt->graph_steps = best.graph_steps;
+        opt->pair_lanes = best.pair_lanes;
+        opt->pair_block = best.pair_block;
+        opt->list_waves = best.list_waves;
+        if (report) {
+            report->ms_per_step_start = start;
+            report->ms_per_step_best = best_ms;
+            report->trials = trials;
+        }
+        return TIS_OK;
+    });
+    tis_destroy(c);
+    return out;
+}
+
 extern "C" int tis_set_force_field(tis_ctx *c, const double *hb_E, const double *st3_E, const double *st2_E,
                                    const double *ion_R) {
     if (!c) return fail(Fail{TIS_EINVAL, "tis_set_force_field: bad arguments"});
@@ -931,13 +1023,13 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
         // forward: the step's forces with a fresh long-range term
         k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
         c->launch_hb(st, 0);
-        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * N);
+        const int nmb = c->n_rep + c->nb_bonded, npr = c->pair_grid();
         if (D.precision == TIS_PREC_FP64) {
             k_force<double, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-            k_force<double, 1><<<npr, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+            k_force<double, 1><<<npr, c->pair_block, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
         } else {
             k_force<float, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-            k_force<float, 1><<<npr, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
+            k_force<float, 1><<<npr, c->pair_block, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
         }
         k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_target, c->fm_forces, c->fm_loss);
         // backward: force tangents along v = F - target, parameter sums
@@ -1155,13 +1247,13 @@ extern "C" int tis_eval_forces(tis_ctx *c, int rep, unsigned mask, double *f_out
             c->launch_hb(c->s_main, 0);
         }
         const unsigned m = mask & (TIS_TERM_STACK | TIS_TERM_BACKBONE | TIS_TERM_SR | TIS_TERM_LR);
-        const int nmb = c->n_rep + c->nb_bonded, npr = c->grid(kPairLanes * D.n_tot);
+        const int nmb = c->n_rep + c->nb_bonded, npr = c->pair_grid();
         if (D.precision == TIS_PREC_FP64) {
             k_force<double, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
-            k_force<double, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
+            k_force<double, 1><<<npr, c->pair_block, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
         } else {
             k_force<float, 0><<<nmb, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
-            k_force<float, 1><<<npr, kBlock, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
+            k_force<float, 1><<<npr, c->pair_block, 0, c->s_main>>>(D, c->nb_bonded, m, kEvalFull);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(c->s_main));

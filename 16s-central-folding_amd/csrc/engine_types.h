@@ -69,6 +69,7 @@ struct Dev {
     int rng;
     uint64_t seed;
     double noise_scale;
+    int pair_lanes;               // lanes per particle in the pair gather
     RepParams *rp;
     RepState *rs;
     // particles (global index)

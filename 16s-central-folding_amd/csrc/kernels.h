@@ -1188,7 +1188,7 @@ TIS_DEV bool bonded_term(const Dev &D, double4 *F, int g, int nbond, int nang, i
 }
 
 // ------------------------------------------------------------ k_force
-constexpr int kPairLanes = 4;   // lanes per particle in the pair gather
+constexpr int kPairLanes = 4;   // default lanes per particle in the pair gather (D.pair_lanes)
 // ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
 // backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
 // that the register-heavy stacking code does not lower the occupancy of the
@@ -1277,10 +1277,12 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         return;
     }
-    // ---- pair gather: kPairLanes lanes per particle (cell order), each summing a
-    // strided share of the neighbours, then a butterfly reduction inside the group
-    const int t = (blk * blockDim.x + threadIdx.x) / kPairLanes;
-    const int sub = threadIdx.x % kPairLanes;
+    // ---- pair gather: D.pair_lanes lanes per particle (cell order; a power of
+    // two up to 64), each summing a strided share of the neighbours, then a
+    // butterfly reduction inside the group
+    const int lanes = D.pair_lanes;
+    const int t = (blk * blockDim.x + threadIdx.x) / lanes;
+    const int sub = threadIdx.x % lanes;
     const bool valid = t < D.n_tot;
     const int p = valid ? D.sorted[t] : D.sorted[0];
     const int r = D.prep[p];
@@ -1292,15 +1294,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     const bool chg_i = (pti >> 8) & 1;
     const int *nb = D.pool + P.pool0 + D.nb_start[p];
     const int nsr = valid ? D.nb_nsr[p] : 0, nlr = valid ? D.nb_nlr[p] : 0;
-    auto group_sum = [](double v) {
-#pragma unroll
-        for (int o = 1; o < kPairLanes; o <<= 1) v += __shfl_xor(v, o, 64);
+    auto group_sum = [lanes](double v) {
+        for (int o = 1; o < lanes; o <<= 1) v += __shfl_xor(v, o, 64);
         return v;
     };
     if (mask & 8) {
         R fx = 0, fy = 0, fz = 0;
         double ew = 0, ec = 0;
-        for (int q = sub; q < nsr; q += kPairLanes) {
+        for (int q = sub; q < nsr; q += lanes) {
             const int e = nb[q];
             const bool bonded = e < 0;
             const int j = bonded ? ~e : e;
@@ -1342,7 +1343,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         double ec = 0;
         const int n_lr = lr ? nlr : 0;
         const int *nb_lr = nb + P.cap_sr;
-        for (int q = sub; q < n_lr; q += kPairLanes) {
+        for (int q = sub; q < n_lr; q += lanes) {
             const int j = nb_lr[q];
             double3 dd;
             const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);

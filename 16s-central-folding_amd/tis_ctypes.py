@@ -61,7 +61,12 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
                 ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
                 ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
-                ("graph_steps", C.c_int), ("exchange_every", C.c_longlong)]
+                ("graph_steps", C.c_int), ("exchange_every", C.c_longlong), ("pair_lanes", C.c_int),
+                ("pair_block", C.c_int), ("list_waves", C.c_int)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("ms_per_step_start", C.c_double), ("ms_per_step_best", C.c_double), ("trials", C.c_int)]
 
 
 class Energies(C.Structure):
@@ -176,6 +181,7 @@ def load_library(path=LIB_PATH):
         lib.tis_get_walkers.argtypes = [C.c_void_p, _ip, C.c_int]
         lib.tis_force_match.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.tis_set_force_field.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        lib.tis_autotune.argtypes = [C.POINTER(TisSystem), C.c_int, C.POINTER(Options), C.c_int, C.POINTER(Tuning)]
         lib.tis_observe.argtypes = [C.c_void_p, C.c_longlong, C.c_double]
         lib.tis_observe_now.argtypes = [C.c_void_p]
         lib.tis_get_observables.argtypes = [C.c_void_p, C.c_int, C.POINTER(Observables), _dp, C.c_int, _dp,
@@ -290,6 +296,22 @@ def default_options(lib):
     o = Options()
     lib.tis_default_options(C.byref(o))
     return o
+
+
+LAUNCH_FIELDS = ("graph_steps", "pair_lanes", "pair_block", "list_waves")
+
+
+def autotune(lib, handles, trial_steps=400, **options):
+    """tis_autotune over the replicas of `handles` with the given options
+    (precision, rng, seed, ...); returns ({launch field: value}, Tuning dict)."""
+    arr = (TisSystem * len(handles))(*[h.struct for h in handles])
+    o = default_options(lib)
+    for k, v in options.items():
+        setattr(o, k, v)
+    rep = Tuning()
+    check(lib, lib.tis_autotune(arr, len(handles), C.byref(o), trial_steps, C.byref(rep)), "tis_autotune")
+    return ({k: getattr(o, k) for k in LAUNCH_FIELDS},
+            {name: getattr(rep, name) for name, _ in Tuning._fields_})
 
 
 class Engine:

@@ -4,8 +4,10 @@ Workload (the benchmark's batched configuration): 16 replicas of the 16SCD
 system, replica k in a box of 350 + 100 k / 15 A with c_Mg = 0.5 * 40^(k/15) mM
 MgCl2 and 50 mM KCl (3,308 .. 9,469 particles, 86,655 in all), T = 0.596
 kcal/mol (300 K), seeds 1..16, FP32 pair and multi-body forces with FP64
-positions, velocities and sums, counter-based Philox noise, one hipGraph per
-50 steps.  The replicas differ in ion count, so no exchange moves are
+positions, velocities and sums, counter-based Philox noise, launch shapes
+(hipGraph length, pair-gather lanes and block, list-build grid) from
+tis_autotune run on a scratch copy before the timed region.  The replicas
+differ in ion count, so no exchange moves are
 attempted between them (an exchange needs two copies of one system).
 
 Printed: one JSON line.  value = aggregate replica-steps per second of the
@@ -17,7 +19,7 @@ kernel from HIP-event timings, and the FP64 CPU oracle timed on this host.
 of a seeded 1,500-nt random-sequence chain (self-avoiding walk in a sphere of
 75 A, secondary stacks on ~60% of adjacent pairs, 400 HB candidates) in a
 600 A box with 10 mM MgCl2 + 100 mM KCl (32,914 particles), same precision,
-noise and graph settings; its line reports steps/s, ns/day, the whole-step
+noise and tuning; its line reports steps/s, ns/day, the whole-step
 HBM rate and, from the committed counter summary, the VALU issue fraction.
 """
 import argparse
@@ -196,8 +198,9 @@ def run_chain(args, lib):
                           c_mg=0.010, c_k=0.100, seed=1, box=600.0)
     log(f"built the synthetic chain: {d['n_particles']} particles")
     h = T.SystemHandle(d)
+    launch, tuning = tune(lib, [h], args)
     eng = T.Engine(lib, [h], precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
-                   use_graph=0 if args.no_graph else 1, graph_steps=50)
+                   use_graph=0 if args.no_graph else 1, **launch)
     eng.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -243,7 +246,7 @@ def run_chain(args, lib):
         "config": {"workload": "1,500-nt synthetic chain, 600 A box, 10 mM MgCl2 + 100 mM KCl, T=0.596, "
                                "stacks on ~60% of adjacent pairs, 400 HB candidates",
                    "replicas": 1, "particles": d["n_particles"], "precision": "mixed", "rng": "philox",
-                   "graph_steps": 50},
+                   "launch": launch, "autotune": tuning},
         "ns_per_day": value * FS_PER_STEP * 86400 * 1e-6,
         "roofline": None if dominant is None else {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -258,6 +261,19 @@ def run_chain(args, lib):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def tune(lib, handles, args):
+    """Launch shapes for the timed run: tis_autotune's choice on a scratch copy of
+    the batch (its trials are outside the timed region), or the defaults."""
+    if args.no_autotune or args.no_graph:
+        o = T.default_options(lib)
+        return {k: getattr(o, k) for k in T.LAUNCH_FIELDS}, None
+    launch, rep = T.autotune(lib, handles, trial_steps=args.tune_steps, precision=T.PREC_MIXED,
+                             rng=T.RNG_PHILOX, seed=2024)
+    log(f"autotune: {launch} ({rep['trials']} settings, {rep['ms_per_step_start']:.4f} -> "
+        f"{rep['ms_per_step_best']:.4f} ms/step)")
+    return launch, rep
 
 
 def log(msg):
@@ -275,6 +291,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200, help="CPU steps per replica for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for counter collection)")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of tis_autotune's")
+    ap.add_argument("--tune-steps", type=int, default=400, help="steps per autotuner trial")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -292,8 +310,9 @@ def main():
     systems = batched_systems(lib)
     log(f"built {len(systems)} replicas")
     handles = [T.SystemHandle(d) for d in systems]
+    launch, tuning = tune(lib, handles, args)
     eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
-                   use_graph=0 if args.no_graph else 1, graph_steps=50)
+                   use_graph=0 if args.no_graph else 1, **launch)
     eng.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -357,7 +376,7 @@ def main():
         "config": {"workload": "16 x 16SCD replicas, c_Mg = 0.5*40^(k/15) mM MgCl2 + 50 mM KCl, "
                                "box 350+100k/15 A, T=0.596, independent (no exchange)",
                    "replicas": n_rep, "particles": int(sum(d["n_particles"] for d in systems)),
-                   "precision": "mixed", "rng": "philox", "graph_steps": 50},
+                   "precision": "mixed", "rng": "philox", "launch": launch, "autotune": tuning},
         "ns_per_day_aggregate": value * FS_PER_STEP * 86400 * 1e-6,
         "plain_fp64_single": None if plain is None else {
             "value": plain, "unit": "steps/s", "particles": d1["n_particles"],

@@ -204,6 +204,12 @@ typedef struct tis_options {
     int use_graph;        /* capture each tis_step chunk in a hipGraph */
     int graph_steps;      /* steps per captured graph */
     long long exchange_every; /* steps between replica-exchange attempts; 0 = none */
+    /* launch shapes (no counterpart in the reference; chosen by tis_autotune).
+     * Results do not depend on them beyond the summation order of each
+     * particle's pair forces. */
+    int pair_lanes;       /* lanes per particle in the pair gather: power of two, 1..64 (4) */
+    int pair_block;       /* workgroup size of the pair gather: multiple of 64, <= 256 (256) */
+    int list_waves;       /* wavefronts of the neighbour-list build grid, 64..65536 (4096) */
 } tis_options;
 
 /* Fills *opt with the reference's settings (FP64, stream RNG off => Philox, skin 3, ...). */
@@ -213,6 +219,25 @@ void tis_default_options(tis_options *opt);
  * fast.cpp:6219-6239 (MPI_Init, list buffers, populate_lists). */
 int tis_create(const tis_system *reps, int n_rep, const tis_options *opt, tis_ctx **out);
 void tis_destroy(tis_ctx *ctx);
+
+/* Autotuner (no counterpart in the reference).  Times tis_step on a scratch
+ * context built from the same replicas and options (the caller's systems are
+ * not modified) and picks, by coordinate descent, the fastest of
+ *   graph_steps  25, 50, 100, 200    (steps per captured hipGraph)
+ *   pair_lanes   2, 4, 8, 16         (lanes per particle in the pair gather)
+ *   pair_block   64, 128, 256        (its workgroup size)
+ *   list_waves   1024 .. 16384       (grid of the neighbour-list build)
+ * writing them into *opt.  Each setting is timed twice over trial_steps steps
+ * (after trial_steps / 4 warm-up steps) and its faster run is kept.  The list
+ * skin, the list radii and the long-range cadence are not tuned: they decide
+ * which pairs the reference treats as short- or long-range at each step, so
+ * they are part of the model, not of its schedule. */
+typedef struct tis_tuning {
+    double ms_per_step_start;  /* the options passed in */
+    double ms_per_step_best;   /* the options written back */
+    int trials;                /* settings timed */
+} tis_tuning;
+int tis_autotune(const tis_system *reps, int n_rep, tis_options *opt, int trial_steps, tis_tuning *report);
 
 /* State in/out (xyz-interleaved, 3*N doubles each; any pointer may be NULL on get).
  * set_state also rebuilds the pair lists (populate_lists, fast.cpp:1546-1737) and
