@@ -393,6 +393,7 @@ void validate(const tis_system &s, int r) {
         chk(s.site_hb_idx, s.n_hb_incid, s.n_hb, "site HB");
     }
     if (s.n_st2 > 4096) bad("more than 4096 secondary stacks");
+    if (s.n_hb_incid >= (1 << 19)) bad("2^19 or more HB site incidences");
     for (int k = 0; k < 2 * s.n_st3; ++k)
         if (s.st3_disable && (s.st3_disable[k] < -1 || s.st3_disable[k] >= s.n_st2)) bad("st3_disable");
 }

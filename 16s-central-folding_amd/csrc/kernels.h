@@ -220,34 +220,40 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *nc
 }
 
 // Visits every particle of the neighbour cells, 64 per call of visit(j, x),
-// lanes past a cell's end getting j = -1.  Two chunks are in flight: the load of
-// the next chunk is issued before the current one is tested.
+// lanes past the last candidate getting j = -1.  The cells' particles are taken
+// as one sequence (cell c covers positions [excl_c, incl_c) of it, prefix sums
+// over the lanes that hold the cells), so every chunk of 64 lanes is full
+// whatever the cell occupancies; a lane finds its cell by a scalar walk over the
+// few cells that overlap its chunk.  Two chunks are in flight: the load of the
+// next chunk is issued before the current one is tested.
 template <class F>
 TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
     const int lane = threadIdx.x & 63;
     int ncell, cstart, clen;
     neighbour_cells(D, P, cell, &ncell, &cstart, &clen);
+    const int incl = wave_incl_scan(clen);
+    const int total = readlane_i(incl, 63);
+    if (total == 0) return;
     const double4 *cp = D.cpos + P.p0;
-    // scalar cursor (cell c, offset o within it); advance skips empty cells
-    int c = 0, o = 0;
-    auto skip_empty = [&]() {
-        while (c < ncell && o >= readlane_i(clen, c)) { ++c; o = 0; }
+    int c = 0;   // first cell whose range ends after the chunk start (uniform)
+    auto load = [&](int t0, double4 *x) {
+        while (readlane_i(incl, c) <= t0) ++c;
+        const int t = t0 + lane;
+        int addr = -1;
+        for (int cc = c; cc < ncell; ++cc) {
+            const int in = readlane_i(incl, cc), ex = in - readlane_i(clen, cc);
+            if (ex >= t0 + 64) break;
+            if (t >= ex && t < in) addr = readlane_i(cstart, cc) + (t - ex);
+        }
+        *x = addr >= 0 ? cp[addr] : make_double4(0, 0, 0, -1.0);
     };
-    auto load = [&](int cc, int oo, double4 *x) {
-        const int s = readlane_i(cstart, cc), n = readlane_i(clen, cc);
-        const bool ok = oo + lane < n;
-        *x = ok ? cp[s + oo + lane] : make_double4(0, 0, 0, -1.0);
-    };
-    skip_empty();
-    if (c >= ncell) return;
     double4 cur;
-    load(c, o, &cur);
-    for (;;) {
-        o += 64;
-        skip_empty();
-        const bool more = c < ncell;
+    load(0, &cur);
+    for (int t0 = 0;;) {
+        t0 += 64;
+        const bool more = t0 < total;
         double4 nxt = make_double4(0, 0, 0, -1.0);
-        if (more) load(c, o, &nxt);
+        if (more) load(t0, &nxt);
         visit((int)cur.w, cur);
         if (!more) break;
         cur = nxt;
@@ -379,13 +385,23 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
     __syncthreads();
     const int n_act = chunk_off[kMaxHbChunks];
     if (n_act <= kActMax) {
-        for (int k = threadIdx.x; k < P.nhb; k += blockDim.x) {
-            const int g = P.hb0 + k;
-            const int lr = D.act_idx[g];
-            if (lr < 0) continue;
-            const int a = chunk_off[k / kStatusBlock] + lr;
-            D.act_idx[g] = a;
-            D.act_hb[P.hb0 + a] = g;
+        // eight candidates per thread per round, their ranks loaded before any store
+        constexpr int U = 8;
+        for (int k0 = threadIdx.x; k0 < P.nhb; k0 += U * blockDim.x) {
+            int lr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * blockDim.x;
+                lr[u] = k < P.nhb ? D.act_idx[P.hb0 + k] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (lr[u] < 0) continue;
+                const int k = k0 + u * blockDim.x, g = P.hb0 + k;
+                const int a = chunk_off[k / kStatusBlock] + lr[u];
+                D.act_idx[g] = a;
+                D.act_hb[P.hb0 + a] = g;
+            }
         }
     }
     __syncthreads();
@@ -399,9 +415,12 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         const int s = s0 + threadIdx.x;
         const int cnt = s < P.nsite ? D.site_cnt[P.site0 + s] : 0;
         if (s < P.nsite) D.site_cnt[P.site0 + s] = 0;   // cleared for the next build
-        int tot_s, tot_i;
-        const int rank = block_excl_sum(cnt > 0, scratch, &tot_s);
-        const int off = block_excl_sum(cnt, scratch, &tot_i);
+        // one scan for both prefixes: active-site flags in the low 12 bits (at most
+        // 1024 per round), incidence counts above them (a replica has fewer than
+        // 2^19 site incidences, checked at creation)
+        int tot;
+        const int packed = block_excl_sum((cnt > 0 ? 1 : 0) | (cnt << 12), scratch, &tot);
+        const int rank = packed & 0xfff, off = packed >> 12, tot_s = tot & 0xfff, tot_i = tot >> 12;
         if (s < P.nsite) {
             fill[s] = 0;
             if (cnt > 0) {
