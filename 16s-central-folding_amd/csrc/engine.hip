@@ -113,7 +113,7 @@ struct tis_ctx {
     int grid(int n) const { return (n + kBlock - 1) / kBlock; }
     // launch shape of the pair gather and the list build (tunable, results unchanged
     // up to the summation order of a particle's pair forces)
-    int pair_block = kBlock, list_waves = kListWaves;
+    int pair_block = kBlock, list_waves = kListWaves, hb_groups = 4;
     int pair_grid() const { return (int)(((long long)D.pair_lanes * D.n_tot + pair_block - 1) / pair_block); }
     int list_grid() const { return list_waves * 64 / kBlock; }
 
@@ -192,11 +192,10 @@ struct tis_ctx {
     // HB terms on stream st: energies and gates, the valence resolution, forces.
     void launch_hb(hipStream_t st, int mode) {
         const dim3 g2((kActMax + kBlock - 1) / kBlock, n_rep);
-        timed("k_hbpre", st, [&] {
-            if (D.precision == TIS_PREC_FP64) k_hbpre<double><<<g2, kBlock, 0, st>>>(D);
-            else k_hbpre<float><<<g2, kBlock, 0, st>>>(D);
+        timed("k_hbres", st, [&] {
+            if (D.precision == TIS_PREC_FP64) k_hbres<double><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
+            else k_hbres<float><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
         });
-        timed("k_hbres", st, [&] { k_hbres<<<n_rep, kResBlock, kHbLds, st>>>(D, mode); });
         timed("k_hbforce", st, [&] {
             if (D.precision == TIS_PREC_FP64) k_hbforce<double><<<g2, kBlock, 0, st>>>(D);
             else k_hbforce<float><<<g2, kBlock, 0, st>>>(D);
@@ -402,13 +401,14 @@ void validate(const tis_system &s, int r) {
 
 // Launch-shape options: lanes per particle a power of two up to 64, pair block a
 // multiple of 64 up to the kernels' launch bound (and of the lanes), list grid
-// 64 .. 65536 wavefronts.
+// 64 .. 65536 wavefronts, 1 .. 16 workgroups per replica in the HB resolution.
 static void check_launch_options(const tis_options &o) {
     const int l = o.pair_lanes, b = o.pair_block, w = o.list_waves;
     if (l < 1 || l > 64 || (l & (l - 1))) throw Fail{TIS_EINVAL, "pair_lanes must be a power of two, 1..64"};
     if (b < 64 || b > kBlock || b % 64) throw Fail{TIS_EINVAL, "pair_block must be a multiple of 64, at most 256"};
     if (w < 64 || w > 65536 || (w * 64) % kBlock) throw Fail{TIS_EINVAL, "list_waves must be 64..65536, a multiple of 4"};
     if (o.graph_steps < 0) throw Fail{TIS_EINVAL, "graph_steps"};
+    if (o.hb_groups < 1 || o.hb_groups > kMaxHbGroups) throw Fail{TIS_EINVAL, "hb_groups must be 1..16"};
 }
 
 extern "C" void tis_default_options(tis_options *o) {
@@ -427,6 +427,7 @@ extern "C" void tis_default_options(tis_options *o) {
     o->pair_lanes = kPairLanes;
     o->pair_block = kBlock;
     o->list_waves = kListWaves;
+    o->hb_groups = 4;
 }
 
 static int fail(const Fail &f) {
@@ -479,7 +480,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join_mb, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        HIP_TRY(hipFuncSetAttribute((const void *)k_hbres, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hbres<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_hbres<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
         // ---- offsets and per-replica parameters
         int p0 = 0, b0 = 0, a0 = 0, s30 = 0, s20 = 0, h0 = 0, si0 = 0, i0 = 0, cell0 = 0;
         c->rp.resize(n_rep);
@@ -514,6 +516,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.pair_lanes = o.pair_lanes;
         c->pair_block = o.pair_block;
         c->list_waves = o.list_waves;
+        c->hb_groups = o.hb_groups;
+        D.hb_groups = o.hb_groups;
         auto &own = c->owned;
         D.rp = dalloc<RepParams>(own, n_rep);
         D.rs = dalloc<RepState>(own, n_rep);
@@ -677,6 +681,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
+        D.hb_iter_part = dalloc<int>(own, (size_t)kMaxHbGroups * n_rep);
         D.act_hb = dalloc<int>(own, nhb_tot);
         D.act_idx = dalloc<int>(own, nhb_tot);
         D.act_E = dalloc<double>(own, nhb_tot);
@@ -917,6 +922,9 @@ static void apply_launch(tis_ctx *c, const tis_options &o) {
     c->opt.pair_block = o.pair_block;
     c->opt.list_waves = o.list_waves;
     c->opt.graph_steps = o.graph_steps;
+    c->opt.hb_groups = o.hb_groups;
+    c->hb_groups = o.hb_groups;
+    c->D.hb_groups = o.hb_groups;
     c->D.pair_lanes = o.pair_lanes;
     c->pair_block = o.pair_block;
     c->list_waves = o.list_waves;
@@ -963,10 +971,12 @@ extern "C" int tis_autotune(const tis_system *reps, int n_rep, tis_options *opt,
         sweep(&tis_options::pair_lanes, {2, 4, 8, 16});
         sweep(&tis_options::pair_block, {64, 128, 256});
         sweep(&tis_options::list_waves, {1024, 2048, 4096, 8192, 16384});
+        if (best.rng == TIS_RNG_PHILOX) sweep(&tis_options::hb_groups, {1, 2, 4, 8});
         opt->graph_steps = best.graph_steps;
         opt->pair_lanes = best.pair_lanes;
         opt->pair_block = best.pair_block;
         opt->list_waves = best.list_waves;
+        opt->hb_groups = best.hb_groups;
         if (report) {
             report->ms_per_step_start = start;
             report->ms_per_step_best = best_ms;

@@ -70,6 +70,7 @@ struct Dev {
     uint64_t seed;
     double noise_scale;
     int pair_lanes;               // lanes per particle in the pair gather
+    int hb_groups;                // workgroups per replica in the HB resolution
     RepParams *rp;
     RepState *rs;
     // particles (global index)
@@ -108,6 +109,7 @@ struct Dev {
     int *chunk_cnt;               // live candidates per status chunk, kMaxHbChunks per replica
     // HB dynamic (rebuild products)
     signed char *hb_status;       // -1 / 0 per candidate
+    int *hb_iter_part;            // per replica and resolution group: serial depth of its wavefronts
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off
     int *act_hb;                  // per replica region [hb0, hb0 + nhb): live candidate ids
     int *act_idx;                 // candidate id -> live index or -1

@@ -501,6 +501,7 @@ TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fa
 // ------------------------------------------------------------ k_hb
 // Valence resolution: one workgroup of kResWaves wavefronts per replica.
 constexpr int kResWaves = 16;
+constexpr int kMaxHbGroups = 16;   // workgroups per replica in the resolution (D.hb_groups)
 constexpr int kResBlock = 64 * kResWaves;
 constexpr int kSeqWin = 1024;      // draw window of the sequential form (one wavefront)
 constexpr int kCompWin = 192;      // draw window of each wavefront in the component form
@@ -508,9 +509,9 @@ static_assert(kResWaves * kCompWin >= kSeqWin, "window regions are shared by the
 static_assert(kASiteMax <= 64 * 64, "over-subscribed sites must fit a 64 x 64 bit mask");
 // LDS layout of one replica's working set (bytes, in this order):
 //   E[kActMax] double | uwin[kResWaves * kCompWin] double |
-//   par, excess, apack [kASiteMax] int | A1, croot [kASiteMax] short |
+//   par, excess, apack [kASiteMax] int | A1, croot, cidx [kASiteMax] short |
 //   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag[kActMax] u8
-constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 2 * kASiteMax * 2 +
+constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 3 * kASiteMax * 2 +
                           (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
 static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of one CU");
 
@@ -824,42 +825,17 @@ TIS_DEV void uf_union(volatile int *par, int a, int b) {
     }
 }
 
-// Energy and distance gate of every live candidate (hydrogen_bonds, fast.cpp:2180-2237):
-// E = E_hb exp(-Q) when |r - r0| <= 2 A for the (k11, k21) pair, gated otherwise.
-// Grid (kActMax / kBlock, replicas); also clears the replica's E2/E3 sums.
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_hbpre(Dev D) {
-    const int r = blockIdx.y;
-    RepState &S = D.rs[r];
-    const RepParams &P = D.rp[r];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S.e[EI_E2_HB] = 0.0;
-        S.e[EI_E3_HB] = 0.0;
-    }
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((S.overflow & 2) || a >= S.n_act) return;
-    const int g = D.act_hb[P.hb0 + a];
-    V3<R> v = diff<R>(D.pos[D.hb_b[6 * g + 1]], D.pos[D.hb_b[6 * g + 4]]);
-    const R rr = sqrt(dot(v, v));
-    if (fabs(rr - (R)D.hb_g[6 * g]) > (R)2) {
-        D.act_flag[P.hb0 + a] = FLAG_GATED;
-        D.act_E[P.hb0 + a] = 0.0;
-        return;
-    }
-    R q, psi0[3];
-    hb_geometry<R>(D, g, &q, psi0, nullptr);
-    D.act_E[P.hb0 + a] = (double)((R)D.hb_E[g] * exp(-q));
-    D.act_flag[P.hb0 + a] = FLAG_LIVE;
-}
-
-// Forces of the candidates that survived the resolution, and the E2/E3 sums.
-// Grid (kActMax / kBlock, replicas).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
     const int r = blockIdx.y;
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
     __shared__ double red[8];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // serial depth of the resolution: max over its groups
+        int m = 0;
+        for (int q = 0; q < D.hb_groups; ++q) m = max(m, D.hb_iter_part[kMaxHbGroups * r + q]);
+        S.hb_iters = m;
+    }
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_act = (S.overflow & 2) ? 0 : S.n_act;
     if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
@@ -880,9 +856,9 @@ __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
     }
 }
 
-// Valence resolution of one replica per workgroup, on the energies and gates of
-// k_hbpre; writes the outcome of every live candidate.  mode bit 0: a step
-// (advance the stream cursor past the HB draws and build the noise table).
+// Energies, gates and valence resolution of the live HB candidates; writes the
+// outcome of every live candidate.  mode bit 0: a step (advance the stream
+// cursor past the HB draws and build the noise table).
 //
 // Reference stream: the exact sequential process on wavefront 0.  Philox draws:
 // the over-subscribed sites split into the connected components of the graph
@@ -894,8 +870,22 @@ __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
 // among C's over-subscribed sites: the components' processes give the same joint
 // distribution of switched-off candidates as the single sequential loop, with
 // the serial depth of the largest wavefront's share instead of the total.
+//
+// First, the energy and distance gate of every live candidate
+// (hydrogen_bonds, fast.cpp:2180-2237): E = E_hb exp(-Q) when |r - r0| <= 2 A for
+// the (k11, k21) pair, gated otherwise; computed in precision R into LDS by every
+// group (group 0 also stores the energies for k_hbforce).
+//
+// Grid (groups, replicas).  With Philox draws the components of a replica are
+// shared out over the groups (component k to group k mod groups, each group's
+// wavefronts taking its components in turn); every group builds the tables and
+// components itself and writes back the outcome of the candidates whose
+// component it ran (candidates outside every component: group 0).  Each
+// component draws from its own Philox stream, so the outcome does not depend on
+// the number of groups.  The reference-stream process runs in group 0.
+template <class R>
 __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
-    const int r = blockIdx.x;
+    const int r = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x;
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
     extern __shared__ unsigned char lds[];
@@ -906,7 +896,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     int *apack = excess + kASiteMax;
     short *A1 = (short *)(apack + kASiteMax);
     short *croot = A1 + kASiteMax;
-    short *aoff = croot + kASiteMax;
+    short *cidx = croot + kASiteMax;
+    short *aoff = cidx + kASiteMax;
     short *hoff = aoff + kASiteMax + 4;
     short *alist = hoff + kActMax + 4;
     short *hinc = alist + kAIncMax;
@@ -916,7 +907,12 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __shared__ long long sh_draws;
     __shared__ int sh_err, sh_next;
     const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
+    if (threadIdx.x == 0) {
+        D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
+        if (grp == 0) { S.e[EI_E2_HB] = 0.0; S.e[EI_E3_HB] = 0.0; }   // summed by k_hbforce
+    }
     if (S.overflow & 2) return;
+    if (D.rng == 1 && grp > 0) return;
     if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     // ---- the rebuild tables, energies and gates
@@ -926,8 +922,20 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
     for (int i = tid; i < n_hinc; i += nt) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
     for (int a = tid; a < n_act; a += nt) {
-        E[a] = D.act_E[P.hb0 + a];
-        flag[a] = D.act_flag[P.hb0 + a];
+        const int g = D.act_hb[P.hb0 + a];
+        const V3<R> v = diff<R>(D.pos[D.hb_b[6 * g + 1]], D.pos[D.hb_b[6 * g + 4]]);
+        const R rr = sqrt(dot(v, v));
+        double e = 0.0;
+        unsigned char f = FLAG_GATED;
+        if (!(fabs(rr - (R)D.hb_g[6 * g]) > (R)2)) {
+            R q, psi0[3];
+            hb_geometry<R>(D, g, &q, psi0, nullptr);
+            e = (double)((R)D.hb_E[g] * exp(-q));
+            f = FLAG_LIVE;
+        }
+        E[a] = e;
+        flag[a] = f;
+        if (grp == 0) D.act_E[P.hb0 + a] = e;
     }
     __syncthreads();
     // ---- excess = live incidences - valence per active site; the
@@ -995,7 +1003,10 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             const bool root = ci < nA0 && par[ci] == ci;
             int tot;
             const int rank = block_excl_sum(root, scratch, &tot);
-            if (root) croot[ncomp + rank] = (short)ci;
+            if (root) {
+                croot[ncomp + rank] = (short)ci;
+                cidx[ci] = (short)(ncomp + rank);
+            }
             ncomp += tot;
         }
         __syncthreads();
@@ -1003,7 +1014,9 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         double *ub = uwin + wv * kCompWin;
         int wave_iters = 0;
         bool ovf = false;
-        for (int k = wv; k < ncomp;) {
+        for (int j = wv;;) {
+            const int k = grp + ngrp * j;
+            if (k >= ncomp) break;
             const int root = croot[k], cfirst = root >> 6, c0 = cfirst << 6;
             // member mask: word w from the ballot over indices c0 + 64 w .. + 63
             unsigned long long m = 0;
@@ -1020,9 +1033,9 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
                          &base, &ovf);
             wave_iters += it;
             if (ovf) break;
-            int nk = 0;
-            if (lane == 0) nk = atomicAdd(&sh_next, 1);
-            k = __shfl(nk, 0, 64);
+            int nj = 0;
+            if (lane == 0) nj = atomicAdd(&sh_next, 1);
+            j = __shfl(nj, 0, 64);
         }
         if (lane == 0) {
             atomicMax(&sh_iters, wave_iters);
@@ -1031,12 +1044,20 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     }
     __syncthreads();
     for (int a = tid; a < n_act; a += nt) {
+        int owner = 0;
+        if (D.rng == 0 && flag[a] != FLAG_GATED) {
+            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
+                const int ci = A1[hinc[q]];
+                if (ci >= 0) { owner = cidx[par[ci]] % ngrp; break; }
+            }
+        }
+        if (owner != grp) continue;
         D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
     }
     if (tid == 0) {
-        S.hb_iters = sh_iters;
-        S.hb_draws = sh_draws;
+        D.hb_iter_part[kMaxHbGroups * r + grp] = sh_iters;
+        if (D.rng == 1) S.hb_draws = sh_draws;
         if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);
     }
     // ---- reference-stream noise table: 1500 Box-Muller pairs after the HB draws

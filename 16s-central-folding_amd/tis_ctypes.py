@@ -62,7 +62,7 @@ class Options(C.Structure):
                 ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
                 ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
                 ("graph_steps", C.c_int), ("exchange_every", C.c_longlong), ("pair_lanes", C.c_int),
-                ("pair_block", C.c_int), ("list_waves", C.c_int)]
+                ("pair_block", C.c_int), ("list_waves", C.c_int), ("hb_groups", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -298,7 +298,7 @@ def default_options(lib):
     return o
 
 
-LAUNCH_FIELDS = ("graph_steps", "pair_lanes", "pair_block", "list_waves")
+LAUNCH_FIELDS = ("graph_steps", "pair_lanes", "pair_block", "list_waves", "hb_groups")
 
 
 def autotune(lib, handles, trial_steps=400, **options):

@@ -59,12 +59,13 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
                    + 4 B per SR list entry + 4 B per LR entry / lr_every
       k_multibody  52 B per stack, 16 B per bond, 20 B per angle (+ 32 B force
                    per particle touched, counted as 32 B per particle)
-      k_hbpre      120 B per live HB candidate (6 positions of 32 B would be
-                   192 B; the two shared with the gate pair are counted once: 4
-                   bead positions 128 B minus the 8 B the gate reuses)
-      k_hbres      27 B per live candidate (energy 8, gate 1, id 4, outcome 2,
-                   incidence offsets 4, two incidence entries 8), 12 B per site
-      k_hbforce    same per-candidate inputs as k_hbpre plus 6 x 24 B forces
+      k_hbres      147 B per live HB candidate: its gate and energy, 120 B (6
+                   positions of 32 B would be 192 B; the two shared with the gate
+                   pair are counted once: 4 bead positions 128 B minus the 8 B the
+                   gate reuses), and the resolution's 27 B (energy 8, gate 1, id 4,
+                   outcome 2, incidence offsets 4, two incidence entries 8); 12 B
+                   per site
+      k_hbforce    the same 120 B of bead positions plus 6 x 24 B forces
       k_integrate  148 B per particle (x, x_old, v, 3 forces read, 3 states written)
       k_commit     128 B per particle
     """
@@ -78,10 +79,8 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
             tot += 80 * n + 4 * e["n_sr_pairs"] * 2 + 4.0 * e["n_lr_pairs"] * 2 / lr_every
         elif kernel == "k_multibody":
             tot += 52 * (d["n_st3"] + d["n_st2"]) + 16 * d["n_bonds"] + 20 * d["n_angles"] + 32 * d["n_beads"]
-        elif kernel == "k_hbpre":
-            tot += 120 * nact
         elif kernel == "k_hbres":
-            tot += 27 * nact + 12 * d["n_hb_sites"]
+            tot += 147 * nact + 12 * d["n_hb_sites"]
         elif kernel == "k_hbforce":
             tot += (120 + 144) * nact
         elif kernel == "k_integrate":

@@ -210,6 +210,8 @@ typedef struct tis_options {
     int pair_lanes;       /* lanes per particle in the pair gather: power of two, 1..64 (4) */
     int pair_block;       /* workgroup size of the pair gather: multiple of 64, <= 256 (256) */
     int list_waves;       /* wavefronts of the neighbour-list build grid, 64..65536 (4096) */
+    int hb_groups;        /* workgroups per replica sharing the components of the HB valence
+                             resolution (Philox draws), 1..16 (4) */
 } tis_options;
 
 /* Fills *opt with the reference's settings (FP64, stream RNG off => Philox, skin 3, ...). */
@@ -227,6 +229,7 @@ void tis_destroy(tis_ctx *ctx);
  *   pair_lanes   2, 4, 8, 16         (lanes per particle in the pair gather)
  *   pair_block   64, 128, 256        (its workgroup size)
  *   list_waves   1024 .. 16384       (grid of the neighbour-list build)
+ *   hb_groups    1, 2, 4, 8          (workgroups per replica in the HB resolution; Philox only)
  * writing them into *opt.  Each setting is timed twice over trial_steps steps
  * (after trial_steps / 4 warm-up steps) and its faster run is kept.  The list
  * skin, the list radii and the long-range cadence are not tuned: they decide

@@ -47,11 +47,12 @@ def test_default_options_are_the_reference_settings(lib):
     o = T.default_options(lib)
     assert (o.skin, o.r_sr, o.r_lr, o.lr_every) == (3.0, 15.0, 53.0, 10)
     assert o.precision == T.PREC_FP64 and o.noise_scale == 1.0
-    assert (o.pair_lanes, o.pair_block, o.list_waves, o.graph_steps) == (4, 256, 4096, 50)
+    assert (o.pair_lanes, o.pair_block, o.list_waves, o.graph_steps, o.hb_groups) == (4, 256, 4096, 50, 4)
 
 
 @pytest.mark.parametrize("field,value", [("pair_lanes", 3), ("pair_lanes", 128), ("pair_block", 96),
-                                         ("pair_block", 512), ("list_waves", 10), ("graph_steps", -1)])
+                                         ("pair_block", 512), ("list_waves", 10), ("graph_steps", -1),
+                                         ("hb_groups", 0), ("hb_groups", 17)])
 def test_create_rejects_bad_launch_shapes_without_a_device(lib, cfg1, field, value):
     h = T.SystemHandle(cfg1[0])
     o = T.default_options(lib)

@@ -42,9 +42,44 @@ def test_autotune_returns_a_valid_setting(lib, sys1, cfg1):
     assert 0 < rep["ms_per_step_best"] <= rep["ms_per_step_start"]
     assert best["pair_lanes"] in (2, 4, 8, 16) and best["pair_block"] in (64, 128, 256)
     assert best["graph_steps"] in (25, 50, 100, 200) and best["list_waves"] in (1024, 2048, 4096, 8192, 16384)
+    assert best["hb_groups"] in (1, 2, 4, 8)
     eng = T.Engine(lib, hs, precision=T.PREC_MIXED, **best)
     eng.step(200)
     for r in range(2):
         pos, _, _ = eng.get_state(r)
         assert np.isfinite(pos).all()
     eng.close()
+
+
+@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
+def test_hb_outcome_does_not_depend_on_the_resolution_groups(lib, sys1, prec):
+    """Components of the Philox resolution are shared out over 1..16 workgroups
+    per replica; each component draws from its own stream, so at one state the
+    outcome, the energies and the forces are those of one workgroup (forces to
+    the rounding of the per-bead atomics), and 30-step trajectories agree to the
+    rounding those atomics leave (amplified by FP32 force arithmetic in mixed
+    precision)."""
+    d2, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.050, seed=4, box=450.0)
+    hs = [sys1, T.SystemHandle(d2)]
+    ref, states = None, None
+    for g in (1, 2, 4, 8, 16):
+        eng = T.Engine(lib, hs, precision=prec, rng=T.RNG_PHILOX, seed=3, hb_groups=g)
+        eng.step(30)
+        x30 = [eng.get_state(r)[0] for r in range(2)]
+        if states is None:
+            states = [eng.get_state(r) for r in range(2)]
+        res = []
+        for r in range(2):
+            eng.set_state(*states[r], klok=31, rep=r)
+            f, e = eng.eval(T.TERM_HB, rep=r)
+            res.append((eng.hb_outcome(r), f, e["e2_hb"] + e["e3_hb"], x30[r]))
+        eng.close()
+        if ref is None:
+            ref = res
+            assert all((o == 2).any() for o, _, _, _ in res)   # the resolution switched candidates off
+            continue
+        for (o, f, e, x), (o0, f0, e0, x0) in zip(res, ref):
+            assert np.array_equal(o, o0), g
+            assert np.linalg.norm(f - f0) <= 1e-12 * np.linalg.norm(f0), g
+            assert e == pytest.approx(e0, rel=1e-12), g
+            assert np.abs(x - x0).max() < (1e-9 if prec == T.PREC_FP64 else 1e-6), g
