@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_jvp_pairs(Dev D, double *grad_ion_r)
             const bool is_sr = q < nsr;
             const int e = is_sr ? nb[q] : nb[P.cap_sr + q - nsr];
             const bool bonded = e < 0;
-            const int j = bonded ? ~e : e;
+            const int j = D.sorted[bonded ? ~e : e];
             double3 dd;
             const double r2v = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
             const double4 vj = D.jv[j];

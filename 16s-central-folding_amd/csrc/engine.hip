@@ -534,6 +534,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.cell_start = dalloc<int>(own, cell0);
         D.sorted = dalloc<int>(own, N);
         D.cpos = dalloc<double4>(own, N);
+        D.crank = dalloc<int>(own, N);
         D.nb_start = dalloc<int>(own, N);
         D.nb_nsr = dalloc<int>(own, N);
         D.nb_nlr = dalloc<int>(own, N);
@@ -1300,14 +1301,15 @@ extern "C" int tis_get_pairs(tis_ctx *c, int rep, int which, int *ij_out, long l
         HIP_TRY(hipMemcpy(st.data(), c->D.nb_start + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(ns.data(), c->D.nb_nsr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(nl.data(), c->D.nb_nlr + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<int> pool((size_t)P.pool_cap);
+        std::vector<int> pool((size_t)P.pool_cap), sorted(P.n);
         HIP_TRY(hipMemcpy(pool.data(), c->D.pool + P.pool0, pool.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(sorted.data(), c->D.sorted + P.p0, P.n * sizeof(int), hipMemcpyDeviceToHost));
         std::vector<std::pair<int, int>> pr;
         for (int i = 0; i < P.n; ++i) {
             const int b = st[i] + (which ? P.cap_sr : 0), n = which ? nl[i] : ns[i];
             for (int q = 0; q < n; ++q) {
-                const int e = pool[b + q];
-                const int j = (e < 0 ? ~e : e) - P.p0;
+                const int e = pool[b + q];   // cell-order index of the partner
+                const int j = sorted[(e < 0 ? ~e : e) - P.p0] - P.p0;
                 if (i < j) pr.push_back({i, j});
             }
         }

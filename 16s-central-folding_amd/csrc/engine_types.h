@@ -81,11 +81,13 @@ struct Dev {
     int *prep;                    // replica of each particle
     // cells and lists
     int *cell_of, *cell_rank, *cell_start, *sorted;
-    double4 *cpos;                // positions in cell order, .w = particle index
+    double4 *cpos;                // positions in cell order (current: k_commit updates them),
+                                  // .w = particle index (low word) and type word (high word)
+    int *crank;                   // cell-order index of each particle (last rebuild)
     int *excl;                    // kExcl bonded / 1-3 partners per particle (global, -1 none)
     unsigned char *bsub;          // per bonded term: subtract its pair's WCA in the step path too
     int *nb_start, *nb_nsr, *nb_nlr;
-    int *pool;
+    int *pool;                    // neighbour entries: cell-order indices (~q for bonded / 1-3)
     // topology
     int2 *bond_ij;
     double2 *bond_kr;

@@ -80,6 +80,13 @@ TIS_DEV int block_excl_sum(int v, int *scratch, int *total) {
 
 TIS_DEV double4 d4add(double4 a, double4 b) { return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, 0.0); }
 
+// The .w word of a cell-ordered position holds the particle index (low 32 bits)
+// and its type word (high 32 bits), so one 32 B load gives a neighbour's
+// position, identity and type.
+TIS_DEV double cp_pack(int p, int ptype) { return __hiloint2double(ptype, p); }
+TIS_DEV int cp_index(double w) { return __double2loint(w); }
+TIS_DEV int cp_type(double w) { return __double2hiint(w); }
+
 template <class R>
 TIS_DEV void atomic_add_force(double4 *f, int i, V3<R> v) {
     unsafeAtomicAdd(&f[i].x, (double)v.x);
@@ -186,7 +193,8 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
         const int q = P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p];
         const double4 x = D.pos[p];
         D.sorted[q] = p;
-        D.cpos[q] = make_double4(x.x, x.y, x.z, (double)p);
+        D.crank[p] = q;
+        D.cpos[q] = make_double4(x.x, x.y, x.z, cp_pack(p, D.ptype[p]));
     }
     if (threadIdx.x == 0) {
         S.max_sr = 0;
@@ -219,8 +227,9 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *nc
     }
 }
 
-// Visits every particle of the neighbour cells, 64 per call of visit(j, x),
-// lanes past the last candidate getting j = -1.  The cells' particles are taken
+// Visits every particle of the neighbour cells, 64 per call of visit(j, x, q)
+// (particle j, its cell-ordered position x at global cell-order index q), lanes
+// past the last candidate getting j = -1.  The cells' particles are taken
 // as one sequence (cell c covers positions [excl_c, incl_c) of it, prefix sums
 // over the lanes that hold the cells), so every chunk of 64 lanes is full
 // whatever the cell occupancies; a lane finds its cell by a scalar walk over the
@@ -236,7 +245,8 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
     if (total == 0) return;
     const double4 *cp = D.cpos + P.p0;
     int c = 0;   // first cell whose range ends after the chunk start (uniform)
-    auto load = [&](int t0, double4 *x) {
+    const double none = cp_pack(-1, 0);
+    auto load = [&](int t0, double4 *x, int *q) {
         while (readlane_i(incl, c) <= t0) ++c;
         const int t = t0 + lane;
         int addr = -1;
@@ -245,18 +255,22 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
             if (ex >= t0 + 64) break;
             if (t >= ex && t < in) addr = readlane_i(cstart, cc) + (t - ex);
         }
-        *x = addr >= 0 ? cp[addr] : make_double4(0, 0, 0, -1.0);
+        *x = addr >= 0 ? cp[addr] : make_double4(0, 0, 0, none);
+        *q = P.p0 + addr;
     };
     double4 cur;
-    load(0, &cur);
+    int qcur;
+    load(0, &cur, &qcur);
     for (int t0 = 0;;) {
         t0 += 64;
         const bool more = t0 < total;
-        double4 nxt = make_double4(0, 0, 0, -1.0);
-        if (more) load(t0, &nxt);
-        visit((int)cur.w, cur);
+        double4 nxt = make_double4(0, 0, 0, none);
+        int qnxt = -1;
+        if (more) load(t0, &nxt, &qnxt);
+        visit(cp_index(cur.w), cur, qcur);
         if (!more) break;
         cur = nxt;
+        qcur = qnxt;
     }
 }
 
@@ -279,23 +293,23 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
         const long long stride = (long long)P.cap_sr + P.cap_lr;
         for (int t = w0; t < P.n; t += nw) {
             const double4 xi = D.cpos[P.p0 + t];
-            const int p = (int)xi.w;
-            const bool chg_i = (D.ptype[p] >> 8) & 1;
+            const int p = cp_index(xi.w);
+            const bool chg_i = (cp_type(xi.w) >> 8) & 1;
             const int cell = D.cell_of[p];
-            // bonded / 1-3 partners of p: their SR entries are stored as ~j, so
-            // the step path leaves out their WCA term
+            // entries are cell-order indices q of the partners; bonded / 1-3
+            // partners of p are stored as ~q, so the step path leaves out their WCA term
             int ex[kExcl];
 #pragma unroll
             for (int e = 0; e < kExcl; ++e) ex[e] = D.excl[kExcl * p + e];
             int *out_sr = D.pool + P.pool0 + (long long)(p - P.p0) * stride;
             int *out_lr = out_sr + P.cap_sr;
             int a = 0, b = 0;
-            for_neighbour_cells_wave(D, P, cell, [&](int j, const double4 &xj) {
+            for_neighbour_cells_wave(D, P, cell, [&](int j, const double4 &xj, int qj) {
                 bool isr = false, ilr = false;
                 if (j >= 0 && j != p) {
                     const double r2 = min_image_r2(xi, xj, P.L, P.hL);
                     isr = r2 <= P.rsr2;
-                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((D.ptype[j] >> 8) & 1);
+                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((cp_type(xj.w) >> 8) & 1);
                 }
                 const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
                 if (mode == 1) {
@@ -304,8 +318,8 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 #pragma unroll
                     for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
                     const int ka = a + __popcll(ms & below), kb = b + __popcll(ml & below);
-                    if (isr && ka < P.cap_sr) out_sr[ka] = is_ex ? ~j : j;
-                    if (ilr && kb < P.cap_lr) out_lr[kb] = j;
+                    if (isr && ka < P.cap_sr) out_sr[ka] = is_ex ? ~qj : qj;
+                    if (ilr && kb < P.cap_lr) out_lr[kb] = qj;
                 }
                 a += __popcll(ms);
                 b += __popcll(ml);
@@ -915,19 +929,71 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     if (D.rng == 1 && grp > 0) return;
     if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    // ---- the rebuild tables, energies and gates
-    for (int i = tid; i <= n_as; i += nt) aoff[i] = (short)D.asite_off[P.site0 + r + i];
-    for (int i = tid; i < n_ainc; i += nt) alist[i] = (short)D.asite_list[P.inc0 + i];
-    for (int i = tid; i <= n_act; i += nt) hoff[i] = (short)D.ahb_inc_off[P.hb0 + r + i];
+    // ---- the rebuild tables, site valences, candidate energies and gates.  A
+    // thread issues all its loads of one level before it uses any: level 1 the
+    // tables, the live candidates' ids and the active sites' ids; level 2 the
+    // candidates' gate beads and r0 and the sites' valences; level 3 the gate
+    // positions, then the geometry of the candidates inside the gate.
     const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
-    for (int i = tid; i < n_hinc; i += nt) hinc[i] = (short)D.ahb_inc[P.inc0 + i];
+    if (n_hinc > kAIncMax) {                      // inconsistent rebuild tables
+        if (tid == 0) atomicOr(&S.overflow, 2);
+        return;
+    }
+    constexpr int KA = (kASiteMax + kResBlock) / kResBlock;   // slots per thread: sites (+1)
+    constexpr int KC = (kActMax + kResBlock) / kResBlock;     // live candidates (+1)
+    constexpr int KI = (kAIncMax + kResBlock - 1) / kResBlock;  // incidences
+    int t_aoff[KA], t_sid[KA], t_hoff[KC], t_alist[KI], t_hinc[KI];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int i = tid + k * kResBlock;
+        t_aoff[k] = i <= n_as ? D.asite_off[P.site0 + r + i] : 0;
+        t_sid[k] = i < n_as ? D.asite_id[P.site0 + i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int i = tid + k * kResBlock;
+        t_hoff[k] = i <= n_act ? D.ahb_inc_off[P.hb0 + r + i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        const int i = tid + k * kResBlock;
+        t_alist[k] = i < n_ainc ? D.asite_list[P.inc0 + i] : 0;
+        t_hinc[k] = i < n_hinc ? D.ahb_inc[P.inc0 + i] : 0;
+    }
+    // the thread's first candidate goes through the same levels (later ones, rare
+    // below 1024 live candidates, load in turn)
+    const int g0 = tid < n_act ? D.act_hb[P.hb0 + tid] : 0;
+    int t_val[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) t_val[k] = tid + k * kResBlock < n_as ? D.site_val[t_sid[k]] : 0;
+    const int b10 = tid < n_act ? D.hb_b[6 * g0 + 1] : 0, b40 = tid < n_act ? D.hb_b[6 * g0 + 4] : 0;
+    const double r00 = tid < n_act ? D.hb_g[6 * g0] : 0.0;
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int i = tid + k * kResBlock;
+        if (i <= n_as) aoff[i] = (short)t_aoff[k];
+        if (i < n_as) excess[i] = t_val[k];       // the valence, until the excess replaces it
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+        if (tid + k * kResBlock <= n_act) hoff[tid + k * kResBlock] = (short)t_hoff[k];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        const int i = tid + k * kResBlock;
+        if (i < n_ainc) alist[i] = (short)t_alist[k];
+        if (i < n_hinc) hinc[i] = (short)t_hinc[k];
+    }
+#pragma unroll 1
     for (int a = tid; a < n_act; a += nt) {
-        const int g = D.act_hb[P.hb0 + a];
-        const V3<R> v = diff<R>(D.pos[D.hb_b[6 * g + 1]], D.pos[D.hb_b[6 * g + 4]]);
+        const bool first = a == tid;
+        const int g = first ? g0 : D.act_hb[P.hb0 + a];
+        const int b1 = first ? b10 : D.hb_b[6 * g + 1], b4 = first ? b40 : D.hb_b[6 * g + 4];
+        const double r0 = first ? r00 : D.hb_g[6 * g];
+        const V3<R> v = diff<R>(D.pos[b1], D.pos[b4]);
         const R rr = sqrt(dot(v, v));
         double e = 0.0;
         unsigned char f = FLAG_GATED;
-        if (!(fabs(rr - (R)D.hb_g[6 * g]) > (R)2)) {
+        if (!(fabs(rr - (R)r0) > (R)2)) {
             R q, psi0[3];
             hb_geometry<R>(D, g, &q, psi0, nullptr);
             e = (double)((R)D.hb_E[g] * exp(-q));
@@ -948,7 +1014,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         if (s < n_as) {
             int c = 0;
             for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
-            ex = c - D.site_val[D.asite_id[P.site0 + s]];
+            ex = c - excess[s];
             excess[s] = ex;
         }
         int tot;
@@ -1324,12 +1390,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     const int t = (blk * blockDim.x + threadIdx.x) / lanes;
     const int sub = threadIdx.x % lanes;
     const bool valid = t < D.n_tot;
-    const int p = valid ? D.sorted[t] : D.sorted[0];
+    // own and neighbour positions from the cell-ordered copy (kept current by
+    // k_commit): neighbours in the same few cells share cache lines
+    const double4 xi = D.cpos[valid ? t : 0];
+    const int p = cp_index(xi.w);
     const int r = D.prep[p];
     const RepParams &P = D.rp[r];
     RepState &S = D.rs[r];
-    const double4 xi = D.pos[p];
-    const int pti = D.ptype[p];
+    const int pti = cp_type(xi.w);
     const int ti = pti & 15;
     const bool chg_i = (pti >> 8) & 1;
     const int *nb = D.pool + P.pool0 + D.nb_start[p];
@@ -1344,10 +1412,10 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         for (int q = sub; q < nsr; q += lanes) {
             const int e = nb[q];
             const bool bonded = e < 0;
-            const int j = bonded ? ~e : e;
+            const double4 xj = D.cpos[bonded ? ~e : e];
             double3 dd;
-            const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
-            const int ptj = D.ptype[j];
+            const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
+            const int ptj = cp_type(xj.w);
             const int ab = ti * kNT + (ptj & 15);
             // a bonded / 1-3 pair carries no net WCA force (the reference adds it
             // here and removes it in the bonded terms); only evaluations, which
@@ -1384,11 +1452,11 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         const int n_lr = lr ? nlr : 0;
         const int *nb_lr = nb + P.cap_sr;
         for (int q = sub; q < n_lr; q += lanes) {
-            const int j = nb_lr[q];
+            const double4 xj = D.cpos[nb_lr[q]];
             double3 dd;
-            const double r2 = min_image_r2(xi, D.pos[j], P.L, P.hL, &dd);
+            const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
             if (r2 > kCoulCut2) continue;
-            const int ab = ti * kNT + (D.ptype[j] & 15);
+            const int ab = ti * kNT + (cp_type(xj.w) & 15);
             const R rr = sqrt((R)r2);
             const R f = (R)P.qlb[ab] / (rr * (R)r2);
             fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
@@ -1512,10 +1580,16 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
         }
     }
     if (p >= D.n_tot) return;
-    D.pos[p] = D.pos[D.n_tot + p];
+    const double4 x = D.pos[D.n_tot + p];
+    D.pos[p] = x;
     D.old[p] = D.old[D.n_tot + p];
     D.vel[p] = D.vel[D.n_tot + p];
     D.f_mb[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+    // the cell-ordered copy the pair gather reads (its .w, identity and type, stays)
+    double4 *c = D.cpos + D.crank[p];
+    c->x = x.x;
+    c->y = x.y;
+    c->z = x.z;
 }
 
 // ------------------------------------------------------------ replica exchange

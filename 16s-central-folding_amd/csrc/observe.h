@@ -55,9 +55,10 @@ __global__ void __launch_bounds__(kBlock) k_obs_contacts(Dev D, double cut2) {
         for (int q = 0; q < nsr; ++q) {
             const int e = nb[q];
             if (e < 0) continue;                       // bonded / 1-3 partner: a bead
-            const int cj = (D.ptype[e] >> 4) & 15;
+            const double4 xj = D.cpos[e];              // entries are cell-order indices
+            const int cj = (cp_type(xj.w) >> 4) & 15;
             if (cj < 3 || cj > 5) continue;
-            if (min_image_r2(xi, D.pos[e], P.L, P.hL) <= cut2) ++cnt[cj - 3];
+            if (min_image_r2(xi, xj, P.L, P.hL) <= cut2) ++cnt[cj - 3];
         }
         double4 a = D.obs_p[p];
         a.x += cnt[0]; a.y += cnt[1]; a.z += cnt[2]; a.w += cnt[0] > 0 ? 1.0 : 0.0;
@@ -66,8 +67,9 @@ __global__ void __launch_bounds__(kBlock) k_obs_contacts(Dev D, double cut2) {
         for (int q = 0; q < nsr; ++q) {
             const int e = nb[q];
             if (e < 0) continue;
-            if (((D.ptype[e] >> 4) & 15) != 0) continue;
-            if (min_image_r2(xi, D.pos[e], P.L, P.hL) <= cut2) {
+            const double4 xj = D.cpos[e];
+            if (((cp_type(xj.w) >> 4) & 15) != 0) continue;
+            if (min_image_r2(xi, xj, P.L, P.hL) <= cut2) {
                 atomicAdd(&D.obs_cnt[4 * r + cls - 3], 1);
                 break;
             }
