@@ -150,16 +150,16 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "copies": 5}))
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
+def measured_traffic(kernel, summary="pmc_traffic_batched16_16SCD.json"):
+    """HBM bytes per launch of `kernel` from a committed counter summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy of known size), or None."""
-    path = os.path.join(REPO, "profiles", "pmc_traffic_batched16_16SCD.json")
+    path = os.path.join(REPO, "profiles", summary)
     try:
         with open(path) as f:
             k = json.load(f)["kernels"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
-    return None if k is None else k["hbm_bytes"]
+    return None if k is None else k.get("hbm_bytes")
 
 
 def cpu_baseline(systems, steps_each):
@@ -184,10 +184,10 @@ def measured_valu():
     """Per-kernel VALU issue fraction on the chain workload from the committed
     counter summary (SQ_INSTS_VALU per dispatch over the dispatch's duration
     times 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per cycle), or None."""
-    path = os.path.join(REPO, "profiles", "pmc_valu_chain1500.json")
+    path = os.path.join(REPO, "profiles", "pmc_chain1500.json")
     try:
         with open(path) as f:
-            return json.load(f)["kernels"]
+            return {k: v for k, v in json.load(f)["kernels"].items() if "valu_frac" in v}
     except (OSError, ValueError, KeyError):
         return None
 
@@ -250,7 +250,8 @@ def run_chain(args, lib):
         "roofline": None if dominant is None else {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
-            "traffic": None, "bytes_per_launch": bytes_launch, "avg_ms": avg_ms},
+            "traffic": measured_traffic(dominant, "pmc_chain1500.json"), "bytes_per_launch": bytes_launch,
+            "avg_ms": avg_ms},
         "roofline_step": {"bound": "hbm", "bytes_per_step": b_step, "achieved": b_step * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
         "valu_issue_frac": None if valu is None else {k: v["valu_frac"] for k, v in valu.items()},
@@ -323,9 +324,11 @@ def run_force_match(args, lib, n_snap=256):
 def tune(lib, handles, args):
     """Launch shapes for the timed run: tis_autotune's choice on a scratch copy of
     the batch (its trials are outside the timed region), or the defaults."""
-    if args.no_autotune or args.no_graph:
+    if args.launch is not None or args.no_autotune or args.no_graph:
         o = T.default_options(lib)
-        return {k: getattr(o, k) for k in T.LAUNCH_FIELDS}, None
+        launch = {k: getattr(o, k) for k in T.LAUNCH_FIELDS}
+        launch.update(args.launch or {})
+        return launch, None
     launch, rep = T.autotune(lib, handles, trial_steps=args.tune_steps, precision=T.PREC_MIXED,
                              rng=T.RNG_PHILOX, seed=2024)
     log(f"autotune: {launch} ({rep['trials']} settings, {rep['ms_per_step_start']:.4f} -> "
@@ -350,6 +353,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for counter collection)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of tis_autotune's")
     ap.add_argument("--tune-steps", type=int, default=400, help="steps per autotuner trial")
+    ap.add_argument("--launch", type=json.loads, default=None, metavar="JSON",
+                    help="launch shapes to use instead of tis_autotune's (e.g. for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
