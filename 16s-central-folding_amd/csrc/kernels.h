@@ -673,6 +673,10 @@ TIS_DEV bool hb_accept(double u, double dE, double T, double invT) {
     return u < exp(dE / T);
 }
 
+// Sites with at most this many live candidates walk with the tests on the walk's
+// path only; longer walks evaluate all tests of a pass at once.
+constexpr int kWalkSeq = 8;
+
 TIS_DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -688,14 +692,16 @@ TIS_DEV void wave_lds_sync() {
 //
 // The set is a bit mask over over-subscribed site indices ci (site order), word w
 // held by lane w in register w0 on entry: bit b stands for index c0 + b, whose
-// candidate list is apack[ci] = offset | length << 16 in alist; A1 maps a site
-// back to its index (-1 if it was not over-subscribed).  The pick is a wavefront
-// prefix count; all acceptance tests of a pass are evaluated at once (lane t
-// holds pair (i, c), c < i, triangular index t = i(i-1)/2 + c) and the walk reads
-// their bits.  Draws come from an LDS window of `win` values that the wavefront
+// candidate list is apack[ci] = offset | length << 16 in alist and whose excess
+// is excess[ci]; a candidate's incidences hinc hold the over-subscribed index of
+// each of its sites (-1 for the others, whose excess no longer matters: it never
+// rises).  The pick is a wavefront prefix count; a walk over more than kWalkSeq
+// candidates evaluates all acceptance tests of a pass at once (lane t holds pair
+// (i, c), c < i, triangular index t = i(i-1)/2 + c) and reads their bits, a
+// shorter one evaluates the tests on its path.  Draws come from an LDS window of `win` values that the wavefront
 // refills.  Sets *ovf on a site with more than 64 live candidates or an
 // inconsistent table.
-TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack, const short *A1,
+TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack,
                           const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
                           const double *E, double T, double *ubuf, int win, const HbDraws &dr, int comp,
                           int *iters_out, long long *base_out, bool *ovf) {
@@ -765,7 +771,7 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         // ---- walk: k moves to i when u_{nK+i} < exp((E_i - E_k)/T)
         int k = 0;
         const int npair = nK * (nK - 1) / 2;
-        if (npair <= 256) {
+        if (nK > kWalkSeq && npair <= 256) {
             unsigned long long m[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
@@ -782,10 +788,12 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
                 if ((mt >> (t & 63)) & 1ull) k = i;
             }
         } else {
+            // a short walk: only the tests on its path, draws loaded at once
+            const double ul = lane < nK ? u[nK + lane] : 0.0;
             double Ek = readlane_d(Ev, 0);
             for (int i = 1; i < nK; ++i) {
                 const double Ei = readlane_d(Ev, i);
-                if (hb_accept(u[nK + i], Ei - Ek, T, invT)) { k = i; Ek = Ei; }
+                if (hb_accept(readlane_d(ul, i), Ei - Ek, T, invT)) { k = i; Ek = Ei; }
             }
         }
         const int off = readlane_i(bv, k);
@@ -797,10 +805,11 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         int b = -1;
         bool cleared = false;
         if (lane < hn) {
-            const int st = hinc[h0 + lane];
-            const int ia = A1[st];
-            cleared = atomicSub(&excess[st], 1) == 1;
-            b = ia - c0;
+            const int ci2 = hinc[h0 + lane];   // over-subscribed index of the site, or -1
+            if (ci2 >= 0) {
+                cleared = atomicSub(&excess[ci2], 1) == 1;
+                b = ci2 - c0;
+            }
         }
         unsigned long long cm = __ballot(cleared && b >= 0 && b < 64 * 64);
         while (cm) {
@@ -1015,16 +1024,24 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             int c = 0;
             for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
             ex = c - excess[s];
-            excess[s] = ex;
         }
         int tot;
         const int rank = block_excl_sum(ex > 0, scratch, &tot);
         if (s < n_as) {
             A1[s] = (short)(ex > 0 ? nA0 + rank : -1);
-            if (ex > 0) apack[nA0 + rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
+            // excess re-indexed by over-subscribed index (nA0 + rank <= s: no
+            // site of this or a later round is overwritten; this round's reads
+            // are done, block_excl_sum synchronises)
+            if (ex > 0) {
+                apack[nA0 + rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
+                excess[nA0 + rank] = ex;
+            }
         }
         nA0 += tot;
     }
+    __syncthreads();
+    // incidences as over-subscribed indices (-1: a site that is not)
+    for (int q = tid; q < n_hinc; q += nt) hinc[q] = A1[hinc[q]];
     __syncthreads();
     const HbDraws dr(D, r, S);
     if (D.rng == 1) {
@@ -1036,7 +1053,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             int iters = 0;
             long long base = 0;
             bool ovf = false;
-            resolve_wave(w0, excess, 0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, uwin, kSeqWin, dr, 0, &iters,
+            resolve_wave(w0, excess, 0, nA0, apack, alist, hoff, hinc, flag, E, P.T, uwin, kSeqWin, dr, 0, &iters,
                          &base, &ovf);
             if (lane == 0) {
                 sh_iters = iters;
@@ -1053,7 +1070,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             if (flag[a] != FLAG_LIVE) continue;
             int first = -1;
             for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = A1[hinc[q]];
+                const int ci = hinc[q];
                 if (ci < 0) continue;
                 if (first < 0) first = ci;
                 else uf_union(vpar, first, ci);
@@ -1095,7 +1112,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
             int it = 0;
             long long base = 0;
-            resolve_wave(m, excess, c0, nA0, apack, A1, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
+            resolve_wave(m, excess, c0, nA0, apack, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
                          &base, &ovf);
             wave_iters += it;
             if (ovf) break;
@@ -1113,7 +1130,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         int owner = 0;
         if (D.rng == 0 && flag[a] != FLAG_GATED) {
             for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = A1[hinc[q]];
+                const int ci = hinc[q];
                 if (ci >= 0) { owner = cidx[par[ci]] % ngrp; break; }
             }
         }
