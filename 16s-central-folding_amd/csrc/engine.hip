@@ -278,8 +278,9 @@ struct tis_ctx {
         launch_hb(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
-            if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
-            else k_force<float, 0><<<n_rep + nb_bonded, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
+            const int nmb = n_rep * D.st_slices + nb_bonded;
+            if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
+            else k_force<float, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
         launch_rebuild_lists(s_main);
@@ -693,6 +694,12 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.ahb_inc_off = dalloc<int>(own, nhb_tot + n_rep);
         D.ahb_inc = dalloc<int>(own, inc_base);
         D.st_energy = dalloc<double>(own, st3E.size());
+        D.st_part = dalloc<double>(own, 2 * (size_t)kMaxStSlices * n_rep);
+        {   // one stack per thread in a step: slices of kBlock stacks
+            int mx = 1;
+            for (const RepParams &q : c->rp) mx = std::max(mx, q.nst3 + q.nst2);
+            D.st_slices = std::min(kMaxStSlices, (mx + kBlock - 1) / kBlock);
+        }
         D.maxwell = dalloc<double>(own, 3000 * (size_t)n_rep);
         D.stream_base = dalloc<long long>(own, n_rep);
         D.stream_len = dalloc<long long>(own, n_rep);

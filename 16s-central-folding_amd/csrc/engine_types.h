@@ -71,6 +71,7 @@ struct Dev {
     double noise_scale;
     int pair_lanes;               // lanes per particle in the pair gather
     int hb_groups;                // workgroups per replica in the HB resolution
+    int st_slices;                // stack workgroups per replica in a step
     RepParams *rp;
     RepState *rs;
     // particles (global index)
@@ -124,6 +125,7 @@ struct Dev {
     int *ahb_inc;                 // region [inc0, ...): active-site index of each incidence
     // output / scratch
     double *st_energy;            // per tertiary stack
+    double *st_part;              // per replica and stack slice: E2, E3 stack sums of the step
     double *maxwell;              // 3000 per replica (stream mode)
     int32_t *stream;              // raw glibc values, region per replica
     long long *stream_base, *stream_len;

@@ -1312,6 +1312,7 @@ TIS_DEV bool bonded_term(const Dev &D, double4 *F, int g, int nbond, int nang, i
 
 // ------------------------------------------------------------ k_force
 constexpr int kPairLanes = 4;   // default lanes per particle in the pair gather (D.pair_lanes)
+constexpr int kMaxStSlices = 16;  // stack workgroups per replica in a step (D.st_slices)
 // ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
 // backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
 // that the register-heavy stacking code does not lower the occupancy of the
@@ -1328,9 +1329,13 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     const bool want_e = eval == kEvalFull || eval == kEvalEnergy;
     const bool split = want_e;
     const bool write_f = eval != kEvalEnergy;
-    if (ROLE == 0 && (int)blockIdx.x < D.n_rep) {
+    // stack blocks: in a step each replica's stacks are shared out over
+    // D.st_slices workgroups (one stack per thread), whose energy sums k_integrate
+    // adds up; evaluations use one workgroup per replica
+    const int sb = eval == kEvalStep ? D.st_slices : 1;
+    if (ROLE == 0 && (int)blockIdx.x < D.n_rep * sb) {
         if (!(mask & 1)) return;
-        const int r = blockIdx.x;
+        const int r = blockIdx.x / sb, slice = blockIdx.x % sb;
         const RepParams &P = D.rp[r];
         RepState &S = D.rs[r];
         __shared__ unsigned char off[4096];
@@ -1350,7 +1355,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         __syncthreads();
         double e2 = 0, e3 = 0;
-        for (int k = threadIdx.x; k < P.nst3 + P.nst2; k += blockDim.x) {
+        for (int k = slice * blockDim.x + threadIdx.x; k < P.nst3 + P.nst2; k += sb * blockDim.x) {
             if (k < P.nst3) {
                 const int g = P.st30 + k;
                 const double e = (double)tertiary_stack<R>(D, F, g, nullptr);
@@ -1364,10 +1369,18 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         double s2 = block_sum_double(e2, red);
         double s3 = block_sum_double(e3, red);
-        if (threadIdx.x == 0) { S.e[EI_E2_ST] = s2; S.e[EI_E3_ST] = s3; }
+        if (threadIdx.x == 0) {
+            if (sb == 1) {
+                S.e[EI_E2_ST] = s2;
+                S.e[EI_E3_ST] = s3;
+            } else {
+                D.st_part[2 * (kMaxStSlices * r + slice)] = s2;
+                D.st_part[2 * (kMaxStSlices * r + slice) + 1] = s3;
+            }
+        }
         return;
     }
-    int blk = ROLE == 0 ? (int)blockIdx.x - D.n_rep : (int)blockIdx.x;
+    int blk = ROLE == 0 ? (int)blockIdx.x - D.n_rep * sb : (int)blockIdx.x;
     if (ROLE == 0) {
         if (blk >= nb_bonded) return;
         if (!(mask & 4)) return;
@@ -1536,6 +1549,15 @@ TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S
 
 __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < D.n_rep && D.st_slices > 1) {   // stack energies of the step's slices
+        double s2 = 0, s3 = 0;
+        for (int q = 0; q < D.st_slices; ++q) {
+            s2 += D.st_part[2 * (kMaxStSlices * p + q)];
+            s3 += D.st_part[2 * (kMaxStSlices * p + q) + 1];
+        }
+        D.rs[p].e[EI_E2_ST] = s2;
+        D.rs[p].e[EI_E3_ST] = s3;
+    }
     if (p < D.n_tot) {
         const int r = D.prep[p];
         const RepParams &P = D.rp[r];

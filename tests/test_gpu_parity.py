@@ -361,3 +361,28 @@ def test_exchange_cadence_in_steps_and_incompatible_neighbours(lib):
         p, _, v = eng.get_state(r)
         assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
     eng.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(T.PREC_FP64, 1e-12), (T.PREC_MIXED, 1e-6)])
+def test_step_stack_energies_equal_an_evaluation(lib, prec, tol):
+    """In a step the stacks of a replica are spread over several workgroups and
+    their energy sums added up after the streams join; the E2 / E3 stack energies
+    a step reports are those of an evaluation of the state it started from (two
+    replicas of different systems in one batch)."""
+    hs = []
+    for c, box, seed in ((0.005, 350.0, 1), (0.02, 450.0, 2)):
+        d, _ = T.build_16scd(lib, T=0.596, c_mg=c, c_k=0.050, seed=seed, box=box)
+        hs.append(T.SystemHandle(d))
+    eng = T.Engine(lib, hs, precision=prec, rng=T.RNG_PHILOX, seed=9, use_graph=0)
+    eng.step(20)
+    want = []
+    for r in range(2):
+        _, e = eng.eval(T.TERM_STACK, rep=r)
+        want.append((e["e2_stack"], e["e3_stack"]))
+    eng.step(1)
+    for r in range(2):
+        e = eng.energies(r)
+        assert e["e2_stack"] == pytest.approx(want[r][0], rel=tol), r
+        assert e["e3_stack"] == pytest.approx(want[r][1], rel=tol), r
+        assert want[r][0] != 0.0 and want[r][1] != 0.0
+    eng.close()
