@@ -820,7 +820,7 @@ extern "C" int tis_set_state(tis_ctx *c, int rep, const double *pos, const doubl
         all[rep].klok = klok;
         all[rep].progress = 0;
         all[rep].lr_due = 1;
-        all[rep].maxd2 = 0;
+        for (unsigned long long &m : all[rep].maxd2) m = 0;
         all[rep].overflow = 0;
         HIP_TRY(hipMemcpy(c->D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
         c->launch_rebuild();

@@ -14,6 +14,7 @@ constexpr int kNT = 9;            // parameter types
 constexpr int kMaxCells = 8192;   // cells per replica (LDS counting sort)
 constexpr int kBlock = 256;
 constexpr int kExcl = 8;          // bonded and 1-3 partners per particle
+constexpr int kMaxSlots = 8;      // words of a replica's largest-displacement maximum
 
 // Static parameters of one replica (read-only on the device).
 struct RepParams {
@@ -40,7 +41,7 @@ struct RepParams {
 struct RepState {
     long long klok;               // step about to be executed
     double progress;              // PROGRESS, fast.cpp:1768-1775
-    unsigned long long maxd2;     // bits of max |x - x_old|^2 of the current step
+    unsigned long long maxd2[8];  // bits of max |x - x_old|^2 of the current step, in kMaxSlots words
     int need_rebuild;
     int lr_due;
     long long n_rebuilds;

@@ -1558,6 +1558,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         D.rs[p].e[EI_E2_ST] = s2;
         D.rs[p].e[EI_E3_ST] = s3;
     }
+    double d2 = 0.0;   // squared displacement of this thread's particle
+    int rp = -1;       // its replica
     if (p < D.n_tot) {
         const int r = D.prep[p];
         const RepParams &P = D.rp[r];
@@ -1578,23 +1580,27 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
             }
         }
         const double dx = xn[0] - on[0], dy = xn[1] - on[1], dz = xn[2] - on[2];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        // max over the wavefront first when it lies in one replica (d2 >= 0, so its
-        // bits order like the values), then one atomic per wavefront
-        const int r0 = __builtin_amdgcn_readfirstlane(r);
-        if (__ballot(r == r0) == __ballot(true)) {
-            double m = d2;
-            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-            if ((threadIdx.x & 63) == __ffsll((long long)__ballot(true)) - 1)
-                atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(m));
-        } else {
-            atomicMax(&S.maxd2, (unsigned long long)__double_as_longlong(d2));
-        }
+        d2 = dx * dx + dy * dy + dz * dz;
+        rp = r;
         // all reads of the old state of this step happen before any write: the
-        // new state goes to the second buffer set, swapped by the host
+        // new state goes to the second buffer set, copied back by k_commit
         D.pos[D.n_tot + p] = make_double4(xn[0], xn[1], xn[2], 0.0);
         D.old[D.n_tot + p] = make_double4(on[0], on[1], on[2], 0.0);
         D.vel[D.n_tot + p] = make_double4(vn[0], vn[1], vn[2], 0.0);
+    }
+    // largest displacement of the step per replica (d2 >= 0, so its bits order
+    // like the values): a wavefront maximum when the wavefront lies in one
+    // replica, then one atomic per wavefront into one of kMaxSlots words of the
+    // replica (by wavefront index, so no one word takes every wavefront's atomic)
+    const int r0 = __builtin_amdgcn_readfirstlane(rp);
+    const int slot = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kMaxSlots;
+    if (__ballot(rp == r0) == ~0ull) {
+        double m = d2;
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0 && r0 >= 0)
+            atomicMax(&D.rs[r0].maxd2[slot], (unsigned long long)__double_as_longlong(m));
+    } else if (rp >= 0) {
+        atomicMax(&D.rs[rp].maxd2[slot], (unsigned long long)__double_as_longlong(d2));
     }
 }
 
@@ -1609,11 +1615,15 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
         for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
             RepState &S = D.rs[r];
             const RepParams &P = D.rp[r];
-            const double m = sqrt(__longlong_as_double((long long)S.maxd2));
+            unsigned long long mb = 0;
+            for (int q = 0; q < kMaxSlots; ++q) {
+                mb = mb > S.maxd2[q] ? mb : S.maxd2[q];
+                S.maxd2[q] = 0;
+            }
+            const double m = sqrt(__longlong_as_double((long long)mb));
             S.progress += m + m;
             S.need_rebuild = S.progress > P.skin;
             if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
-            S.maxd2 = 0;
             S.klok += 1;
             S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
         }
