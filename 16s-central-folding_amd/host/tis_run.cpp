@@ -17,6 +17,14 @@
 //   -L<box A>  -N<steps, 3e9>  -W<output interval, 1e6>  -P<64|32 force precision>
 //   -R<philox|glibc>  -G (run the second deck of the _DD variant)  -I<input dir>
 //   -S<Philox seed>  -B<steps per device call, 10000>
+//   -O<sample interval>  equilibrium observables accumulated on the device every
+//            that many steps and written at each output step as means over the
+//            window since the previous one (the reference's offline analysis,
+//            analysis/mg_correlation and analysis/shape, without frames):
+//     Observables_16SCD_EPS_Mg_Cl_K_D..._s<seed>.dat  "klok samples Rg Rg^2 D S
+//            bound_Mg bound_Cl bound_K E2_HB E3_HB E2_STACK E3_STACK"
+//     MgBound_16SCD_EPS_Mg_Cl_K_D..._s<seed>.dat      per phosphate, the fraction of
+//            samples with a Mg ion within r_kT = 7.2684 A ("%.4f ", chain order)
 // The slot x[0] of each Coordinates frame is written as 0 (the reference leaves it
 // uninitialised); partial trailing frames are rejected on restart.
 #include <algorithm>
@@ -45,6 +53,7 @@ struct Args {
     std::string input_dir = ".";
     unsigned long long philox_seed = 0;
     long long chunk = 10000;
+    long long observe_every = 0;
 };
 
 std::string tag(const Args &a, const char *seed_prefix) {
@@ -53,6 +62,8 @@ std::string tag(const Args &a, const char *seed_prefix) {
                   a.c_k, seed_prefix, a.seed);
     return b;
 }
+
+constexpr double kRkT = 7.2684;   // P-ion contact distance of the reference's analysis
 
 int die(const char *what, int rc) {
     std::fprintf(stderr, "tis_run: %s failed (%d): %s\n", what, rc, tis_last_error());
@@ -81,12 +92,11 @@ struct StreamFeeder {
 };
 
 struct Output {
-    FILE *contacts = nullptr, *coords = nullptr, *energies = nullptr;
+    FILE *contacts = nullptr, *coords = nullptr, *energies = nullptr, *obs = nullptr, *mgb = nullptr;
     std::string n_contacts, n_coords, n_energies;
     ~Output() {
-        if (contacts) std::fclose(contacts);
-        if (coords) std::fclose(coords);
-        if (energies) std::fclose(energies);
+        for (FILE *f : {contacts, coords, energies, obs, mgb})
+            if (f) std::fclose(f);
     }
 };
 
@@ -121,6 +131,7 @@ int main(int argc, char **argv) {
             case 'I': a.input_dir = v; break;
             case 'S': a.philox_seed = std::strtoull(v, nullptr, 10); break;
             case 'B': a.chunk = std::max(1LL, std::atoll(v)); break;
+            case 'O': a.observe_every = std::max(0LL, std::atoll(v)); break;
             default: std::printf("unrecognized argument %s\n", argv[i]); break;
         }
     }
@@ -179,6 +190,11 @@ int main(int argc, char **argv) {
     std::vector<double> frame_buf(3 * (size_t)(N + 1));
     StreamFeeder feeder((unsigned)(a.T * 1000 * a.seed), draws);
     const int n_st3 = sys->n_st3;
+    const int n_beads = sys->n_beads;
+    std::vector<double> per_bead(4 * (size_t)n_beads);
+    std::vector<int> p_beads;   // phosphate beads in chain order
+    for (int i = 0; i < n_beads; ++i)
+        if (sys->cls[i] == 0) p_beads.push_back(i);
 
     auto run_deck = [&](const char *seed_prefix, long long klok_first) -> int {
         const std::string t = tag(a, seed_prefix);
@@ -195,6 +211,15 @@ int main(int argc, char **argv) {
         }
         int r = tis_set_step_counter(ctx, 0, klok_first);
         if (r) return die("tis_set_step_counter", r);
+        if (a.observe_every > 0) {
+            o.obs = std::fopen(("Observables_16SCD_EPS_Mg_Cl_K_" + t + ".dat").c_str(), "a");
+            o.mgb = std::fopen(("MgBound_16SCD_EPS_Mg_Cl_K_" + t + ".dat").c_str(), "a");
+            if (!o.obs || !o.mgb) {
+                std::fprintf(stderr, "tis_run: cannot open the observables files\n");
+                return 1;
+            }
+            if ((r = tis_observe(ctx, a.observe_every, kRkT))) return die("tis_observe", r);
+        }
         long long klok = klok_first;
         while (klok <= a.nsteps) {
             if (klok % a.write_every == 0) {
@@ -217,6 +242,21 @@ int main(int argc, char **argv) {
                 std::fflush(o.coords);
                 std::fprintf(o.energies, "%le %le %le %le %le\n", e.e2_hb, e.e3_hb, e.e2_stack, e3, e.rg);
                 std::fflush(o.energies);
+                if (o.obs) {
+                    tis_observables ob;
+                    if ((r = tis_get_observables(ctx, 0, &ob, per_bead.data(), n_beads, nullptr, 0, nullptr, 0)))
+                        return die("tis_get_observables", r);
+                    if (ob.n_samples > 0) {
+                        std::fprintf(o.obs, "%lld %lld %le %le %le %le %le %le %le %le %le %le %le\n", klok,
+                                     ob.n_samples, ob.rg, ob.rg2, ob.asphericity, ob.shape, ob.bound_mg, ob.bound_cl,
+                                     ob.bound_k, ob.e2_hb, ob.e3_hb, ob.e2_stack, ob.e3_stack);
+                        for (int b : p_beads) std::fprintf(o.mgb, "%.4f ", per_bead[4 * (size_t)b + 3]);
+                        std::fprintf(o.mgb, "\n");
+                        std::fflush(o.obs);
+                        std::fflush(o.mgb);
+                    }
+                    if ((r = tis_observe(ctx, a.observe_every, kRkT))) return die("tis_observe", r);   // next window
+                }
                 if (klok % (5 * a.write_every) == 0 || klok == a.nsteps) {
                     const long long idx = klok / (5 * a.write_every);
                     const bool last = klok == a.nsteps;

@@ -85,3 +85,11 @@ def test_null_context_calls_fail_cleanly(lib):
     e = T.Energies()
     assert lib.tis_get_energies(None, 0, C.byref(e)) == -1
     lib.tis_destroy(None)
+
+
+def test_observables_entry_points_reject_bad_arguments_without_a_device(lib):
+    o = T.Observables()
+    assert lib.tis_observe(None, 10, 7.0) == -1
+    assert lib.tis_observe_now(None) == -1
+    assert lib.tis_get_observables(None, 0, C.byref(o), None, 0, None, 0, None, 0) == -1
+    assert lib.tis_autotune(None, 1, None, 100, None) == -1
