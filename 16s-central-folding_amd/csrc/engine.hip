@@ -686,6 +686,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.hb_iter_part = dalloc<int>(own, (size_t)kMaxHbGroups * n_rep);
         D.act_hb = dalloc<int>(own, nhb_tot);
         D.act_idx = dalloc<int>(own, nhb_tot);
+        D.act_tmp = dalloc<int>(own, nhb_tot);
         D.act_E = dalloc<double>(own, nhb_tot);
         D.act_flag = dalloc<unsigned char>(own, nhb_tot);
         D.asite_id = dalloc<int>(own, nsite_tot);

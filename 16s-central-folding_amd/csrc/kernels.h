@@ -368,7 +368,8 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D) {
     // chunk offsets
     int tot;
     const int rank = block_excl_sum(live, scratch, &tot);
-    if (k < P.nhb) D.act_idx[P.hb0 + k] = live ? rank : -1;
+    if (k < P.nhb) D.act_idx[P.hb0 + k] = -1;
+    if (live) D.act_tmp[P.hb0 + blockIdx.x * kStatusBlock + rank] = P.hb0 + k;   // the chunk's live ids, packed
     if (threadIdx.x == 0) D.chunk_cnt[kMaxHbChunks * r + blockIdx.x] = tot;
 }
 
@@ -399,23 +400,18 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
     __syncthreads();
     const int n_act = chunk_off[kMaxHbChunks];
     if (n_act <= kActMax) {
-        // eight candidates per thread per round, their ranks loaded before any store
-        constexpr int U = 8;
-        for (int k0 = threadIdx.x; k0 < P.nhb; k0 += U * blockDim.x) {
-            int lr[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int k = k0 + u * blockDim.x;
-                lr[u] = k < P.nhb ? D.act_idx[P.hb0 + k] : -1;
+        // live index a lies in chunk c with chunk_off[c] <= a < chunk_off[c + 1]
+        // (binary search; empty chunks share their offset with the next one)
+        for (int a = threadIdx.x; a < n_act; a += blockDim.x) {
+            int lo = 0, hi = nchunk;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (chunk_off[mid] <= a) lo = mid;
+                else hi = mid;
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (lr[u] < 0) continue;
-                const int k = k0 + u * blockDim.x, g = P.hb0 + k;
-                const int a = chunk_off[k / kStatusBlock] + lr[u];
-                D.act_idx[g] = a;
-                D.act_hb[P.hb0 + a] = g;
-            }
+            const int g = D.act_tmp[P.hb0 + lo * kStatusBlock + (a - chunk_off[lo])];
+            D.act_idx[g] = a;
+            D.act_hb[P.hb0 + a] = g;
         }
     }
     __syncthreads();
@@ -424,32 +420,35 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         if (threadIdx.x == 0) atomicOr(&S.overflow, 2);
         return;
     }
-    int n_as = 0, n_inc = 0;
-    for (int s0 = 0; s0 < P.nsite; s0 += blockDim.x) {
-        const int s = s0 + threadIdx.x;
-        const int cnt = s < P.nsite ? D.site_cnt[P.site0 + s] : 0;
-        if (s < P.nsite) D.site_cnt[P.site0 + s] = 0;   // cleared for the next build
-        // one scan for both prefixes: active-site flags in the low 12 bits (at most
-        // 1024 per round), incidence counts above them (a replica has fewer than
-        // 2^19 site incidences, checked at creation)
-        int tot;
-        const int packed = block_excl_sum((cnt > 0 ? 1 : 0) | (cnt << 12), scratch, &tot);
-        const int rank = packed & 0xfff, off = packed >> 12, tot_s = tot & 0xfff, tot_i = tot >> 12;
-        if (s < P.nsite) {
-            fill[s] = 0;
-            if (cnt > 0) {
-                site_map[s] = n_as + rank;
-                if (n_as + rank < kASiteMax) {
-                    D.asite_id[P.site0 + n_as + rank] = P.site0 + s;
-                    D.asite_off[P.site0 + r + n_as + rank] = n_inc + off;
-                    aoff_l[n_as + rank] = n_inc + off;
-                }
-            } else {
-                site_map[s] = -1;
+    // active sites in site order: each thread a contiguous range of sites, one
+    // scan each for the active-site ranks and the incidence offsets
+    const int per = (P.nsite + blockDim.x - 1) / blockDim.x;
+    const int s_lo = min(P.nsite, (int)threadIdx.x * per), s_hi = min(P.nsite, s_lo + per);
+    int my_as = 0, my_inc = 0;
+    for (int s = s_lo; s < s_hi; ++s) {
+        const int c = D.site_cnt[P.site0 + s];
+        my_as += c > 0;
+        my_inc += c;
+    }
+    int n_as, n_inc;
+    int rank = block_excl_sum(my_as, scratch, &n_as);
+    int off = block_excl_sum(my_inc, scratch, &n_inc);
+    for (int s = s_lo; s < s_hi; ++s) {
+        const int cnt = D.site_cnt[P.site0 + s];
+        D.site_cnt[P.site0 + s] = 0;   // cleared for the next build
+        fill[s] = 0;
+        if (cnt > 0) {
+            site_map[s] = rank;
+            if (rank < kASiteMax) {
+                D.asite_id[P.site0 + rank] = P.site0 + s;
+                D.asite_off[P.site0 + r + rank] = off;
+                aoff_l[rank] = off;
             }
+            ++rank;
+            off += cnt;
+        } else {
+            site_map[s] = -1;
         }
-        n_as += tot_s;
-        n_inc += tot_i;
     }
     if (threadIdx.x == 0 && n_as <= kASiteMax) {
         D.asite_off[P.site0 + r + n_as] = n_inc;
