@@ -17,16 +17,17 @@
 namespace tis {
 
 // v = F - F^target per particle into D.jv, and sum |v|^2 into *loss.
-__global__ void __launch_bounds__(kBlock) k_fm_residual(Dev D, const double4 *target, double4 *forces,
+__global__ void __launch_bounds__(kBlock) k_fm_residual(Dev D, const double *target, double4 *forces,
                                                           double *loss) {
     __shared__ double red[8];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (p < D.n_tot) {
-        const double4 a = D.f_sr[p], b = D.f_lr[p], c = D.f_mb[p], t = target[p];
+        const double4 a = D.f_sr[p], b = D.f_lr[p], c = D.f_mb[p];
+        const double tx = target[3 * (size_t)p], ty = target[3 * (size_t)p + 1], tz = target[3 * (size_t)p + 2];
         const double fx = a.x + b.x + c.x, fy = a.y + b.y + c.y, fz = a.z + b.z + c.z;
         forces[p] = make_double4(fx, fy, fz, 0.0);
-        const double vx = fx - t.x, vy = fy - t.y, vz = fz - t.z;
+        const double vx = fx - tx, vy = fy - ty, vz = fz - tz;
         D.jv[p] = make_double4(vx, vy, vz, 0.0);
         s = vx * vx + vy * vy + vz * vz;
     }
@@ -177,6 +178,41 @@ __global__ void __launch_bounds__(kBlock) k_jvp_hb(Dev D, double *g_hb) {
     double vd = 0.0;
     sextet_forces<Dual>(D, D.jf, D.hb_b + 6 * g, D.hb_g + 6 * g, -e, psi0, Dual(5.0), Dual(1.5), Dual(0.15), &vd);
     g_hb[g] = vd / D.hb_E[g];
+}
+
+}  // namespace tis
+
+namespace tis {
+
+// xyz of n double4 values, times `scale`, into a packed double[3 n] (the layout
+// the C ABI hands back).
+__global__ void __launch_bounds__(kBlock) k_pack3(const double4 *src, int n, double scale, double *dst) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double4 v = src[p];
+    dst[3 * (size_t)p] = scale * v.x;
+    dst[3 * (size_t)p + 1] = scale * v.y;
+    dst[3 * (size_t)p + 2] = scale * v.z;
+}
+
+// dst[k] = scale * sum_r src[r * n + k]: per-term gradients of replicas of one
+// system, summed over the replicas on the device.
+__global__ void __launch_bounds__(kBlock) k_sum_replicas(const double *src, int n, int n_rep, double scale,
+                                                          double *dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double s = 0.0;
+    for (int r = 0; r < n_rep; ++r) s += src[(size_t)r * n + k];
+    dst[k] = scale * s;
+}
+
+// Parameters of replica 0's region copied to every replica (replicas of one
+// system): dst[(r * n + k) * stride + at] = src[k].
+__global__ void __launch_bounds__(kBlock) k_set_param(double *dst, int n, int n_rep, int stride, int at,
+                                                       const double *src) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * n_rep) return;
+    dst[i * stride + at] = src[i % n];
 }
 
 }  // namespace tis

@@ -92,10 +92,11 @@ struct tis_ctx {
     long long obs_every = 0;
     double obs_cut = 7.2684;
     // force field as uploaded (for tis_set_force_field) and force-matching buffers
-    std::vector<double> h_hbE, h_st3E;
+    std::vector<double> h_hbE, h_st3E;   // as uploaded at creation (sizes: all replicas' terms)
     std::vector<double2> h_st2rE;
     std::vector<std::vector<double>> type_R, type_eps, type_mass, type_q;
-    double4 *fm_target = nullptr, *fm_forces = nullptr;
+    double4 *fm_forces = nullptr;
+    double *fm_t3 = nullptr, *fm_out3 = nullptr, *fm_small = nullptr;   // packed target / outputs, parameter staging
     double *fm_loss = nullptr, *fm_ion = nullptr, *fm_ghb = nullptr, *fm_gst3 = nullptr, *fm_gst2 = nullptr;
 
     ~tis_ctx() {
@@ -732,8 +733,10 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         // ---- force matching: tangents, target, outputs
         D.jv = dalloc<double4>(own, N);
         D.jf = dalloc<double4>(own, N);
-        c->fm_target = dalloc<double4>(own, N);
         c->fm_forces = dalloc<double4>(own, N);
+        c->fm_t3 = dalloc<double>(own, 3 * (size_t)N);
+        c->fm_out3 = dalloc<double>(own, 3 * (size_t)N);
+        c->fm_small = dalloc<double>(own, std::max<size_t>({1, c->h_hbE.size(), c->h_st3E.size(), c->h_st2rE.size()}));
         c->fm_loss = dalloc<double>(own, 1);
         c->fm_ion = dalloc<double>(own, 3);
         c->fm_ghb = dalloc<double>(own, std::max<size_t>(1, c->h_hbE.size()));
@@ -1005,20 +1008,27 @@ extern "C" int tis_set_force_field(tis_ctx *c, const double *hb_E, const double 
     return guarded([&]() -> int {
         HIP_TRY(hipStreamSynchronize(c->s_main));
         Dev &D = c->D;
-        for (int r = 0; r < c->n_rep; ++r) {
-            const RepParams &P = c->rp[r];
-            if (hb_E) for (int k = 0; k < P.nhb; ++k) c->h_hbE[P.hb0 + k] = hb_E[k];
-            if (st3_E) for (int k = 0; k < P.nst3; ++k) c->h_st3E[P.st30 + k] = st3_E[k];
-            if (st2_E) for (int k = 0; k < P.nst2; ++k) c->h_st2rE[P.st20 + k].y = st2_E[k];
-            if (ion_R) {
+        // per-term sets: replica 0's values uploaded once and copied to every
+        // replica's region on the device (the replicas describe one system)
+        auto set = [&](double *dst, int n, int stride, int at, const double *src) {
+            if (!src || n <= 0) return;
+            HIP_TRY(hipMemcpyAsync(c->fm_small, src, n * sizeof(double), hipMemcpyHostToDevice, c->s_main));
+            const long long tot = (long long)n * c->n_rep;
+            k_set_param<<<(int)((tot + kBlock - 1) / kBlock), kBlock, 0, c->s_main>>>(dst, n, c->n_rep, stride, at,
+                                                                                     c->fm_small);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(c->s_main));
+        };
+        const RepParams &P0 = c->rp[0];
+        set(D.hb_E, P0.nhb, 1, 0, hb_E);
+        set(D.st3_E, P0.nst3, 1, 0, st3_E);
+        set((double *)D.st2_rE, P0.nst2, 2, 1, st2_E);
+        if (ion_R)
+            for (int r = 0; r < c->n_rep; ++r) {
                 for (int t = 0; t < 3; ++t) c->type_R[r][6 + t] = ion_R[t];
                 type_tables(c->rp[r], c->type_R[r].data(), c->type_eps[r].data(), c->type_q[r].data(),
-                            c->type_mass[r].data(), P.lB);
+                            c->type_mass[r].data(), c->rp[r].lB);
             }
-        }
-        if (hb_E) upload(D.hb_E, c->h_hbE);
-        if (st3_E) upload(D.st3_E, c->h_st3E);
-        if (st2_E) upload(D.st2_rE, c->h_st2rE);
         if (ion_R) HIP_TRY(hipMemcpy(D.rp, c->rp.data(), c->n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
         return TIS_OK;
     });
@@ -1034,9 +1044,7 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
         const int N = D.n_tot;
         hipStream_t st = c->s_main;
         HIP_TRY(hipStreamSynchronize(st));
-        std::vector<double4> t4(N);
-        for (int p = 0; p < N; ++p) t4[p] = make_double4(target[3 * p], target[3 * p + 1], target[3 * p + 2], 0.0);
-        HIP_TRY(hipMemcpy(c->fm_target, t4.data(), N * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(c->fm_t3, target, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(c->fm_loss, 0, sizeof(double), st));
         HIP_TRY(hipMemsetAsync(c->fm_ion, 0, 3 * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(c->fm_ghb, 0, std::max<size_t>(1, c->h_hbE.size()) * sizeof(double), st));
@@ -1051,7 +1059,7 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
             k_force<float, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
             k_force<float, 1><<<npr, c->pair_block, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
         }
-        k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_target, c->fm_forces, c->fm_loss);
+        k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_t3, c->fm_forces, c->fm_loss);
         // backward: force tangents along v = F - target, parameter sums
         k_jvp_pairs<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_ion);
         k_jvp_multibody<<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, c->fm_gst3, c->fm_gst2);
@@ -1064,28 +1072,29 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
         double l = 0;
         HIP_TRY(hipMemcpy(&l, c->fm_loss, sizeof(double), hipMemcpyDeviceToHost));
         *loss = l / M;
+        // device results packed on the context's stream, then copied on that stream
+        // (the stream does not synchronise with the null stream)
+        auto fetch = [&](double *out, const double *dev, size_t n) {
+            HIP_TRY(hipMemcpyAsync(out, dev, n * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        };
         if (forces) {
-            std::vector<double4> f(N);
-            HIP_TRY(hipMemcpy(f.data(), c->fm_forces, N * sizeof(double4), hipMemcpyDeviceToHost));
-            for (int p = 0; p < N; ++p) { forces[3 * p] = f[p].x; forces[3 * p + 1] = f[p].y; forces[3 * p + 2] = f[p].z; }
+            k_pack3<<<c->grid(N), kBlock, 0, st>>>(c->fm_forces, N, 1.0, c->fm_out3);
+            fetch(forces, c->fm_out3, 3 * (size_t)N);
         }
         if (grad_pos) {
-            std::vector<double4> g(N);
-            HIP_TRY(hipMemcpy(g.data(), D.jf, N * sizeof(double4), hipMemcpyDeviceToHost));
-            for (int p = 0; p < N; ++p) {
-                grad_pos[3 * p] = sc * g[p].x; grad_pos[3 * p + 1] = sc * g[p].y; grad_pos[3 * p + 2] = sc * g[p].z;
-            }
+            k_pack3<<<c->grid(N), kBlock, 0, st>>>(D.jf, N, sc, c->fm_out3);
+            fetch(grad_pos, c->fm_out3, 3 * (size_t)N);
         }
-        auto summed = [&](const double *dev, size_t total, int n_local, int RepParams::*off, double *out) {
-            std::vector<double> h(std::max<size_t>(1, total));
-            HIP_TRY(hipMemcpy(h.data(), dev, std::max<size_t>(1, total) * sizeof(double), hipMemcpyDeviceToHost));
-            for (int k = 0; k < n_local; ++k) out[k] = 0.0;
-            for (int r = 0; r < c->n_rep; ++r)
-                for (int k = 0; k < n_local; ++k) out[k] += sc * h[c->rp[r].*off + k];
+        // per-term gradients: summed over the replicas (one system) on the device
+        auto summed = [&](const double *dev, int n_local, double *out) {
+            if (n_local <= 0) return;
+            k_sum_replicas<<<(n_local + kBlock - 1) / kBlock, kBlock, 0, st>>>(dev, n_local, c->n_rep, sc, c->fm_small);
+            fetch(out, c->fm_small, (size_t)n_local);
         };
-        if (grad_hb_E) summed(c->fm_ghb, c->h_hbE.size(), c->rp[0].nhb, &RepParams::hb0, grad_hb_E);
-        if (grad_st3_E) summed(c->fm_gst3, c->h_st3E.size(), c->rp[0].nst3, &RepParams::st30, grad_st3_E);
-        if (grad_st2_E) summed(c->fm_gst2, c->h_st2rE.size(), c->rp[0].nst2, &RepParams::st20, grad_st2_E);
+        if (grad_hb_E) summed(c->fm_ghb, c->rp[0].nhb, grad_hb_E);
+        if (grad_st3_E) summed(c->fm_gst3, c->rp[0].nst3, grad_st3_E);
+        if (grad_st2_E) summed(c->fm_gst2, c->rp[0].nst2, grad_st2_E);
         if (grad_ion_R) {
             double g3[3];
             HIP_TRY(hipMemcpy(g3, c->fm_ion, 3 * sizeof(double), hipMemcpyDeviceToHost));
