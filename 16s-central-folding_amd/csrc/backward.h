@@ -40,10 +40,15 @@ __global__ void __launch_bounds__(kBlock) k_fm_residual(Dev D, const double *tar
 // afterwards).  Also sums, for the radii of Mg, Cl and K (types 6, 7, 8),
 // sum_pairs (v_i - v_j) . d_ij df/dD_LJ  (counted once per ordered pair from
 // each side: the i side adds v_i . d_ij df/dD).
+// kJvpLanes lanes per particle, each a strided share of its neighbours, then a
+// butterfly reduction of the tangent force inside the group.
+constexpr int kJvpLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_jvp_pairs(Dev D, double *grad_ion_r) {
     __shared__ double red[8];
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) / kJvpLanes;
+    const int sub = threadIdx.x % kJvpLanes;
     double gr[3] = {0.0, 0.0, 0.0};
+    double tf[3] = {0.0, 0.0, 0.0};
     if (p < D.n_tot) {
         const int r = D.prep[p];
         const RepParams &P = D.rp[r];
@@ -53,7 +58,7 @@ __global__ void __launch_bounds__(kBlock) k_jvp_pairs(Dev D, double *grad_ion_r)
         const int *nb = D.pool + P.pool0 + D.nb_start[p];
         const int nsr = D.nb_nsr[p], nlr = D.nb_nlr[p];
         Dual fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int q = 0; q < nsr + nlr; ++q) {
+        for (int q = sub; q < nsr + nlr; q += kJvpLanes) {
             const bool is_sr = q < nsr;
             const int e = is_sr ? nb[q] : nb[P.cap_sr + q - nsr];
             const bool bonded = e < 0;
@@ -90,8 +95,14 @@ __global__ void __launch_bounds__(kBlock) k_jvp_pairs(Dev D, double *grad_ion_r)
             fy += f * dy;
             fz += f * dz;
         }
-        D.jf[p] = make_double4(fx.d, fy.d, fz.d, 0.0);
+        tf[0] = fx.d;
+        tf[1] = fy.d;
+        tf[2] = fz.d;
     }
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+        for (int o = 1; o < kJvpLanes; o <<= 1) tf[t] += __shfl_xor(tf[t], o, 64);
+    if (p < D.n_tot && sub == 0) D.jf[p] = make_double4(tf[0], tf[1], tf[2], 0.0);
     for (int t = 0; t < 3; ++t) {
         const double s = block_sum_double(gr[t], red);
         if (threadIdx.x == 0 && s != 0.0) unsafeAtomicAdd(&grad_ion_r[t], s);

@@ -1061,7 +1061,7 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
         }
         k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_t3, c->fm_forces, c->fm_loss);
         // backward: force tangents along v = F - target, parameter sums
-        k_jvp_pairs<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_ion);
+        k_jvp_pairs<<<c->grid(kJvpLanes * N), kBlock, 0, st>>>(D, c->fm_ion);
         k_jvp_multibody<<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, c->fm_gst3, c->fm_gst2);
         k_jvp_hb<<<dim3((kActMax + kBlock - 1) / kBlock, c->n_rep), kBlock, 0, st>>>(D, c->fm_ghb);
         k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
