@@ -321,6 +321,44 @@ def run_force_match(args, lib, n_snap=256):
     }))
 
 
+def run_remd(args, lib, n_rep=16, every=1000):
+    """Temperature replica exchange on the device: 16 copies of the 5 mM 16SCD
+    system (3,659 particles each) on a geometric ladder T_k = 0.596 * 1.25^(k/15)
+    kcal/mol (300-375 K), swap attempts between neighbours every 1,000 steps
+    inside tis_step (energies, swap, Metropolis decision, restore: no host round
+    trip)."""
+    temps = [0.596 * 1.25 ** (k / (n_rep - 1)) for k in range(n_rep)]
+    handles = []
+    for k, t in enumerate(temps):
+        d, _ = T.build_16scd(lib, T=t, c_mg=0.005, c_k=0.050, seed=1, box=350.0)
+        handles.append(T.SystemHandle(d))
+    launch, tuning = tune(lib, handles, args)
+    eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024, exchange_every=every,
+                   **launch)
+    eng.step(max(every, args.warmup))
+    a0, c0, _ = eng.exchange_stats()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    dt = time.perf_counter() - t0
+    att, acc, _ = eng.exchange_stats()
+    att, acc = att - a0, acc - c0
+    value = n_rep * args.steps / dt
+    eng.close()
+    print(json.dumps({
+        "metric": "Langevin steps/s, temperature replica exchange on the device (16 x 16SCD at 5 mM)",
+        "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": max(every, args.warmup),
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic: 16SCD chain from the reference PDB, ions placed by the reference's seeded rule",
+        "config": {"workload": "16 x 16SCD, 5 mM MgCl2 + 50 mM KCl, 350 A, T ladder 0.596 * 1.25^(k/15), "
+                               "exchange attempts every 1,000 steps", "replicas": n_rep,
+                   "precision": "mixed", "rng": "philox", "launch": launch, "autotune": tuning},
+        "exchange_attempts": int(att.sum()), "exchange_accepts": int(acc.sum()),
+        "acceptance_per_pair": [None if a == 0 else round(float(c) / float(a), 3) for a, c in zip(att, acc)],
+        "roofline": None, "cpu_baseline": None,
+    }))
+
+
 def tune(lib, handles, args):
     """Launch shapes for the timed run: tis_autotune's choice on a scratch copy of
     the batch (its trials are outside the timed region), or the defaults."""
@@ -343,7 +381,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched16", "chain1500", "forcematch256"], default="batched16")
+    ap.add_argument("--workload", choices=["batched16", "chain1500", "forcematch256", "remd16"], default="batched16")
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=250)
     ap.add_argument("--profile-steps", type=int, default=300)
@@ -371,6 +409,9 @@ def main():
         return
     if args.workload == "forcematch256":
         run_force_match(args, lib)
+        return
+    if args.workload == "remd16":
+        run_remd(args, lib)
         return
     systems = batched_systems(lib)
     log(f"built {len(systems)} replicas")
