@@ -495,7 +495,9 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             for (int a = 0; a < 3; ++a) {
                 P.L[a] = s.box[a];
                 P.hL[a] = 0.5 * s.box[a];
-                P.M[a] = std::max(1, (int)(s.box[a] / o.r_lr));
+                // cells at least r_lr / 2 wide (the list build searches +- 2 cells),
+                // at most 20 per axis (the LDS counting sort holds kMaxCells)
+                P.M[a] = std::min(20, std::max(1, (int)(2.0 * s.box[a] / o.r_lr)));
                 P.lc[a] = s.box[a] / P.M[a];
             }
             if ((long)P.M[0] * P.M[1] * P.M[2] > kMaxCells) throw Fail{TIS_EINVAL, "box too large for the cell grid"};

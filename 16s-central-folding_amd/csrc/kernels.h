@@ -203,27 +203,49 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
 }
 
 // ------------------------------------------------------------ neighbour cells
-// The <= 27 cells around `cell` (fewer when a dimension has < 3 cells), one per
-// lane: lane c < *ncell gets the cell's first cell-order index and length.
-TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ncell, int *cstart, int *clen) {
+// Cells are at least half the long-range list radius wide, so the particles
+// within r_lr of a particle in cell (c0, c1, c2) lie in cells c +- 2 along each
+// axis.  Cells are numbered with z fastest, so the five cells c2 - 2 .. c2 + 2 of
+// one (x, y) column are contiguous in cell order (two pieces where the range
+// wraps the box): the neighbourhood is at most 5 x 5 columns x 2 pieces = 50
+// contiguous ranges of particles, one per lane: lane l < *nseg gets its range's
+// first cell-order index and length (0 for an empty piece).  An axis with fewer
+// than 5 cells contributes each of its cells once.
+TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *nseg, int *sstart, int *slen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
-    const int lo0 = M0 >= 3 ? -1 : 0, hi0 = M0 >= 2 ? 1 : 0;
-    const int lo1 = M1 >= 3 ? -1 : 0, hi1 = M1 >= 2 ? 1 : 0;
-    const int lo2 = M2 >= 3 ? -1 : 0, hi2 = M2 >= 2 ? 1 : 0;
-    const int w0 = hi0 - lo0 + 1, w1 = hi1 - lo1 + 1, w2 = hi2 - lo2 + 1;
-    *ncell = w0 * w1 * w2;
+    const int wx = M0 < 5 ? M0 : 5, wy = M1 < 5 ? M1 : 5;
+    *nseg = 2 * wx * wy;
     const int lane = threadIdx.x & 63;
-    *cstart = 0;
-    *clen = 0;
-    if (lane < *ncell) {
-        const int a = lane / (w1 * w2) + lo0, b = (lane / w2) % w1 + lo1, c = lane % w2 + lo2;
-        int n0 = c0 + a; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0;
-        int n1 = c1 + b; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0;
-        int n2 = c2 + c; n2 += n2 < 0 ? M2 : 0; n2 -= n2 >= M2 ? M2 : 0;
-        const int nc = (n0 * M1 + n1) * M2 + n2;
-        *cstart = D.cell_start[P.cell0 + nc];
-        *clen = D.cell_start[P.cell0 + nc + 1] - *cstart;
+    *sstart = 0;
+    *slen = 0;
+    const int col = lane >> 1, piece = lane & 1;
+    if (col < wx * wy) {
+        const int ix = col / wy, iy = col % wy;
+        int n0 = ix, n1 = iy;
+        if (M0 >= 5) { n0 = c0 - 2 + ix; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0; }
+        if (M1 >= 5) { n1 = c1 - 2 + iy; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0; }
+        int zlo = 0, zhi = M2 - 1;           // this piece's z range (zhi < zlo: empty)
+        if (M2 >= 5) {
+            const int lo = c2 - 2, hi = c2 + 2;
+            if (lo < 0) {
+                zlo = piece ? lo + M2 : 0;
+                zhi = piece ? M2 - 1 : hi;
+            } else if (hi >= M2) {
+                zlo = piece ? 0 : lo;
+                zhi = piece ? hi - M2 : M2 - 1;
+            } else {
+                zlo = lo;
+                zhi = piece ? lo - 1 : hi;
+            }
+        } else if (piece) {
+            zhi = -1;
+        }
+        if (zhi >= zlo) {
+            const int base = P.cell0 + (n0 * M1 + n1) * M2;
+            *sstart = D.cell_start[base + zlo];
+            *slen = D.cell_start[base + zhi + 1] - *sstart;
+        }
     }
 }
 
