@@ -108,6 +108,14 @@ TIS_DEV R bond_coef(V3<R> d, R k, R r0, R *r_out) {
     return (R)2 * k * (r0 - r) / r;
 }
 
+// A cosine clamped to [-1, 1]: FP32 rounding carries the cosine of a straight
+// (or zero) angle just past -1 (or 1), where acos and sqrt(1 - c^2) are NaN;
+// inside [-1, 1] the value is unchanged.
+template <class R>
+TIS_DEV R unit_clamp(R c) {
+    return c < (R)-1 ? (R)-1 : (c > (R)1 ? (R)1 : c);
+}
+
 // Angle at b of (a, b, c): returns theta; g[0], g[2] = d cos(theta) / d x_a, x_c.
 template <class R>
 TIS_DEV R angle_grad(V3<R> ba, V3<R> bc, R *cos_out, V3<R> *g0, V3<R> *g2) {
@@ -129,8 +137,10 @@ TIS_DEV R dihedral_grad(V3<R> b1, V3<R> b2, V3<R> b3, V3<R> g[4]) {
     R b22 = dot(b2, b2);
     R lb2 = sqrt(b22);
     R psi = atan2(dot(b1, n) * lb2, dot(m, n));
-    V3<R> g1 = m * (-lb2 / dot(m, m));
-    V3<R> g4 = n * (lb2 / dot(n, n));
+    // collinear beads (m or n = 0) leave the angle undefined: no force from it
+    const R mm = dot(m, m), nn = dot(n, n);
+    V3<R> g1 = mm > (R)0 ? m * (-lb2 / mm) : m * (R)0;
+    V3<R> g4 = nn > (R)0 ? n * (lb2 / nn) : n * (R)0;
     R p12 = dot(b1, b2) / b22, p32 = dot(b3, b2) / b22;
     g[0] = g1;
     g[1] = g4 * p32 - g1 * ((R)1 + p12);

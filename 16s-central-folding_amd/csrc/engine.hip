@@ -310,6 +310,8 @@ struct tis_ctx {
             if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour slot overflow (more SR/LR partners than the slot set up at creation) in replica " + std::to_string(r)};
             if (s[r].overflow & 2) throw Fail{TIS_EOVERFLOW, "HB working set above the LDS caps in replica " + std::to_string(r)};
             if (s[r].overflow & 4) throw Fail{TIS_EOVERFLOW, "uniform stream exhausted in replica " + std::to_string(r)};
+            if (s[r].overflow & 32)
+                throw Fail{TIS_ENONFINITE, "non-finite coordinates in replica " + std::to_string(r)};
             if (s[r].overflow & 16)
                 throw Fail{TIS_EOVERFLOW, "HB valence resolution: a site has more live candidates than the kernel holds "
                                           "(64) in replica " + std::to_string(r)};
@@ -811,6 +813,9 @@ extern "C" int tis_set_state(tis_ctx *c, int rep, const double *pos, const doubl
     return guarded([&]() -> int {
         const RepParams &P = c->rp[rep];
         std::vector<double4> a(P.n), b(P.n), v(P.n);
+        for (int i = 0; i < 3 * P.n; ++i)
+            if (!std::isfinite(pos[i]) || !std::isfinite(pos_old[i]) || !std::isfinite(vel[i]))
+                throw Fail{TIS_EINVAL, "tis_set_state: non-finite coordinate or velocity"};
         for (int i = 0; i < P.n; ++i) {
             a[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0);
             b[i] = make_double4(pos_old[3 * i], pos_old[3 * i + 1], pos_old[3 * i + 2], 0);

@@ -566,7 +566,7 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
     V3<R> ga, gb;
     angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
     angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
-    R th1 = acos(c1), th2 = acos(c2);
+    R th1 = acos(unit_clamp(c1)), th2 = acos(unit_clamp(c2));
     R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
     R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
     R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
@@ -614,7 +614,9 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
         V3<R> g0, g2;
         angle_grad<R>(ba, bc, &c, &g0, &g2);
         if (c < (R)-0.99995) continue;
+        c = unit_clamp(c);
         R s = sqrt((R)1 - c * c);
+        if (!(s > (R)0)) continue;        // the reference's zero sine: no force
         R th = acos(c);
         R coef = (R)-2 * (kth * w) * ((R)geo[1 + a] - th) / s;
         V3<R> f0 = g0 * coef, f2 = g2 * coef;
@@ -1149,7 +1151,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __syncthreads();
     for (int a = tid; a < n_act; a += nt) {
         int owner = 0;
-        if (D.rng == 0 && flag[a] != FLAG_GATED) {
+        if (ngrp > 1 && D.rng == 0 && flag[a] != FLAG_GATED) {
             for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
                 const int ci = hinc[q];
                 if (ci >= 0) { owner = cidx[par[ci]] % ngrp; break; }
@@ -1231,7 +1233,7 @@ TIS_DEV R tertiary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
     V3<R> ga, gb;
     angle_grad<R>(diff<R>(p1, p0), v14, &c1, &ga, &gb);
     angle_grad<R>(diff<R>(p4, p3), diff<R>(p4, p1), &c2, &ga, &gb);
-    R th1 = acos(c1), th2 = acos(c2);
+    R th1 = acos(unit_clamp(c1)), th2 = acos(unit_clamp(c2));
     R ps = dihedral_only<R>(diff<R>(p0, p1), diff<R>(p1, p4), diff<R>(p4, p3));
     R ps1 = dihedral_only<R>(diff<R>(p4, p1), diff<R>(p1, p0), diff<R>(p0, p2));
     R ps2 = dihedral_only<R>(diff<R>(p1, p4), diff<R>(p4, p3), diff<R>(p3, p5));
@@ -1317,8 +1319,9 @@ TIS_DEV bool bonded_term(const Dev &D, double4 *F, int g, int nbond, int nang, i
         R c;
         V3<R> g0, g2;
         angle_grad<R>(ba, bc, &c, &g0, &g2);
+        c = unit_clamp(c);
         R sn = sqrt((R)1 - c * c), th = acos(c);
-        R coef = (R)-2 * (R)kt.x * ((R)kt.y - th) / sn;
+        R coef = sn > (R)0 ? (R)-2 * (R)kt.x * ((R)kt.y - th) / sn : (R)0;
         V3<R> f0 = g0 * coef, f2 = g2 * coef;
         atomic_add_force(F, i, f0);
         atomic_add_force(F, k, f2);
@@ -1642,6 +1645,9 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
                 S.maxd2[q] = 0;
             }
             const double m = sqrt(__longlong_as_double((long long)mb));
+            // a NaN or infinite coordinate (its bits order above every finite
+            // displacement) stops the run with an error instead of propagating
+            if (!(m < 1e30)) S.overflow |= 32;
             S.progress += m + m;
             S.need_rebuild = S.progress > P.skin;
             if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }

@@ -35,7 +35,8 @@ enum {
     TIS_EIO = -3,       /* input file missing or malformed */
     TIS_EDEVICE = -4,   /* HIP runtime error */
     TIS_EOVERFLOW = -5, /* a device list or random stream ran out of room */
-    TIS_ESTATE = -6     /* call out of order (e.g. step before set_state) */
+    TIS_ESTATE = -6,    /* call out of order (e.g. step before set_state) */
+    TIS_ENONFINITE = -7 /* a replica's coordinates became NaN or infinite (the run stops) */
 };
 const char *tis_last_error(void);
 

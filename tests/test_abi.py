@@ -52,7 +52,7 @@ def test_default_options_are_the_reference_settings(lib):
 
 @pytest.mark.parametrize("field,value", [("pair_lanes", 3), ("pair_lanes", 128), ("pair_block", 96),
                                          ("pair_block", 512), ("list_waves", 10), ("graph_steps", -1),
-                                         ("hb_groups", 0), ("hb_groups", 17)])
+                                         ("hb_groups", -1), ("hb_groups", 17)])
 def test_create_rejects_bad_launch_shapes_without_a_device(lib, cfg1, field, value):
     h = T.SystemHandle(cfg1[0])
     o = T.default_options(lib)

@@ -209,7 +209,24 @@ def test_graph_replays_match_eager_steps(lib, sys1):
     b.close()
 
 
-def test_component_resolution_has_the_sequential_distribution(lib, sys1):
+@pytest.fixture(scope="module")
+def sequential_hb(lib, sys1):
+    """Kept counts and resolved HB energies of 1500 resolutions of the reference's
+    sequential process (oracle, glibc draws) at the configuration of sys1."""
+    n = 1500
+    o = O.Oracle(sys1)
+    d = sys1.d
+    o.set_state(d["pos"], d["pos_old"], d["vel"])
+    kept_o, e_o = 0, []
+    for s in range(n):
+        o.set_stream(T.glibc_stream(lib, 5000 + s, 5000))
+        _, _, e = o.eval(T.TERM_HB)
+        kept_o = kept_o + (o.hb_outcome() == 1)
+        e_o.append(e["e2_hb"] + e["e3_hb"])
+    return kept_o, np.array(e_o), o.hb_iterations()
+
+
+def test_component_resolution_has_the_sequential_distribution(lib, sys1, sequential_hb):
     """Production HB valence resolution (independent components on separate
     wavefronts, Philox draws) against the reference's sequential process (oracle,
     glibc draws) on the same configuration: for every candidate, the frequency
@@ -220,37 +237,29 @@ def test_component_resolution_has_the_sequential_distribution(lib, sys1):
     well below the sequential iteration count."""
     n = 1500
     eng = T.Engine(lib, [sys1], precision=T.PREC_FP64, rng=T.RNG_PHILOX, seed=11)
-    kept_g, e_g = 0, []
+    kept_g, e_g, rounds = 0, [], []
     for s in range(n):
         eng.set_step_counter(1000 + s)
         _, e = eng.eval(T.TERM_HB)
         kept_g = kept_g + (eng.hb_outcome() == 1)
         e_g.append(e["e2_hb"] + e["e3_hb"])
-    rounds = eng.energies()["hb_rounds"]
+        if s < 50:
+            rounds.append(eng.energies()["hb_rounds"])
     eng.close()
-    o = O.Oracle(sys1)
-    d = sys1.d
-    o.set_state(d["pos"], d["pos_old"], d["vel"])
-    kept_o, e_o = 0, []
-    for s in range(n):
-        o.set_stream(T.glibc_stream(lib, 5000 + s, 5000))
-        _, _, e = o.eval(T.TERM_HB)
-        kept_o = kept_o + (o.hb_outcome() == 1)
-        e_o.append(e["e2_hb"] + e["e3_hb"])
-    iters = o.hb_iterations()
+    kept_o, e_o, iters = sequential_hb
     pg, po = kept_g / n, kept_o / n
     pp = (kept_g + kept_o) / (2 * n)
     var = pp * (1 - pp) * 2 / n
     z = np.where(var > 0, (pg - po) / np.sqrt(np.where(var > 0, var, 1)), 0.0)
-    e_g, e_o = np.array(e_g), np.array(e_o)
+    e_g = np.array(e_g)
     se = np.sqrt(e_g.var() / n + e_o.var() / n)
     contested = int(np.sum((pp > 0) & (pp < 1)))
     print("contested candidates", contested, "max |z|", np.abs(z).max(), "mean E gpu", e_g.mean(),
-          "ref", e_o.mean(), "se", se, "rounds", rounds, "sequential iterations", iters)
+          "ref", e_o.mean(), "se", se, "rounds", np.mean(rounds), max(rounds), "sequential iterations", iters)
     assert contested > 50
     assert np.abs(z).max() < 5.0
     assert abs(e_g.mean() - e_o.mean()) < 4 * se
-    assert rounds < iters / 2
+    assert max(rounds) < iters / 2
 
 
 def test_philox_resolution_is_deterministic(lib, sys1):
