@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_jvp_hb(Dev D, double *g_hb) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if ((S.overflow & 2) || a >= S.n_act) return;
     if (D.act_flag[P.hb0 + a] != FLAG_LIVE) return;
-    const int g = D.act_hb[P.hb0 + a];
+    const int g = hb_tab(D, S.tab).act_hb[P.hb0 + a];
     Dual q, psi0[3];
     hb_geometry<Dual>(D, g, &q, psi0, nullptr);
     const Dual e = Dual(D.hb_E[g]) * exp(-q);

@@ -60,7 +60,7 @@ struct tis_ctx {
     int n_rep = 0;
     tis_options opt{};
     hipStream_t s_main = nullptr, s_hb = nullptr, s_mb = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_mb = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_mb = nullptr, ev_join_rb = nullptr;
     std::vector<void *> owned;
     Dev D{};
     std::vector<RepParams> rp;
@@ -106,6 +106,7 @@ struct tis_ctx {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_join_mb) (void)hipEventDestroy(ev_join_mb);
+        if (ev_join_rb) (void)hipEventDestroy(ev_join_rb);
         if (s_main) (void)hipStreamDestroy(s_main);
         if (s_hb) (void)hipStreamDestroy(s_hb);
         if (s_mb) (void)hipStreamDestroy(s_mb);
@@ -157,13 +158,15 @@ struct tis_ctx {
         timed("k_bin", sl, [&] { k_bin<<<n_rep, 1024, 0, sl>>>(D); });
         timed("k_list", sl, [&] { k_list<<<list_grid(), kBlock, 0, sl>>>(D, 1); });
     }
-    void launch_rebuild_hb(hipStream_t sh) {
-        timed("k_hbstatus", sh, [&] { k_hbstatus<<<dim3(max_hb_chunks(), n_rep), kStatusBlock, 0, sh>>>(D); });
-        timed("k_hbsites", sh, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), sh>>>(D); });
+    // lag: inside a step, beside the resolution, into the table buffer it is not
+    // reading (switched by k_commit)
+    void launch_rebuild_hb(hipStream_t sh, int lag) {
+        timed("k_hbstatus", sh, [&] { k_hbstatus<<<dim3(max_hb_chunks(), n_rep), kStatusBlock, 0, sh>>>(D, lag); });
+        timed("k_hbsites", sh, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), sh>>>(D, lag); });
     }
     void launch_rebuild() {
         launch_rebuild_lists(s_main);
-        launch_rebuild_hb(s_main);
+        launch_rebuild_hb(s_main, 0);
     }
     // A step rebuilds at its start when the previous step's trigger fired; after
     // the last step of a call, do that rebuild now so that lists, HB statuses and
@@ -271,11 +274,16 @@ struct tis_ctx {
 
     // One Langevin step.  The HB resolution and the multi-body terms run on
     // their own streams, concurrently with the pair gather; the integrator joins all.
+    // With D.hb_lag the HB tables of a rebuild are built on the multi-body stream,
+    // beside the resolution, which reads the tables of the previous rebuild: every
+    // candidate within its gate (r <= r0 + 2) is listed in both, because
+    // r0 + 2 + (skin + the last step's displacement) < r_sr (checked at creation),
+    // so the live set, and with it the resolution, is the same.
     void launch_step() {
         HIP_TRY(hipEventRecord(ev_fork, s_main));
         HIP_TRY(hipStreamWaitEvent(s_hb, ev_fork, 0));
         HIP_TRY(hipStreamWaitEvent(s_mb, ev_fork, 0));
-        launch_rebuild_hb(s_hb);
+        if (!D.hb_lag) launch_rebuild_hb(s_hb, 0);
         launch_hb(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
@@ -284,6 +292,10 @@ struct tis_ctx {
             else k_force<float, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
+        if (D.hb_lag) {   // after the multi-body terms on their stream (the shorter chain)
+            launch_rebuild_hb(s_mb, 1);
+            HIP_TRY(hipEventRecord(ev_join_rb, s_mb));
+        }
         launch_rebuild_lists(s_main);
         timed("k_pairs", s_main, [&] {
             if (D.precision == TIS_PREC_FP64) k_force<double, 1><<<pair_grid(), pair_block, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
@@ -292,6 +304,7 @@ struct tis_ctx {
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
+        if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));
         timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
     }
 
@@ -413,6 +426,7 @@ static void check_launch_options(const tis_options &o) {
     if (w < 64 || w > 65536 || (w * 64) % kBlock) throw Fail{TIS_EINVAL, "list_waves must be 64..65536, a multiple of 4"};
     if (o.graph_steps < 0) throw Fail{TIS_EINVAL, "graph_steps"};
     if (o.hb_groups < 1 || o.hb_groups > kMaxHbGroups) throw Fail{TIS_EINVAL, "hb_groups must be 1..16"};
+    if (o.hb_table_lag != 0 && o.hb_table_lag != 1) throw Fail{TIS_EINVAL, "hb_table_lag must be 0 or 1"};
 }
 
 extern "C" void tis_default_options(tis_options *o) {
@@ -432,6 +446,7 @@ extern "C" void tis_default_options(tis_options *o) {
     o->pair_block = kBlock;
     o->list_waves = kListWaves;
     o->hb_groups = 4;
+    o->hb_table_lag = 1;
 }
 
 static int fail(const Fail &f) {
@@ -482,6 +497,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         HIP_TRY(hipStreamCreateWithFlags(&c->s_hb, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&c->s_mb, hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join_mb, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_join_rb, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         HIP_TRY(hipFuncSetAttribute((const void *)k_hbres<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
@@ -650,6 +666,13 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         c->h_st2rE = st2rE;
         D.hb_b = dalloc<int>(own, hbb.size()); upload(D.hb_b, hbb);
         D.hb_g = dalloc<double>(own, hbg.size()); upload(D.hb_g, hbg);
+        {   // HB tables rebuilt beside the resolution only where no candidate in its
+            // gate can be missing from the previous rebuild's list: r0 + 2 + skin
+            // plus 1 A for the displacement of the step that fires the trigger
+            double r0max = 0.0;
+            for (size_t k = 0; k < hbg.size(); k += 6) r0max = std::max(r0max, hbg[k]);
+            D.hb_lag = o.hb_table_lag && r0max + 2.0 + o.skin + 1.0 < o.r_sr;
+        }
         D.hb_E = dalloc<double>(own, hbE.size()); upload(D.hb_E, hbE);
         c->h_hbE = hbE;
         for (int r = 0; r < n_rep; ++r) {
@@ -689,16 +712,19 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
         D.hb_iter_part = dalloc<int>(own, (size_t)kMaxHbGroups * n_rep);
-        D.act_hb = dalloc<int>(own, nhb_tot);
+        D.tab_hb = (long long)nhb_tot + n_rep;
+        D.tab_site = (long long)nsite_tot + n_rep;
+        D.tab_inc = std::max<long long>(1, inc_base);
+        D.act_hb = dalloc<int>(own, 2 * (size_t)D.tab_hb);
         D.act_idx = dalloc<int>(own, nhb_tot);
         D.act_tmp = dalloc<int>(own, nhb_tot);
         D.act_E = dalloc<double>(own, nhb_tot);
         D.act_flag = dalloc<unsigned char>(own, nhb_tot);
-        D.asite_id = dalloc<int>(own, nsite_tot);
-        D.asite_off = dalloc<int>(own, nsite_tot + n_rep);
-        D.asite_list = dalloc<int>(own, inc_base);
-        D.ahb_inc_off = dalloc<int>(own, nhb_tot + n_rep);
-        D.ahb_inc = dalloc<int>(own, inc_base);
+        D.asite_id = dalloc<int>(own, 2 * (size_t)D.tab_site);
+        D.asite_off = dalloc<int>(own, 2 * (size_t)D.tab_site);
+        D.asite_list = dalloc<int>(own, 2 * (size_t)D.tab_inc);
+        D.ahb_inc_off = dalloc<int>(own, 2 * (size_t)D.tab_hb);
+        D.ahb_inc = dalloc<int>(own, 2 * (size_t)D.tab_inc);
         D.st_energy = dalloc<double>(own, st3E.size());
         D.st_part = dalloc<double>(own, 2 * (size_t)kMaxStSlices * n_rep);
         {   // one stack per thread in a step: slices of kBlock stacks
@@ -1369,8 +1395,12 @@ extern "C" int tis_get_hb_outcome(tis_ctx *c, int rep, int *out, int n) {
         HIP_TRY(hipStreamSynchronize(c->s_main));
         HIP_TRY(hipStreamSynchronize(c->s_hb));
         std::vector<signed char> v(P.nhb);
+        std::vector<signed char> st(P.nhb);
         if (P.nhb) HIP_TRY(hipMemcpy(v.data(), c->D.hb_out + P.hb0, P.nhb, hipMemcpyDeviceToHost));
-        for (int k = 0; k < P.nhb; ++k) out[k] = v[k];
+        if (P.nhb) HIP_TRY(hipMemcpy(st.data(), c->D.hb_status + P.hb0, P.nhb, hipMemcpyDeviceToHost));
+        // listed by the latest rebuild but not by the one the last resolution read
+        // (tables built beside it): outside its gate by the rule of launch_step
+        for (int k = 0; k < P.nhb; ++k) out[k] = st[k] < 0 ? -1 : (v[k] < 0 ? 0 : v[k]);
         return TIS_OK;
     });
 }

@@ -52,6 +52,11 @@ struct RepState {
     long long stream_cursor;
     long long hb_draws;           // Philox draws of the current step's HB resolution
     int hb_iters;
+    // HB rebuild tables: two buffers; `tab` is the one the resolution reads.  A
+    // rebuild inside a step writes the other one (counts in nx_*) and sets
+    // tab_pending; k_commit then switches (tab_fresh for one step after).
+    int tab, tab_pending, tab_fresh;
+    int nx_act, nx_asite, nx_ainc;
     int max_sr, max_lr;           // largest per-particle SR / LR counts of the last build
     int pad;
     double e[12];                 // energies of the last force evaluation
@@ -72,6 +77,8 @@ struct Dev {
     double noise_scale;
     int pair_lanes;               // lanes per particle in the pair gather
     int hb_groups;                // workgroups per replica in the HB resolution
+    int hb_lag;                   // HB tables rebuilt beside the resolution, used from the next step
+    long long tab_hb, tab_site, tab_inc;   // elements per buffer of the HB rebuild tables
     int st_slices;                // stack workgroups per replica in a step
     RepParams *rp;
     RepState *rs;

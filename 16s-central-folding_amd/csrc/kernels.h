@@ -362,6 +362,15 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 }
 
 // ------------------------------------------------------------ HB rebuild
+// One buffer of the HB rebuild tables (the arrays hold two, tab_* elements apart).
+struct HbTab {
+    int *act_hb, *asite_id, *asite_off, *asite_list, *ahb_inc_off, *ahb_inc;
+};
+TIS_DEV HbTab hb_tab(const Dev &D, int b) {
+    return {D.act_hb + b * D.tab_hb,      D.asite_id + b * D.tab_site,    D.asite_off + b * D.tab_site,
+            D.asite_list + b * D.tab_inc, D.ahb_inc_off + b * D.tab_hb, D.ahb_inc + b * D.tab_inc};
+}
+
 // k_hbstatus: grid-stride over the candidates of the replicas being rebuilt.
 // A candidate is live iff its (k11, k21) bead pair is an SR pair, decided with
 // the list's own FP64 distance test (populate_lists, fast.cpp:1702-1727).  Live
@@ -369,7 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 // after reading them.
 constexpr int kStatusBlock = 1024;
 constexpr int kMaxHbChunks = 64;   // candidates per replica <= kMaxHbChunks * kStatusBlock
-__global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D) {
+__global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
     const int r = blockIdx.y;
     if (!D.rs[r].need_rebuild) return;
     const RepParams &P = D.rp[r];
@@ -382,7 +391,7 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D) {
         const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
         live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
         D.hb_status[g] = live ? 0 : -1;
-        D.hb_out[g] = live ? 0 : -1;
+        if (!lag) D.hb_out[g] = live ? 0 : -1;   // lagged: the resolution running beside owns hb_out
         if (live)
             for (int q = D.hb_inc_off[g]; q < D.hb_inc_off[g + 1]; ++q) atomicAdd(&D.site_cnt[D.hb_inc[q]], 1);
     }
@@ -399,12 +408,14 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D) {
 // live candidate) in site order, each with its live candidates in the order of
 // the static site list HB_ATOM (restored from the precomputed rank of every
 // incidence), and every live candidate's incidences as active-site indices.
-// Dynamic LDS: 2 * nsite ints.
-__global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
+// Dynamic LDS: 2 * nsite ints.  lag: write the buffer the resolution is not
+// reading (it runs beside this kernel) and leave the switch to k_commit.
+__global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
     const int r = blockIdx.x;
     RepState &S = D.rs[r];
     if (!S.need_rebuild) return;
     const RepParams &P = D.rp[r];
+    const HbTab T = hb_tab(D, lag ? S.tab ^ 1 : S.tab);
     extern __shared__ int smem[];
     int *site_map = smem, *fill = smem + P.nsite;
     __shared__ int scratch[20];
@@ -433,7 +444,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
             }
             const int g = D.act_tmp[P.hb0 + lo * kStatusBlock + (a - chunk_off[lo])];
             D.act_idx[g] = a;
-            D.act_hb[P.hb0 + a] = g;
+            T.act_hb[P.hb0 + a] = g;
         }
     }
     __syncthreads();
@@ -462,8 +473,8 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         if (cnt > 0) {
             site_map[s] = rank;
             if (rank < kASiteMax) {
-                D.asite_id[P.site0 + rank] = P.site0 + s;
-                D.asite_off[P.site0 + r + rank] = off;
+                T.asite_id[P.site0 + rank] = P.site0 + s;
+                T.asite_off[P.site0 + r + rank] = off;
                 aoff_l[rank] = off;
             }
             ++rank;
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         }
     }
     if (threadIdx.x == 0 && n_as <= kASiteMax) {
-        D.asite_off[P.site0 + r + n_as] = n_inc;
+        T.asite_off[P.site0 + r + n_as] = n_inc;
         aoff_l[n_as] = n_inc;
     }
     __syncthreads();
@@ -487,24 +498,24 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         const int a = a0 + threadIdx.x;
         int c = 0, g = 0;
         if (a < n_act) {
-            g = D.act_hb[P.hb0 + a];
+            g = T.act_hb[P.hb0 + a];
             c = D.hb_inc_off[g + 1] - D.hb_inc_off[g];
         }
         int tot;
         const int off = block_excl_sum(c, scratch, &tot);
         if (a < n_act) {
-            D.ahb_inc_off[P.hb0 + r + a] = n_ai + off;
+            T.ahb_inc_off[P.hb0 + r + a] = n_ai + off;
             for (int q = 0; q < c; ++q) {
                 const int e = D.hb_inc_off[g] + q;
                 const int ai = site_map[D.hb_inc[e] - P.site0];
-                D.ahb_inc[P.inc0 + n_ai + off + q] = ai;
+                T.ahb_inc[P.inc0 + n_ai + off + q] = ai;
                 const int slot = atomicAdd(&fill[D.hb_inc[e] - P.site0], 1);
                 seg_l[aoff_l[ai] + slot] = (D.inc_rank[e] << 16) | a;
             }
         }
         n_ai += tot;
     }
-    if (threadIdx.x == 0) D.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
+    if (threadIdx.x == 0) T.ahb_inc_off[P.hb0 + r + n_act] = n_ai;
     __syncthreads();
     // order each site's segment by static rank (in LDS), keep the live index
     for (int ai = threadIdx.x; ai < n_as; ai += blockDim.x) {
@@ -518,11 +529,18 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D) {
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n_inc; i += blockDim.x) D.asite_list[P.inc0 + i] = seg_l[i] & 0xffff;
+    for (int i = threadIdx.x; i < n_inc; i += blockDim.x) T.asite_list[P.inc0 + i] = seg_l[i] & 0xffff;
     if (threadIdx.x == 0) {
-        S.n_act = n_act;
-        S.n_asite = n_as;
-        S.n_ainc = n_inc;
+        if (lag) {
+            S.nx_act = n_act;
+            S.nx_asite = n_as;
+            S.nx_ainc = n_inc;
+            S.tab_pending = 1;
+        } else {
+            S.n_act = n_act;
+            S.n_asite = n_as;
+            S.n_ainc = n_inc;
+        }
     }
 }
 
@@ -887,7 +905,7 @@ __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
     if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
     double e2 = 0, e3 = 0;
     if (a < n_act && D.act_flag[P.hb0 + a] == FLAG_LIVE) {
-        const int g = D.act_hb[P.hb0 + a];
+        const int g = hb_tab(D, S.tab).act_hb[P.hb0 + a];
         R q, psi0[3];
         hb_geometry<R>(D, g, &q, psi0, nullptr);
         const double e = D.act_E[P.hb0 + a];
@@ -953,6 +971,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __shared__ long long sh_draws;
     __shared__ int sh_err, sh_next;
     const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
+    const HbTab T = hb_tab(D, S.tab);
     if (threadIdx.x == 0) {
         D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
         if (grp == 0) { S.e[EI_E2_HB] = 0.0; S.e[EI_E3_HB] = 0.0; }   // summed by k_hbforce
@@ -966,7 +985,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     // tables, the live candidates' ids and the active sites' ids; level 2 the
     // candidates' gate beads and r0 and the sites' valences; level 3 the gate
     // positions, then the geometry of the candidates inside the gate.
-    const int n_hinc = D.ahb_inc_off[P.hb0 + r + n_act];
+    const int n_hinc = T.ahb_inc_off[P.hb0 + r + n_act];
     if (n_hinc > kAIncMax) {                      // inconsistent rebuild tables
         if (tid == 0) atomicOr(&S.overflow, 2);
         return;
@@ -978,23 +997,23 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
-        t_aoff[k] = i <= n_as ? D.asite_off[P.site0 + r + i] : 0;
-        t_sid[k] = i < n_as ? D.asite_id[P.site0 + i] : 0;
+        t_aoff[k] = i <= n_as ? T.asite_off[P.site0 + r + i] : 0;
+        t_sid[k] = i < n_as ? T.asite_id[P.site0 + i] : 0;
     }
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
         const int i = tid + k * kResBlock;
-        t_hoff[k] = i <= n_act ? D.ahb_inc_off[P.hb0 + r + i] : 0;
+        t_hoff[k] = i <= n_act ? T.ahb_inc_off[P.hb0 + r + i] : 0;
     }
 #pragma unroll
     for (int k = 0; k < KI; ++k) {
         const int i = tid + k * kResBlock;
-        t_alist[k] = i < n_ainc ? D.asite_list[P.inc0 + i] : 0;
-        t_hinc[k] = i < n_hinc ? D.ahb_inc[P.inc0 + i] : 0;
+        t_alist[k] = i < n_ainc ? T.asite_list[P.inc0 + i] : 0;
+        t_hinc[k] = i < n_hinc ? T.ahb_inc[P.inc0 + i] : 0;
     }
     // the thread's first candidate goes through the same levels (later ones, rare
     // below 1024 live candidates, load in turn)
-    const int g0 = tid < n_act ? D.act_hb[P.hb0 + tid] : 0;
+    const int g0 = tid < n_act ? T.act_hb[P.hb0 + tid] : 0;
     int t_val[KA];
 #pragma unroll
     for (int k = 0; k < KA; ++k) t_val[k] = tid + k * kResBlock < n_as ? D.site_val[t_sid[k]] : 0;
@@ -1018,7 +1037,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
 #pragma unroll 1
     for (int a = tid; a < n_act; a += nt) {
         const bool first = a == tid;
-        const int g = first ? g0 : D.act_hb[P.hb0 + a];
+        const int g = first ? g0 : T.act_hb[P.hb0 + a];
         const int b1 = first ? b10 : D.hb_b[6 * g + 1], b4 = first ? b40 : D.hb_b[6 * g + 4];
         const double r0 = first ? r00 : D.hb_g[6 * g];
         const V3<R> v = diff<R>(D.pos[b1], D.pos[b4]);
@@ -1158,8 +1177,16 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
         }
         if (owner != grp) continue;
-        D.hb_out[D.act_hb[P.hb0 + a]] = (signed char)flag[a];
+        D.hb_out[T.act_hb[P.hb0 + a]] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
+    }
+    // first resolution on a table switched by k_commit: candidates outside it
+    // (inactive in the rebuild that made it) report -1; the listed ones were
+    // written above
+    if (S.tab_fresh) {
+        const int f0 = D.rng == 1 ? 0 : grp, fs = D.rng == 1 ? 1 : ngrp;
+        for (int k = f0 * nt + tid; k < P.nhb; k += fs * nt)
+            if (D.hb_status[P.hb0 + k] < 0) D.hb_out[P.hb0 + k] = -1;
     }
     if (tid == 0) {
         D.hb_iter_part[kMaxHbGroups * r + grp] = sh_iters;
@@ -1651,6 +1678,15 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
             S.progress += m + m;
             S.need_rebuild = S.progress > P.skin;
             if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+            S.tab_fresh = 0;
+            if (S.tab_pending) {   // HB tables built beside this step's resolution
+                S.tab ^= 1;
+                S.n_act = S.nx_act;
+                S.n_asite = S.nx_asite;
+                S.n_ainc = S.nx_ainc;
+                S.tab_pending = 0;
+                S.tab_fresh = 1;
+            }
             S.klok += 1;
             S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
         }

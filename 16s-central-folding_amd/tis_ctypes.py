@@ -62,7 +62,8 @@ class Options(C.Structure):
                 ("noise_scale", C.c_double), ("skin", C.c_double), ("r_sr", C.c_double),
                 ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
                 ("graph_steps", C.c_int), ("exchange_every", C.c_longlong), ("pair_lanes", C.c_int),
-                ("pair_block", C.c_int), ("list_waves", C.c_int), ("hb_groups", C.c_int)]
+                ("pair_block", C.c_int), ("list_waves", C.c_int), ("hb_groups", C.c_int),
+                ("hb_table_lag", C.c_int)]
 
 
 class Tuning(C.Structure):

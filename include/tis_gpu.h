@@ -213,6 +213,12 @@ typedef struct tis_options {
     int list_waves;       /* wavefronts of the neighbour-list build grid, 64..65536 (4096) */
     int hb_groups;        /* workgroups per replica sharing the components of the HB valence
                              resolution (Philox draws), 1..16 (4) */
+    int hb_table_lag;     /* 1 (default): the HB tables of a list rebuild inside tis_step are
+                             built beside that step's valence resolution, which reads the
+                             previous rebuild's tables, and are used from the next step on.
+                             Results are identical (every candidate within its gate is in
+                             both); taken only when max r0 + 2 + skin + 1 < r_sr.  0: built
+                             before the resolution. */
 } tis_options;
 
 /* Fills *opt with the reference's settings (FP64, stream RNG off => Philox, skin 3, ...). */

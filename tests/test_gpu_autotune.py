@@ -83,3 +83,30 @@ def test_hb_outcome_does_not_depend_on_the_resolution_groups(lib, sys1, prec):
             assert np.linalg.norm(f - f0) <= 1e-12 * np.linalg.norm(f0), g
             assert e == pytest.approx(e0, rel=1e-12), g
             assert np.abs(x - x0).max() < (1e-9 if prec == T.PREC_FP64 else 1e-6), g
+
+
+@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
+def test_hb_tables_built_beside_the_resolution_change_nothing(lib, sys1, prec):
+    """hb_table_lag = 1 builds the HB tables of a step's list rebuild beside that
+    step's valence resolution, which reads the previous rebuild's tables; every
+    candidate within its gate is listed in both, so the rebuild counts, the HB
+    outcomes and the trajectory match hb_table_lag = 0 (the trajectory to the
+    rounding the per-bead force atomics leave, amplified over 150 steps)."""
+    d2, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.050, seed=4, box=450.0)
+    hs = [sys1, T.SystemHandle(d2)]
+    runs = []
+    for lag in (0, 1):
+        eng = T.Engine(lib, hs, precision=prec, rng=T.RNG_PHILOX, seed=3, hb_table_lag=lag, graph_steps=25)
+        out = []
+        for _ in range(6):
+            eng.step(25)
+            out.append([(eng.hb_outcome(r), eng.get_state(r)[0], eng.energies(r)["n_rebuilds"]) for r in range(2)])
+        eng.close()
+        runs.append(out)
+    assert all(rb > 1 for _, _, rb in runs[0][-1])          # several rebuilds inside the steps
+    tol = 1e-8 if prec == T.PREC_FP64 else 1e-4
+    for b0, b1 in zip(*runs):
+        for (o0, x0, n0), (o1, x1, n1) in zip(b0, b1):
+            assert n0 == n1
+            assert np.array_equal(o0, o1)
+            assert np.abs(x0 - x1).max() < tol
