@@ -194,16 +194,18 @@ struct tis_ctx {
     }
 
     // HB terms on stream st: energies and gates, the valence resolution, forces.
+    // The groups' energy sums and serial depths are reduced by k_hbsum, or in a
+    // step (mode bit 0) by k_commit.
     void launch_hb(hipStream_t st, int mode) {
-        const dim3 g2((kActMax + kBlock - 1) / kBlock, n_rep);
         timed("k_hbres", st, [&] {
             if (D.precision == TIS_PREC_FP64) k_hbres<double><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
             else k_hbres<float><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
         });
-        timed("k_hbforce", st, [&] {
-            if (D.precision == TIS_PREC_FP64) k_hbforce<double><<<g2, kBlock, 0, st>>>(D);
-            else k_hbforce<float><<<g2, kBlock, 0, st>>>(D);
-        });
+        if (D.precision == TIS_PREC_FP64)
+            timed("k_hbforce64", st, [&] {
+                k_hbforce64<<<dim3((kActMax + kBlock - 1) / kBlock, n_rep), kBlock, 0, st>>>(D);
+            });
+        if (!(mode & 1)) k_hbsum<<<grid(n_rep), kBlock, 0, st>>>(D);
     }
 
     // Energies of every replica at its current configuration, with the pair
@@ -712,13 +714,13 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
         D.hb_iter_part = dalloc<int>(own, (size_t)kMaxHbGroups * n_rep);
+        D.hb_e_part = dalloc<double>(own, 2 * (size_t)kMaxHbGroups * n_rep);
         D.tab_hb = (long long)nhb_tot + n_rep;
         D.tab_site = (long long)nsite_tot + n_rep;
         D.tab_inc = std::max<long long>(1, inc_base);
         D.act_hb = dalloc<int>(own, 2 * (size_t)D.tab_hb);
         D.act_idx = dalloc<int>(own, nhb_tot);
         D.act_tmp = dalloc<int>(own, nhb_tot);
-        D.act_E = dalloc<double>(own, nhb_tot);
         D.act_flag = dalloc<unsigned char>(own, nhb_tot);
         D.asite_id = dalloc<int>(own, 2 * (size_t)D.tab_site);
         D.asite_off = dalloc<int>(own, 2 * (size_t)D.tab_site);

@@ -111,6 +111,7 @@ struct PtLoad<Dual> {
     TIS_DEV static DualPt at(const Dev &D, int i) { return {D.pos[i], D.jv[i]}; }
 };
 
+// Block sum, valid in thread 0; scratch holds one double per wavefront.
 TIS_DEV double block_sum_double(double v, double *scratch) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     int w = wave_id(), nw = (blockDim.x + 63) >> 6;
@@ -889,34 +890,52 @@ TIS_DEV void uf_union(volatile int *par, int a, int b) {
     }
 }
 
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
-    const int r = blockIdx.y;
+// Sums of the resolution groups of each replica (one thread per replica): the
+// HB energies and the serial depth.  Steps do the same inside k_commit.
+TIS_DEV void hb_reduce_parts(const Dev &D, int r) {
     RepState &S = D.rs[r];
-    const RepParams &P = D.rp[r];
-    __shared__ double red[8];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // serial depth of the resolution: max over its groups
-        int m = 0;
-        for (int q = 0; q < D.hb_groups; ++q) m = max(m, D.hb_iter_part[kMaxHbGroups * r + q]);
-        S.hb_iters = m;
+    double e2 = 0.0, e3 = 0.0;
+    int m = 0;
+    for (int q = 0; q < D.hb_groups; ++q) {
+        e2 += D.hb_e_part[2 * (kMaxHbGroups * r + q)];
+        e3 += D.hb_e_part[2 * (kMaxHbGroups * r + q) + 1];
+        m = max(m, D.hb_iter_part[kMaxHbGroups * r + q]);
     }
+    S.e[EI_E2_HB] = e2;
+    S.e[EI_E3_HB] = e3;
+    S.hb_iters = m;
+}
+__global__ void __launch_bounds__(kBlock) k_hbsum(Dev D) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < D.n_rep) hb_reduce_parts(D, r);
+}
+
+// FP64 forces of the kept HB candidates, one thread each, after k_hbres: its
+// FP64 form does not fuse them (the force arithmetic would spill at 1,024
+// threads per workgroup).  The energy is recomputed by the same expression as
+// in k_hbres; the sums are added to group 0's part.
+__global__ void __launch_bounds__(kBlock) k_hbforce64(Dev D) {
+    const int r = blockIdx.y;
+    const RepState &S = D.rs[r];
+    const RepParams &P = D.rp[r];
+    __shared__ double red[kBlock / 64];
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_act = (S.overflow & 2) ? 0 : S.n_act;
     if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
     double e2 = 0, e3 = 0;
     if (a < n_act && D.act_flag[P.hb0 + a] == FLAG_LIVE) {
         const int g = hb_tab(D, S.tab).act_hb[P.hb0 + a];
-        R q, psi0[3];
-        hb_geometry<R>(D, g, &q, psi0, nullptr);
-        const double e = D.act_E[P.hb0 + a];
+        double q, psi0[3];
+        hb_geometry<double>(D, g, &q, psi0, nullptr);
+        const double e = D.hb_E[g] * exp(-q);
         if (D.hb_dode[g]) e3 = e; else e2 = e;
-        sextet_forces<R>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+        sextet_forces<double>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, -e, psi0, 5.0, 1.5, 0.15);
     }
     const double s2 = block_sum_double(e2, red);
     const double s3 = block_sum_double(e3, red);
     if (threadIdx.x == 0) {
-        if (s2 != 0.0) unsafeAtomicAdd(&S.e[EI_E2_HB], s2);
-        if (s3 != 0.0) unsafeAtomicAdd(&S.e[EI_E3_HB], s3);
+        if (s2 != 0.0) unsafeAtomicAdd(&D.hb_e_part[2 * kMaxHbGroups * r], s2);
+        if (s3 != 0.0) unsafeAtomicAdd(&D.hb_e_part[2 * kMaxHbGroups * r + 1], s3);
     }
 }
 
@@ -938,13 +957,14 @@ __global__ void __launch_bounds__(kBlock) k_hbforce(Dev D) {
 // First, the energy and distance gate of every live candidate
 // (hydrogen_bonds, fast.cpp:2180-2237): E = E_hb exp(-Q) when |r - r0| <= 2 A for
 // the (k11, k21) pair, gated otherwise; computed in precision R into LDS by every
-// group (group 0 also stores the energies for k_hbforce).
+// group.  Last, the forces of the kept candidates and the groups' energy sums.
 //
 // Grid (groups, replicas).  With Philox draws the components of a replica are
 // shared out over the groups (component k to group k mod groups, each group's
 // wavefronts taking its components in turn); every group builds the tables and
-// components itself and writes back the outcome of the candidates whose
-// component it ran (candidates outside every component: group 0).  Each
+// components itself and writes back the outcome (and the forces of the kept
+// ones) of the candidates whose component it ran (candidates outside every
+// component: group 0).  Each
 // component draws from its own Philox stream, so the outcome does not depend on
 // the number of groups.  The reference-stream process runs in group 0.
 template <class R>
@@ -972,9 +992,10 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __shared__ int sh_err, sh_next;
     const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
     const HbTab T = hb_tab(D, S.tab);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) {   // this group's parts (summed by k_hbsum or k_commit)
         D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
-        if (grp == 0) { S.e[EI_E2_HB] = 0.0; S.e[EI_E3_HB] = 0.0; }   // summed by k_hbforce
+        D.hb_e_part[2 * (kMaxHbGroups * r + grp)] = 0.0;
+        D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = 0.0;
     }
     if (S.overflow & 2) return;
     if (D.rng == 1 && grp > 0) return;
@@ -1052,7 +1073,6 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         }
         E[a] = e;
         flag[a] = f;
-        if (grp == 0) D.act_E[P.hb0 + a] = e;
     }
     __syncthreads();
     // ---- excess = live incidences - valence per active site; the
@@ -1168,6 +1188,11 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         }
     }
     __syncthreads();
+    // ---- outcome and, for the kept candidates, energies and forces
+    // (hydrogen_bonds_interactions, fast.cpp:2333-2345), by the group that ran
+    // the candidate's component
+    constexpr bool kFusedForces = sizeof(R) == sizeof(float);   // FP64: k_hbforce64
+    double e2 = 0.0, e3 = 0.0;
     for (int a = tid; a < n_act; a += nt) {
         int owner = 0;
         if (ngrp > 1 && D.rng == 0 && flag[a] != FLAG_GATED) {
@@ -1177,8 +1202,25 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
         }
         if (owner != grp) continue;
-        D.hb_out[T.act_hb[P.hb0 + a]] = (signed char)flag[a];
+        const int g = a == tid ? g0 : T.act_hb[P.hb0 + a];
+        D.hb_out[g] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
+        if (kFusedForces && flag[a] == FLAG_LIVE) {
+            R q, psi0[3];
+            hb_geometry<R>(D, g, &q, psi0, nullptr);
+            const double e = E[a];
+            if (D.hb_dode[g]) e3 += e; else e2 += e;
+            sextet_forces<R>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+        }
+    }
+    {
+        __shared__ double red[kResWaves];   // one per wavefront of the block
+        const double s2 = block_sum_double(e2, red);
+        const double s3 = block_sum_double(e3, red);
+        if (tid == 0) {
+            D.hb_e_part[2 * (kMaxHbGroups * r + grp)] = s2;
+            D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = s3;
+        }
     }
     // first resolution on a table switched by k_commit: candidates outside it
     // (inactive in the rebuild that made it) report -1; the listed ones were
@@ -1207,7 +1249,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         double *mw = D.maxwell + 3000 * (size_t)r;
         for (int q = threadIdx.x; q < 1500; q += blockDim.x) {
             const long long c = c0 + 2 * q;
-            if (c + 1 >= lim) atomicOr(&sh_flags, 2);
+            if (c < 0 || c + 1 >= lim) atomicOr(&sh_flags, 2);
             else if (st[c] == 0) atomicOr(&sh_flags, 1);
         }
         __syncthreads();
@@ -1227,11 +1269,11 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             for (int i = 0; i < 3000; i += 2) {
                 double r1;
                 do {
-                    if (c >= lim) { bad = true; r1 = 0.5; break; }
+                    if (c < 0 || c >= lim) { bad = true; r1 = 0.5; break; }
                     r1 = (double)st[c++] / kRandMax;
                 } while (r1 == 0);
                 double r2 = 0.5;
-                if (c < lim) r2 = (double)st[c++] / kRandMax; else bad = true;
+                if (c >= 0 && c < lim) r2 = (double)st[c++] / kRandMax; else bad = true;
                 const double a = sqrt(-2.0 * log(r1)), b = 2.0 * kPi * r2;
                 mw[i] = a * cos(b);
                 mw[i + 1] = a * sin(b);
@@ -1678,6 +1720,7 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
             S.progress += m + m;
             S.need_rebuild = S.progress > P.skin;
             if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+            hb_reduce_parts(D, r);   // the resolution groups of this step
             S.tab_fresh = 0;
             if (S.tab_pending) {   // HB tables built beside this step's resolution
                 S.tab ^= 1;

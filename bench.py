@@ -64,8 +64,7 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
                    pair are counted once: 4 bead positions 128 B minus the 8 B the
                    gate reuses), and the resolution's 27 B (energy 8, gate 1, id 4,
                    outcome 2, incidence offsets 4, two incidence entries 8); 12 B
-                   per site
-      k_hbforce    the same 120 B of bead positions plus 6 x 24 B forces
+                   per site; 144 B per kept candidate (its 6 bead forces)
       k_integrate  148 B per particle (x, x_old, v, 3 forces read, 3 states written)
       k_commit     128 B per particle
     """
@@ -80,9 +79,7 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
         elif kernel == "k_multibody":
             tot += 52 * (d["n_st3"] + d["n_st2"]) + 16 * d["n_bonds"] + 20 * d["n_angles"] + 32 * d["n_beads"]
         elif kernel == "k_hbres":
-            tot += 147 * nact + 12 * d["n_hb_sites"]
-        elif kernel == "k_hbforce":
-            tot += (120 + 144) * nact
+            tot += 147 * nact + 12 * d["n_hb_sites"] + 144 * int(np.sum(eng.hb_outcome(r) == 1))
         elif kernel == "k_integrate":
             tot += 148 * n
         elif kernel == "k_commit":
