@@ -74,8 +74,8 @@ struct tis_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     // kernel timing: when profiling is on, every launch of tis_step is bracketed
-    // by HIP events on the stream it runs on (no graph), and the durations are
-    // accumulated per kernel name
+    // by HIP events on the stream it runs on (no graph), with the device idle
+    // before and after it, and the durations are accumulated per kernel name
     bool profiling = false;
     struct Timed { const char *name; hipStream_t s; hipEvent_t a, b; };
     std::vector<Timed> pending;
@@ -127,14 +127,18 @@ struct tis_ctx {
         }
         return event_pool[event_next++];
     }
-    // Brackets one launch with events when profiling.
+    // Brackets one launch with events when profiling.  The kernel runs alone:
+    // with the streams overlapping, an event pair would also hold the time the
+    // kernel's workgroups wait for CUs held by the other streams' kernels.
     template <class L>
     void timed(const char *name, hipStream_t s, L &&launch) {
         if (!profiling) { launch(); return; }
         hipEvent_t a = next_event(), b = next_event();
+        HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(hipEventRecord(a, s));
         launch();
         HIP_TRY(hipEventRecord(b, s));
+        HIP_TRY(hipDeviceSynchronize());
         pending.push_back({name, s, a, b});
     }
     void collect_times() {

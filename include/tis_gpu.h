@@ -396,7 +396,9 @@ int tis_set_uniform_stream(tis_ctx *ctx, int rep, const int32_t *raw, long long 
 int tis_get_stream_cursor(tis_ctx *ctx, int rep, long long *consumed);
 
 /* Kernel timing.  While profiling is on, tis_step runs without graph capture and
- * brackets every kernel launch with HIP events on the stream it is launched on.
+ * brackets every kernel launch with HIP events on the stream it is launched on;
+ * each kernel runs alone (the device is idle before and after it), so the times
+ * are those of the kernels by themselves, not of the overlapped step.
  * tis_kernel_times writes one line "name launches avg_ms" per kernel, accumulated
  * since profiling was switched on.  (No reference counterpart: the reference has
  * no live timing, only a commented-out ftime block at fast.cpp:6520-6541.) */
