@@ -731,6 +731,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.asite_list = dalloc<int>(own, 2 * (size_t)D.tab_inc);
         D.ahb_inc_off = dalloc<int>(own, 2 * (size_t)D.tab_hb);
         D.ahb_inc = dalloc<int>(own, 2 * (size_t)D.tab_inc);
+        D.act_rec = dalloc<int4>(own, 4 * (size_t)D.tab_hb);
         D.st_energy = dalloc<double>(own, st3E.size());
         D.st_part = dalloc<double>(own, 2 * (size_t)kMaxStSlices * n_rep);
         {   // one stack per thread in a step: slices of kBlock stacks

@@ -132,6 +132,7 @@ struct Dev {
     int *asite_list;              // region [inc0, ...): live indices per active site, static order
     int *ahb_inc_off;             // region [hb0 + rep, ...]: CSR offsets per live candidate
     int *ahb_inc;                 // region [inc0, ...): active-site index of each incidence
+    int4 *act_rec;                // region [2 hb0, ...): candidate id and its six beads, per live index
     // output / scratch
     double *st_energy;            // per tertiary stack
     double *st_part;              // per replica and stack slice: E2, E3 stack sums of the step

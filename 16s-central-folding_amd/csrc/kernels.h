@@ -366,10 +366,12 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 // One buffer of the HB rebuild tables (the arrays hold two, tab_* elements apart).
 struct HbTab {
     int *act_hb, *asite_id, *asite_off, *asite_list, *ahb_inc_off, *ahb_inc;
+    int4 *act_rec;   // per listed candidate: (id, beads 0-2), (beads 3-5, 0)
 };
 TIS_DEV HbTab hb_tab(const Dev &D, int b) {
     return {D.act_hb + b * D.tab_hb,      D.asite_id + b * D.tab_site,    D.asite_off + b * D.tab_site,
-            D.asite_list + b * D.tab_inc, D.ahb_inc_off + b * D.tab_hb, D.ahb_inc + b * D.tab_inc};
+            D.asite_list + b * D.tab_inc, D.ahb_inc_off + b * D.tab_hb, D.ahb_inc + b * D.tab_inc,
+            D.act_rec + 2 * b * D.tab_hb};
 }
 
 // k_hbstatus: grid-stride over the candidates of the replicas being rebuilt.
@@ -446,6 +448,9 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
             const int g = D.act_tmp[P.hb0 + lo * kStatusBlock + (a - chunk_off[lo])];
             D.act_idx[g] = a;
             T.act_hb[P.hb0 + a] = g;
+            const int *hb = D.hb_b + 6 * g;
+            T.act_rec[2 * (P.hb0 + a)] = make_int4(g, hb[0], hb[1], hb[2]);
+            T.act_rec[2 * (P.hb0 + a) + 1] = make_int4(hb[3], hb[4], hb[5], 0);
         }
     }
     __syncthreads();
@@ -571,14 +576,11 @@ static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of o
 
 enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
 
-template <class R>
-TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused) {
-    (void)d_unused;
-    const int *b = D.hb_b + 6 * g;
-    const double *geo = D.hb_g + 6 * g;
-    using Pt = typename PtOf<R>::type;
-    const Pt p0 = PtLoad<R>::at(D, b[0]), p1 = PtLoad<R>::at(D, b[1]), p2 = PtLoad<R>::at(D, b[2]);
-    const Pt p3 = PtLoad<R>::at(D, b[3]), p4 = PtLoad<R>::at(D, b[4]), p5 = PtLoad<R>::at(D, b[5]);
+// HB geometry of candidate positions p[0..5] with its constants geo (r0, theta1,
+// theta2, psi0, psi1, psi2): Q and the re-wrapped dihedral targets.
+template <class R, class Pt>
+TIS_DEV void hb_geometry_at(const Pt *pt, const double *geo, R *q_out, R *psi0) {
+    const Pt &p0 = pt[0], &p1 = pt[1], &p2 = pt[2], &p3 = pt[3], &p4 = pt[4], &p5 = pt[5];
     V3<R> v14 = diff<R>(p1, p4);
     R rr = sqrt(dot(v14, v14));
     R c1, c2;
@@ -601,18 +603,24 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
     q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
     *q_out = q;
 }
+template <class R>
+TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused) {
+    (void)d_unused;
+    const int *b = D.hb_b + 6 * g;
+    using Pt = typename PtOf<R>::type;
+    const Pt p[6] = {PtLoad<R>::at(D, b[0]), PtLoad<R>::at(D, b[1]), PtLoad<R>::at(D, b[2]),
+                     PtLoad<R>::at(D, b[3]), PtLoad<R>::at(D, b[4]), PtLoad<R>::at(D, b[5])};
+    hb_geometry_at<R>(p, D.hb_g + 6 * g, q_out, psi0);
+}
 
 // Forces of the six-bead Gaussian/Lorentzian-type term: with w the coefficient
 // (-E for HB, -E/(1+Q) for stacks), each harmonic piece k (y - y0)^2 is applied
 // with k scaled by w (stacking_*_force calls, fast.cpp:2043-2055, 2333-2345).
 // With vdotf, also returns the sum over the six beads of (force value) . (position
 // tangent), the derivative of v . F along a factor that scales the whole term.
-template <class R>
-TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double *geo, R w, const R psi0[3],
-                           R kr, R kth, R kpsi, double *vdotf = nullptr) {
-    using Pt = typename PtOf<R>::type;
-    const Pt p[6] = {PtLoad<R>::at(D, b[0]), PtLoad<R>::at(D, b[1]), PtLoad<R>::at(D, b[2]),
-                     PtLoad<R>::at(D, b[3]), PtLoad<R>::at(D, b[4]), PtLoad<R>::at(D, b[5])};
+template <class R, class Pt>
+TIS_DEV void sextet_forces_at(double4 *F, const int *b, const Pt *p, const double *geo, R w, const R psi0[3],
+                              R kr, R kth, R kpsi, double *vdotf = nullptr) {
     // the six beads' forces are summed here, then added once each
     V3<R> fb[6];
 #pragma unroll
@@ -662,6 +670,14 @@ TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double 
         for (int q = 0; q < 6; ++q) sdot += value_dot_tangent(fb[q], p[q]);
         *vdotf = sdot;
     }
+}
+template <class R>
+TIS_DEV void sextet_forces(const Dev &D, double4 *F, const int *b, const double *geo, R w, const R psi0[3],
+                           R kr, R kth, R kpsi, double *vdotf = nullptr) {
+    using Pt = typename PtOf<R>::type;
+    const Pt p[6] = {PtLoad<R>::at(D, b[0]), PtLoad<R>::at(D, b[1]), PtLoad<R>::at(D, b[2]),
+                     PtLoad<R>::at(D, b[3]), PtLoad<R>::at(D, b[4]), PtLoad<R>::at(D, b[5])};
+    sextet_forces_at<R>(F, b, p, geo, w, psi0, kr, kth, kpsi, vdotf);
 }
 
 // Uniform source of one step's HB resolution, with the replica's counters
@@ -1034,12 +1050,17 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     }
     // the thread's first candidate goes through the same levels (later ones, rare
     // below 1024 live candidates, load in turn)
-    const int g0 = tid < n_act ? T.act_hb[P.hb0 + tid] : 0;
+    // the thread's first candidate: its record with the tables (level 1), its
+    // bead positions and constants next (level 2)
+    int4 ra0 = make_int4(0, 0, 0, 0), rb0 = ra0;
+    if (tid < n_act) {
+        ra0 = T.act_rec[2 * (P.hb0 + tid)];
+        rb0 = T.act_rec[2 * (P.hb0 + tid) + 1];
+    }
+    const int g0 = ra0.x;
     int t_val[KA];
 #pragma unroll
     for (int k = 0; k < KA; ++k) t_val[k] = tid + k * kResBlock < n_as ? D.site_val[t_sid[k]] : 0;
-    const int b10 = tid < n_act ? D.hb_b[6 * g0 + 1] : 0, b40 = tid < n_act ? D.hb_b[6 * g0 + 4] : 0;
-    const double r00 = tid < n_act ? D.hb_g[6 * g0] : 0.0;
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
@@ -1057,18 +1078,23 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     }
 #pragma unroll 1
     for (int a = tid; a < n_act; a += nt) {
-        const bool first = a == tid;
-        const int g = first ? g0 : T.act_hb[P.hb0 + a];
-        const int b1 = first ? b10 : D.hb_b[6 * g + 1], b4 = first ? b40 : D.hb_b[6 * g + 4];
-        const double r0 = first ? r00 : D.hb_g[6 * g];
-        const V3<R> v = diff<R>(D.pos[b1], D.pos[b4]);
+        int4 ra = ra0, rb = rb0;
+        if (a != tid) {
+            ra = T.act_rec[2 * (P.hb0 + a)];
+            rb = T.act_rec[2 * (P.hb0 + a) + 1];
+        }
+        const int g = ra.x;
+        const double4 pt[6] = {D.pos[ra.y], D.pos[ra.z], D.pos[ra.w], D.pos[rb.x], D.pos[rb.y], D.pos[rb.z]};
+        const double *geo = D.hb_g + 6 * g;
+        const double hE = D.hb_E[g];
+        const V3<R> v = diff<R>(pt[1], pt[4]);
         const R rr = sqrt(dot(v, v));
         double e = 0.0;
         unsigned char f = FLAG_GATED;
-        if (!(fabs(rr - (R)r0) > (R)2)) {
+        if (!(fabs(rr - (R)geo[0]) > (R)2)) {
             R q, psi0[3];
-            hb_geometry<R>(D, g, &q, psi0, nullptr);
-            e = (double)((R)D.hb_E[g] * exp(-q));
+            hb_geometry_at<R>(pt, geo, &q, psi0);
+            e = (double)((R)hE * exp(-q));
             f = FLAG_LIVE;
         }
         E[a] = e;
@@ -1202,15 +1228,24 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
         }
         if (owner != grp) continue;
-        const int g = a == tid ? g0 : T.act_hb[P.hb0 + a];
+        int4 ra = ra0, rb = rb0;
+        if (a != tid) {
+            ra = T.act_rec[2 * (P.hb0 + a)];
+            rb = T.act_rec[2 * (P.hb0 + a) + 1];
+        }
+        const int g = ra.x;
         D.hb_out[g] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
         if (kFusedForces && flag[a] == FLAG_LIVE) {
+            const int bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
+            const double4 pt[6] = {D.pos[bd[0]], D.pos[bd[1]], D.pos[bd[2]], D.pos[bd[3]], D.pos[bd[4]], D.pos[bd[5]]};
+            const double *geo = D.hb_g + 6 * g;
+            const int dode = D.hb_dode[g];
             R q, psi0[3];
-            hb_geometry<R>(D, g, &q, psi0, nullptr);
+            hb_geometry_at<R>(pt, geo, &q, psi0);
             const double e = E[a];
-            if (D.hb_dode[g]) e3 += e; else e2 += e;
-            sextet_forces<R>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+            if (dode) e3 += e; else e2 += e;
+            sextet_forces_at<R>(D.f_mb, bd, pt, geo, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
         }
     }
     {
