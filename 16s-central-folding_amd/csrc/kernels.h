@@ -1123,16 +1123,17 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             if (ex > 0) {
                 apack[nA0 + rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
                 excess[nA0 + rank] = ex;
+                par[nA0 + rank] = nA0 + rank;   // union-find roots of the component form
             }
         }
         nA0 += tot;
     }
     __syncthreads();
-    // incidences as over-subscribed indices (-1: a site that is not)
-    for (int q = tid; q < n_hinc; q += nt) hinc[q] = A1[hinc[q]];
-    __syncthreads();
     const HbDraws dr(D, r, S);
     if (D.rng == 1) {
+        // incidences as over-subscribed indices (-1: a site that is not)
+        for (int q = tid; q < n_hinc; q += nt) hinc[q] = A1[hinc[q]];
+        __syncthreads();
         // reference stream: the exact sequential process on wavefront 0, over all
         // over-subscribed sites
         if (wv == 0) {
@@ -1150,28 +1151,35 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
         }
     } else {
-        // ---- components of the over-subscribed sites (union-find over A0 indices)
+        // ---- components of the over-subscribed sites (union-find over A0 indices;
+        // roots set with the excess).  Each candidate's thread turns its own
+        // incidences into over-subscribed indices (-1: a site that is not) and,
+        // if the candidate is live, joins the sites it touches.
         volatile int *vpar = par;
-        for (int ci = tid; ci < nA0; ci += nt) par[ci] = ci;
-        __syncthreads();
         for (int a = tid; a < n_act; a += nt) {
-            if (flag[a] != FLAG_LIVE) continue;
+            const bool live = flag[a] == FLAG_LIVE;
             int first = -1;
             for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = hinc[q];
-                if (ci < 0) continue;
+                const int ci = A1[hinc[q]];
+                hinc[q] = (short)ci;
+                if (!live || ci < 0) continue;
                 if (first < 0) first = ci;
                 else uf_union(vpar, first, ci);
             }
         }
         __syncthreads();
-        for (int ci = tid; ci < nA0; ci += nt) par[ci] = uf_find(vpar, ci);
-        __syncthreads();
-        // roots (smallest index of each component) in order
+        // roots (smallest index of each component) in order; every index points
+        // at its root afterwards (finds only shorten paths, so they may run
+        // beside each other)
         int ncomp = 0;
         for (int c0 = 0; c0 < nA0; c0 += nt) {
             const int ci = c0 + tid;
-            const bool root = ci < nA0 && par[ci] == ci;
+            int rt = -1;
+            if (ci < nA0) {
+                rt = uf_find(vpar, ci);
+                par[ci] = rt;
+            }
+            const bool root = ci < nA0 && rt == ci;
             int tot;
             const int rank = block_excl_sum(root, scratch, &tot);
             if (root) {
