@@ -375,9 +375,11 @@ def test_exchange_cadence_in_steps_and_incompatible_neighbours(lib):
 @pytest.mark.parametrize("prec,tol", [(T.PREC_FP64, 1e-12), (T.PREC_MIXED, 1e-6)])
 def test_step_stack_energies_equal_an_evaluation(lib, prec, tol):
     """In a step the stacks of a replica are spread over several workgroups and
-    their energy sums added up after the streams join; the E2 / E3 stack energies
-    a step reports are those of an evaluation of the state it started from (two
-    replicas of different systems in one batch)."""
+    their energy sums added up after the streams join, and the HB energy sums of
+    the resolution groups are added up when the step closes; the E2 / E3 stack
+    and HB energies a step reports are those of an evaluation of the state it
+    started from at the same step counter (two replicas of different systems in
+    one batch; Philox draws depend on the step counter only)."""
     hs = []
     for c, box, seed in ((0.005, 350.0, 1), (0.02, 450.0, 2)):
         d, _ = T.build_16scd(lib, T=0.596, c_mg=c, c_k=0.050, seed=seed, box=box)
@@ -386,14 +388,16 @@ def test_step_stack_energies_equal_an_evaluation(lib, prec, tol):
     eng.step(20)
     want = []
     for r in range(2):
-        _, e = eng.eval(T.TERM_STACK, rep=r)
-        want.append((e["e2_stack"], e["e3_stack"]))
+        _, e = eng.eval(T.TERM_STACK | T.TERM_HB, rep=r)
+        want.append((e["e2_stack"], e["e3_stack"], e["e2_hb"], e["e3_hb"]))
     eng.step(1)
     for r in range(2):
         e = eng.energies(r)
         assert e["e2_stack"] == pytest.approx(want[r][0], rel=tol), r
         assert e["e3_stack"] == pytest.approx(want[r][1], rel=tol), r
-        assert want[r][0] != 0.0 and want[r][1] != 0.0
+        assert e["e2_hb"] == pytest.approx(want[r][2], rel=1e-12), r
+        assert e["e3_hb"] == pytest.approx(want[r][3], rel=1e-12), r
+        assert all(w != 0.0 for w in want[r])
     eng.close()
 
 
