@@ -723,7 +723,6 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.tab_site = (long long)nsite_tot + n_rep;
         D.tab_inc = std::max<long long>(1, inc_base);
         D.act_hb = dalloc<int>(own, 2 * (size_t)D.tab_hb);
-        D.act_idx = dalloc<int>(own, nhb_tot);
         D.act_tmp = dalloc<int>(own, nhb_tot);
         D.act_flag = dalloc<unsigned char>(own, nhb_tot);
         D.asite_id = dalloc<int>(own, 2 * (size_t)D.tab_site);

@@ -124,7 +124,6 @@ struct Dev {
     double *hb_e_part;            // per replica and resolution group: its E2, E3 HB sums
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off
     int *act_hb;                  // per replica region [hb0, hb0 + nhb): live candidate ids
-    int *act_idx;                 // candidate id -> live index or -1
     int *act_tmp;                 // rebuild scratch: each status chunk's live ids, packed
     unsigned char *act_flag;      // region [hb0, hb0 + nhb): gate / resolution outcome
     int *asite_id;                // region [site0, ...): site of each active site

@@ -402,7 +402,6 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
     // chunk offsets
     int tot;
     const int rank = block_excl_sum(live, scratch, &tot);
-    if (k < P.nhb) D.act_idx[P.hb0 + k] = -1;
     if (live) D.act_tmp[P.hb0 + blockIdx.x * kStatusBlock + rank] = P.hb0 + k;   // the chunk's live ids, packed
     if (threadIdx.x == 0) D.chunk_cnt[kMaxHbChunks * r + blockIdx.x] = tot;
 }
@@ -446,7 +445,6 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
                 else hi = mid;
             }
             const int g = D.act_tmp[P.hb0 + lo * kStatusBlock + (a - chunk_off[lo])];
-            D.act_idx[g] = a;
             T.act_hb[P.hb0 + a] = g;
             const int *hb = D.hb_b + 6 * g;
             T.act_rec[2 * (P.hb0 + a)] = make_int4(g, hb[0], hb[1], hb[2]);
