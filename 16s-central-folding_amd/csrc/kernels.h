@@ -938,12 +938,16 @@ __global__ void __launch_bounds__(kBlock) k_hbforce64(Dev D) {
     if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
     double e2 = 0, e3 = 0;
     if (a < n_act && D.act_flag[P.hb0 + a] == FLAG_LIVE) {
-        const int g = hb_tab(D, S.tab).act_hb[P.hb0 + a];
+        const HbTab T = hb_tab(D, S.tab);
+        const int4 ra = T.act_rec[2 * (P.hb0 + a)], rb = T.act_rec[2 * (P.hb0 + a) + 1];
+        const int g = ra.x, bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
+        const double4 pt[6] = {D.pos[bd[0]], D.pos[bd[1]], D.pos[bd[2]], D.pos[bd[3]], D.pos[bd[4]], D.pos[bd[5]]};
+        const double *geo = D.hb_g + 6 * g;
         double q, psi0[3];
-        hb_geometry<double>(D, g, &q, psi0, nullptr);
+        hb_geometry_at<double>(pt, geo, &q, psi0);
         const double e = D.hb_E[g] * exp(-q);
         if (D.hb_dode[g]) e3 = e; else e2 = e;
-        sextet_forces<double>(D, D.f_mb, D.hb_b + 6 * g, D.hb_g + 6 * g, -e, psi0, 5.0, 1.5, 0.15);
+        sextet_forces_at<double>(D.f_mb, bd, pt, geo, -e, psi0, 5.0, 1.5, 0.15);
     }
     const double s2 = block_sum_double(e2, red);
     const double s3 = block_sum_double(e3, red);
@@ -1355,7 +1359,8 @@ TIS_DEV R tertiary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
     q += (R)0.15 * (ps1 - psi0[1]) * (ps1 - psi0[1]);
     q += (R)0.15 * (ps2 - psi0[2]) * (ps2 - psi0[2]);
     const R e = (R)D.st3_E[g] / q;
-    sextet_forces<R>(D, F, b, geo, -e / q, psi0, (R)5, (R)1.5, (R)0.15, vdotf);
+    const Pt pt[6] = {p0, p1, p2, p3, p4, p5};
+    sextet_forces_at<R>(F, b, pt, geo, -e / q, psi0, (R)5, (R)1.5, (R)0.15, vdotf);
     return e;
 }
 
