@@ -733,9 +733,10 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.act_rec = dalloc<int4>(own, 4 * (size_t)D.tab_hb);
         D.st_energy = dalloc<double>(own, st3E.size());
         D.st_part = dalloc<double>(own, 2 * (size_t)kMaxStSlices * n_rep);
-        {   // one stack per thread in a step: slices of kBlock stacks
+        {   // one stack per thread in a step: slices of kBlock stack slots (the
+            // tertiary stacks padded to whole wavefronts)
             int mx = 1;
-            for (const RepParams &q : c->rp) mx = std::max(mx, q.nst3 + q.nst2);
+            for (const RepParams &q : c->rp) mx = std::max(mx, ((q.nst3 + 63) & ~63) + q.nst2);
             D.st_slices = std::min(kMaxStSlices, (mx + kBlock - 1) / kBlock);
         }
         D.maxwell = dalloc<double>(own, 3000 * (size_t)n_rep);

@@ -1494,14 +1494,17 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         }
         __syncthreads();
         double e2 = 0, e3 = 0;
-        for (int k = slice * blockDim.x + threadIdx.x; k < P.nst3 + P.nst2; k += sb * blockDim.x) {
+        // tertiary stacks on whole wavefronts of their own (indices padded to a
+        // multiple of 64), so that no wavefront runs both code paths
+        const int n3p = (P.nst3 + 63) & ~63;
+        for (int k = slice * blockDim.x + threadIdx.x; k < n3p + P.nst2; k += sb * blockDim.x) {
             if (k < P.nst3) {
                 const int g = P.st30 + k;
                 const double e = (double)tertiary_stack<R>(D, F, g, nullptr);
                 D.st_energy[g] = e;
                 e3 += e;
-            } else {
-                const int m = k - P.nst3;
+            } else if (k >= n3p) {
+                const int m = k - n3p;
                 if (off[m]) continue;
                 e2 += (double)secondary_stack<R>(D, F, P.st20 + m, nullptr);
             }
