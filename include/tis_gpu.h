@@ -386,7 +386,9 @@ int tis_get_pairs(tis_ctx *ctx, int rep, int which, int *ij_out, long long cap, 
 int tis_get_hb_status(tis_ctx *ctx, int rep, int *status_out, int n);
 /* Parity hook: outcome of every candidate in the last HB evaluation: -1 not in the
  * SR list, 0 outside the distance gate, 1 kept, 2 switched off by the valence
- * resolution (hydrogen_bonds_interactions, fast.cpp:2165-2320). */
+ * resolution (hydrogen_bonds_interactions, fast.cpp:2165-2320).  "SR list" is the
+ * latest list build; a candidate it adds after the last evaluation (HB tables
+ * built beside a step's resolution, hb_table_lag) is outside its gate and reads 0. */
 int tis_get_hb_outcome(tis_ctx *ctx, int rep, int *out, int n);
 
 /* TIS_RNG_STREAM: raw glibc rand() values (0..RAND_MAX) consumed in the reference's
