@@ -85,18 +85,24 @@ def test_hb_outcome_does_not_depend_on_the_resolution_groups(lib, sys1, prec):
             assert np.abs(x - x0).max() < (1e-9 if prec == T.PREC_FP64 else 1e-6), g
 
 
-@pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
-def test_hb_tables_built_beside_the_resolution_change_nothing(lib, sys1, prec):
+@pytest.mark.parametrize("prec,rng", [(T.PREC_FP64, T.RNG_PHILOX), (T.PREC_MIXED, T.RNG_PHILOX),
+                                      (T.PREC_FP64, T.RNG_STREAM)])
+def test_hb_tables_built_beside_the_resolution_change_nothing(lib, sys1, prec, rng):
     """hb_table_lag = 1 builds the HB tables of a step's list rebuild beside that
     step's valence resolution, which reads the previous rebuild's tables; every
     candidate within its gate is listed in both, so the rebuild counts, the HB
     outcomes and the trajectory match hb_table_lag = 0 (the trajectory to the
-    rounding the per-bead force atomics leave, amplified over 150 steps)."""
+    rounding the per-bead force atomics leave, amplified over 150 steps), with
+    Philox draws and with the reference's stream (whose consumption must not
+    change either)."""
     d2, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.050, seed=4, box=450.0)
     hs = [sys1, T.SystemHandle(d2)]
     runs = []
     for lag in (0, 1):
-        eng = T.Engine(lib, hs, precision=prec, rng=T.RNG_PHILOX, seed=3, hb_table_lag=lag, graph_steps=25)
+        eng = T.Engine(lib, hs, precision=prec, rng=rng, seed=3, hb_table_lag=lag, graph_steps=25)
+        if rng == T.RNG_STREAM:   # the reference's draws: the same stream in both runs
+            for r in range(2):
+                eng.set_stream(T.glibc_stream(lib, 11 + r, 1_500_000), rep=r)
         out = []
         for _ in range(6):
             eng.step(25)
