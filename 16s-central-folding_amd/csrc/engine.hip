@@ -73,6 +73,9 @@ struct tis_ctx {
     bool graph_ok = false;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    // launches of tis_step: graph replays, steps replayed from graphs, steps
+    // launched one by one
+    long long n_graph_launch = 0, n_graph_step = 0, n_eager_step = 0;
     // kernel timing: when profiling is on, every launch of tis_step is bracketed
     // by HIP events on the stream it runs on (no graph), with the device idle
     // before and after it, and the durations are accumulated per kernel name
@@ -177,14 +180,8 @@ struct tis_ctx {
     // evaluations reflect the current positions, and clear the trigger.
     void finish_pending_rebuild() {
         launch_rebuild();
-        HIP_TRY(hipStreamSynchronize(s_main));
-        if (profiling) collect_times();
-        std::vector<RepState> all(n_rep);
-        HIP_TRY(hipMemcpy(all.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
-        bool any = false;
-        for (auto &st : all)
-            if (st.need_rebuild) { st.need_rebuild = 0; any = true; }
-        if (any) HIP_TRY(hipMemcpy(D.rs, all.data(), n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+        k_clear_rebuild<<<grid(n_rep), kBlock, 0, s_main>>>(D);
+        HIP_TRY(hipGetLastError());
     }
     int max_hb_chunks() const {
         int m = 1;
@@ -260,12 +257,14 @@ struct tis_ctx {
     // untouched, multi-body forces cleared), then contacts, shape, energies and
     // HB / stack outcomes are added to the accumulators.
     void launch_observe() {
+        k_obs_keep<<<n_rep, kBlock, 0, s_main>>>(D, 0);
         launch_rebuild();
         launch_energies(true);
         k_clear_rebuild<<<grid(n_rep), kBlock, 0, s_main>>>(D);
         k_zero_fmb<<<grid(D.n_tot), kBlock, 0, s_main>>>(D);
         k_obs_contacts<<<grid(D.n_tot), kBlock, 0, s_main>>>(D, obs_cut * obs_cut);
         k_obs_reduce<<<n_rep, kBlock, 0, s_main>>>(D);
+        k_obs_keep<<<n_rep, kBlock, 0, s_main>>>(D, 1);   // the step's energies for the output files
         HIP_TRY(hipGetLastError());
     }
     void reset_observables() {
@@ -767,6 +766,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.obs_hb = dalloc<double>(own, std::max<size_t>(1, h0));
         D.obs_st3 = dalloc<double>(own, std::max<size_t>(1, s30));
         D.obs_cnt = dalloc<int>(own, 4 * (size_t)n_rep);
+        D.obs_keep = dalloc<double>(own, (size_t)kObsKeep * n_rep + std::max<size_t>(1, s30));
+        D.obs_keep_hb = dalloc<signed char>(own, std::max<size_t>(1, h0));
         // ---- force matching: tangents, target, outputs
         D.jv = dalloc<double4>(own, N);
         D.jf = dalloc<double4>(own, N);
@@ -910,6 +911,56 @@ extern "C" int tis_set_step_counter(tis_ctx *c, int rep, long long klok) {
     });
 }
 
+extern "C" int tis_set_rng_epoch(tis_ctx *c, int rep, unsigned epoch) {
+    if (!c || rep < -1 || rep >= c->n_rep) return fail(Fail{TIS_EINVAL, "tis_set_rng_epoch: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        for (int r = 0; r < c->n_rep; ++r) {
+            if (rep >= 0 && r != rep) continue;
+            RepState s = c->get_rs(r);
+            s.rng_epoch = epoch;
+            c->put_rs(r, s);
+        }
+        return TIS_OK;
+    });
+}
+
+// Captures and instantiates the graph of opt.graph_steps steps (launching
+// nothing) unless it exists.
+static void capture_graph(tis_ctx *c) {
+    if (c->graph) return;
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
+    HIP_TRY(hipStreamEndCapture(c->s_main, &g));
+    HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphDestroy(g));
+    c->graph_steps = c->opt.graph_steps;
+}
+
+extern "C" int tis_prepare_graph(tis_ctx *c, int graph_steps) {
+    if (!c || graph_steps < 0) return fail(Fail{TIS_EINVAL, "tis_prepare_graph: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        if (c->graph && (graph_steps != c->graph_steps || !c->opt.use_graph || graph_steps == 0)) {
+            HIP_TRY(hipGraphExecDestroy(c->graph));
+            c->graph = nullptr;
+        }
+        c->opt.graph_steps = graph_steps;
+        if (graph_steps > 0 && c->opt.use_graph && !c->profiling) capture_graph(c);
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_launch_counts(tis_ctx *c, long long *graph_launches, long long *graph_steps,
+                                     long long *eager_steps) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_get_launch_counts: bad arguments"});
+    if (graph_launches) *graph_launches = c->n_graph_launch;
+    if (graph_steps) *graph_steps = c->n_graph_step;
+    if (eager_steps) *eager_steps = c->n_eager_step;
+    return TIS_OK;
+}
+
 // n steps: event-timed eager launches when profiling, else whole captured
 // graphs of graph_steps and eager launches for the remainder.
 static void run_steps(tis_ctx *c, long long n_steps) {
@@ -924,23 +975,19 @@ static void run_steps(tis_ctx *c, long long n_steps) {
             HIP_TRY(hipStreamSynchronize(c->s_mb));
             c->collect_times();
             done += m;
+            c->n_eager_step += m;
         }
     }
     if (c->opt.use_graph && c->opt.graph_steps > 0 && n_steps - done >= c->opt.graph_steps) {
-        if (!c->graph) {
-            hipGraph_t g;
-            HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
-            for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
-            HIP_TRY(hipStreamEndCapture(c->s_main, &g));
-            HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-            HIP_TRY(hipGraphDestroy(g));
-            c->graph_steps = c->opt.graph_steps;
-        }
+        capture_graph(c);
         while (n_steps - done >= c->graph_steps) {
             HIP_TRY(hipGraphLaunch(c->graph, c->s_main));
             done += c->graph_steps;
+            c->n_graph_launch += 1;
+            c->n_graph_step += c->graph_steps;
         }
     }
+    c->n_eager_step += n_steps - done;
     for (; done < n_steps; ++done) c->launch_step();
 }
 
@@ -962,6 +1009,7 @@ extern "C" int tis_step(tis_ctx *c, long long n_steps) {
         HIP_TRY(hipGetLastError());
         if (n_steps > 0) c->finish_pending_rebuild();
         HIP_TRY(hipStreamSynchronize(c->s_main));
+        if (c->profiling) c->collect_times();
         c->check_flags();
         return TIS_OK;
     });

@@ -58,7 +58,8 @@ struct RepState {
     int tab, tab_pending, tab_fresh;
     int nx_act, nx_asite, nx_ainc;
     int max_sr, max_lr;           // largest per-particle SR / LR counts of the last build
-    int pad;
+    unsigned rng_epoch;           // Philox key word: a restart of klok on the same context moves to a
+                                  // new epoch, so no draw of an earlier run repeats
     double e[12];                 // energies of the last force evaluation
 };
 enum { EI_E2_HB, EI_E3_HB, EI_E2_ST, EI_E3_ST, EI_WCA, EI_COUL_SR, EI_COUL_LR, EI_BOND, EI_ANGLE };
@@ -151,6 +152,8 @@ struct Dev {
     double4 *obs_p;
     double *obs_hb, *obs_st3;
     int *obs_cnt;
+    double *obs_keep;             // a step's energies and stack energies kept across a sample
+    signed char *obs_keep_hb;     // and its HB outcomes
 };
 
 }  // namespace tis

@@ -78,6 +78,12 @@ TIS_DEV int block_excl_sum(int v, int *scratch, int *total) {
     return res;
 }
 
+// Philox key of replica r: (seed low word mixed with the replica, seed high word
+// plus the replica's epoch).  The counter carries the step, so draws never
+// repeat within an epoch.
+TIS_DEV uint32_t philox_key0(const Dev &D, int r) { return (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1)); }
+TIS_DEV uint32_t philox_key1(const Dev &D, const RepState &S) { return (uint32_t)(D.seed >> 32) + S.rng_epoch; }
+
 TIS_DEV double4 d4add(double4 a, double4 b) { return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, 0.0); }
 
 // The .w word of a cell-ordered position holds the particle index (low 32 bits)
@@ -691,8 +697,8 @@ struct HbDraws {
         rng = D.rng;
         c0 = (uint32_t)S.klok;
         c1 = (uint32_t)(S.klok >> 32);
-        k0 = (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1));
-        k1 = (uint32_t)(D.seed >> 32);
+        k0 = philox_key0(D, r);
+        k1 = philox_key1(D, S);
         stream = rng ? D.stream + D.stream_base[r] : nullptr;
         cur = S.stream_cursor;
         lim = rng ? D.stream_len[r] : 0;
@@ -1648,7 +1654,7 @@ TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, const RepState &S, 
     const int local = p - P.p0;
     if (D.rng == 0) {
         Philox4 o = philox4x32_10((uint32_t)S.klok, (uint32_t)(S.klok >> 32), (uint32_t)local, kTagNoise,
-                                  (uint32_t)D.seed ^ (0x9E3779B9u * (uint32_t)(r + 1)), (uint32_t)(D.seed >> 32));
+                                  philox_key0(D, r), philox_key1(D, S));
         const double a = sqrt(-2.0 * log(u01_open_closed(o.v[0]))), b = 2.0 * kPi * u01_closed_open(o.v[1]);
         const double c = sqrt(-2.0 * log(u01_open_closed(o.v[2]))), d = 2.0 * kPi * u01_closed_open(o.v[3]);
         out[0] = a * cos(b);

@@ -27,12 +27,15 @@
 //   obs_st3  per tertiary stack: sum of its energy
 //   obs_cnt  per replica, 4 ints: bound-ion counts of the current sample (scratch,
 //            cleared by k_obs_reduce)
+//   obs_keep the step's energies (RepState::e, hb_iters), per-stack energies and HB
+//            outcomes, saved before a sample's evaluation and restored after it
 #pragma once
 #include "kernels.h"
 
 namespace tis {
 
 constexpr int kObsRep = 24;
+constexpr int kObsKeep = 13;   // doubles per replica in obs_keep
 
 // Contact counts, one thread per particle.  A P bead counts the ions among its
 // SR neighbours within the cut; an ion marks itself bound when a P bead among
@@ -129,6 +132,37 @@ __global__ void __launch_bounds__(kBlock) k_obs_reduce(Dev D) {
     for (int k = threadIdx.x; k < P.nst3; k += blockDim.x) {
         const int g = P.st30 + k;
         D.obs_st3[g] += D.st_energy[g];
+    }
+}
+
+// Saves (restore = 0) or restores (restore = 1) what the last step's force call
+// left for the output files and the outcome report: the energy terms and serial
+// depth of each replica, the tertiary stack energies (Contacts columns) and the
+// HB outcomes.  A sample evaluates every term afresh at the current positions,
+// which would otherwise replace them.  Grid: one workgroup per replica.
+__global__ void __launch_bounds__(kBlock) k_obs_keep(Dev D, int restore) {
+    const int r = blockIdx.x;
+    const RepParams &P = D.rp[r];
+    RepState &S = D.rs[r];
+    double *ke = D.obs_keep + (size_t)kObsKeep * r;
+    if (threadIdx.x < 12) {
+        if (restore) S.e[threadIdx.x] = ke[threadIdx.x];
+        else ke[threadIdx.x] = S.e[threadIdx.x];
+    }
+    if (threadIdx.x == 12) {
+        if (restore) S.hb_iters = (int)ke[12];
+        else ke[12] = (double)S.hb_iters;
+    }
+    double *kst = D.obs_keep + (size_t)kObsKeep * D.n_rep;
+    for (int k = threadIdx.x; k < P.nst3; k += blockDim.x) {
+        const int g = P.st30 + k;
+        if (restore) D.st_energy[g] = kst[g];
+        else kst[g] = D.st_energy[g];
+    }
+    for (int k = threadIdx.x; k < P.nhb; k += blockDim.x) {
+        const int g = P.hb0 + k;
+        if (restore) D.hb_out[g] = D.obs_keep_hb[g];
+        else D.obs_keep_hb[g] = D.hb_out[g];
     }
 }
 

@@ -279,8 +279,13 @@ int main(int argc, char **argv) {
         return 0;
     };
     rc = run_deck("", klok_old + 1);
-    // deck 2 (DD.cpp:6405-6544) continues from the final state with klok = 1
-    if (!rc && a.deck2) rc = run_deck("1", 1);
+    // deck 2 (DD.cpp:6405-6544) continues from the final state with klok = 1; its
+    // Philox draws come from epoch 1, so the restarted step counter repeats none of
+    // deck 1's (the glibc stream just continues, as the reference's rand() does)
+    if (!rc && a.deck2) {
+        if ((rc = tis_set_rng_epoch(ctx, 0, 1))) rc = die("tis_set_rng_epoch", rc);
+        else rc = run_deck("1", 1);
+    }
     tis_destroy(ctx);
     tis_system_free(sys);
     return rc;

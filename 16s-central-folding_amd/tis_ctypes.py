@@ -162,6 +162,10 @@ def load_library(path=LIB_PATH):
         lib.tis_set_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_longlong]
         lib.tis_get_state.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         lib.tis_set_step_counter.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
+        lib.tis_set_rng_epoch.argtypes = [C.c_void_p, C.c_int, C.c_uint]
+        lib.tis_prepare_graph.argtypes = [C.c_void_p, C.c_int]
+        lib.tis_get_launch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                              C.POINTER(C.c_longlong)]
         lib.tis_step.argtypes = [C.c_void_p, C.c_longlong]
         lib.tis_get_energies.argtypes = [C.c_void_p, C.c_int, C.POINTER(Energies)]
         lib.tis_get_stack_energies.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
@@ -370,6 +374,17 @@ class Engine:
 
     def set_step_counter(self, klok, rep=0):
         check(self.lib, self.lib.tis_set_step_counter(self.ctx, rep, klok), "tis_set_step_counter")
+
+    def prepare_graph(self, graph_steps):
+        check(self.lib, self.lib.tis_prepare_graph(self.ctx, graph_steps), "tis_prepare_graph")
+
+    def launch_counts(self):
+        v = [C.c_longlong(0) for _ in range(3)]
+        check(self.lib, self.lib.tis_get_launch_counts(self.ctx, *[C.byref(x) for x in v]), "tis_get_launch_counts")
+        return {"graph_launches": v[0].value, "graph_steps": v[1].value, "eager_steps": v[2].value}
+
+    def set_rng_epoch(self, epoch, rep=-1):
+        check(self.lib, self.lib.tis_set_rng_epoch(self.ctx, rep, epoch), "tis_set_rng_epoch")
 
     def energies(self, rep=0):
         e = Energies()

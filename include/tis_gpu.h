@@ -259,10 +259,28 @@ int tis_get_state(tis_ctx *ctx, int rep, double *pos, double *pos_old, double *v
  * continue), as the second deck of the _DD driver does (DD.cpp:6428). */
 int tis_set_step_counter(tis_ctx *ctx, int rep, long long klok);
 
+/* Philox stream family of replica rep (rep = -1: every replica).  Draws are keyed
+ * by (seed, replica, epoch) and counted by the step counter klok, so a driver
+ * that restarts klok on the same context, as the second deck of the _DD driver
+ * does (klok = 1 again, DD.cpp:6428), moves to another epoch and no draw of the
+ * first deck repeats.  (The reference's single rand() stream just keeps
+ * advancing across decks.)  0 at creation; tis_set_state leaves it unchanged. */
+int tis_set_rng_epoch(tis_ctx *ctx, int rep, unsigned epoch);
+
 /* Advances every replica n_steps Langevin steps: the body of fast.cpp:6274-6387
  * without its output block (deterministic_forces_MPI, full_forces,
  * move_rigid_units, check_shifts). */
 int tis_step(tis_ctx *ctx, long long n_steps);
+
+/* Captures and instantiates the hipGraph of graph_steps steps that tis_step
+ * replays, without launching it, so that a later timed tis_step call holds no
+ * capture; graph_steps replaces the context's option (0: launch every step
+ * one by one).  No counterpart in the reference. */
+int tis_prepare_graph(tis_ctx *ctx, int graph_steps);
+/* Since creation: graph replays, steps run inside them, steps launched one by
+ * one (any pointer may be NULL). */
+int tis_get_launch_counts(tis_ctx *ctx, long long *graph_launches, long long *graph_steps,
+                          long long *eager_steps);
 
 /* Replica exchange (temperature ladder).  No counterpart in the reference,
  * whose replicas are separate MPI jobs; this is the on-device swap of the

@@ -136,10 +136,11 @@ def ref_fixtures(tmp):
 
 
 if __name__ == "__main__":
-    tmp = os.path.join(REPO, "_work")
-    os.makedirs(tmp, exist_ok=True)
+    import tempfile
+
     ninfo_fixtures()
-    ref_fixtures(tmp)
+    with tempfile.TemporaryDirectory(prefix="tis_fixtures_") as tmp:   # the harness's record files
+        ref_fixtures(tmp)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -41,7 +41,14 @@ def test_one_sample_matches_the_oracle(lib, system):
     eng = _engine(lib, d)
     eng.step(300)
     eng.observe(0)
+    before = (eng.energies(), eng.stack_energies(), eng.hb_outcome())
     eng.observe_now()
+    # the sample leaves what the last step's force call reported (Energies and
+    # Contacts file columns, HB outcomes) as it was
+    after = (eng.energies(), eng.stack_energies(), eng.hb_outcome())
+    for k in ("e2_hb", "e3_hb", "e2_stack", "e3_stack", "hb_rounds"):
+        assert after[0][k] == before[0][k], k
+    assert np.array_equal(after[1], before[1]) and np.array_equal(after[2], before[2])
     o = eng.observables()
     pos, _, _ = eng.get_state()
     counts, ps, bound, (rg, D, S) = _oracle_sample(d, pos)
@@ -58,10 +65,10 @@ def test_one_sample_matches_the_oracle(lib, system):
     assert o["asphericity"] == pytest.approx(D, rel=1e-9, abs=1e-12)
     assert o["shape"] == pytest.approx(S, rel=1e-9, abs=1e-12)
     # the sample's energies and outcomes are those of an evaluation of this state
+    _, e = eng.eval(T.TERM_ALL)
     kept = (eng.hb_outcome() == 1).astype(float)
     assert np.array_equal(o["hb_kept"], kept)
     assert np.array_equal(o["st3_mean"], eng.stack_energies())
-    _, e = eng.eval(T.TERM_ALL)
     for k in ("e2_hb", "e3_hb", "e2_stack", "e3_stack", "e_wca", "e_coul_sr", "e_coul_lr", "e_bond", "e_angle"):
         assert o[k] == pytest.approx(e[k], rel=1e-12, abs=1e-12), k
     eng.close()

@@ -120,6 +120,26 @@ def test_philox_run_is_stable(lib, sys1):
     eng.close()
 
 
+def test_rng_epoch_selects_fresh_draws(lib, sys1):
+    """Philox draws are keyed by (seed, replica, epoch) and counted by klok: a run
+    restarted at the same klok repeats its draws in the same epoch and draws
+    afresh in another (the _DD second deck restarts klok at 1, DD.cpp:6428)."""
+    d = sys1.d
+    state = [d[k].reshape(-1, 3) for k in ("pos", "pos_old", "vel")]
+    eng = T.Engine(lib, [sys1], precision=T.PREC_FP64, use_graph=0)
+    runs = []
+    for epoch in (0, 1, 0):
+        eng.set_rng_epoch(epoch)
+        eng.set_state(*state, klok=1)
+        eng.step(5)
+        runs.append(eng.get_state()[2])
+    same, other = np.abs(runs[2] - runs[0]).max(), np.abs(runs[1] - runs[0]).max()
+    print("same epoch", same, "other epoch", other)
+    assert same < 1e-9
+    assert other > 1e-3
+    eng.close()
+
+
 @pytest.mark.parametrize("prec", [T.PREC_FP64, T.PREC_MIXED])
 def test_one_three_pair_at_the_wca_singularity_stays_finite(lib, sys1, prec):
     """P 833 and B 835 (a 1-3 pair through S 834) placed 1e-4 A beyond the point
@@ -172,22 +192,55 @@ def test_batched_replicas_equal_single_runs(lib):
     batch.close()
 
 
-def test_high_salt_450A_lists_and_forces(lib):
-    """The 450 A, 20 mM MgCl2 + 100 mM KCl system (14,957 particles): pair sets
-    bit-exact and mixed-precision forces within 1e-5 of the oracle."""
+@pytest.fixture(scope="module")
+def high_salt(lib):
+    """The 450 A, 20 mM MgCl2 + 100 mM KCl system (14,957 particles) and the FP64
+    oracle at its set-up state."""
     d, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.100, seed=1, box=450.0)
     assert d["n_particles"] == 14957
     h = T.SystemHandle(d)
     o = O.Oracle(h)
     o.set_state(d["pos"], d["pos_old"], d["vel"])
+    return h, o
+
+
+def test_high_salt_450A_lists_and_status(lib, high_salt):
+    """Pair sets and HB candidate statuses bit-exact."""
+    h, o = high_salt
     eng = T.Engine(lib, [h], precision=T.PREC_MIXED)
     assert np.array_equal(eng.pairs(0), o.pairs(0))
     assert np.array_equal(eng.pairs(1), o.pairs(1))
-    mask = T.TERM_STACK | T.TERM_BACKBONE | T.TERM_SR | T.TERM_LR
+    assert np.array_equal(eng.hb_status(), o.hb_status())
+    eng.close()
+
+
+@pytest.mark.parametrize("name,mask", TERMS + [("hb", T.TERM_HB)])
+def test_high_salt_450A_per_term_mixed(lib, high_salt, name, mask):
+    """Per-term FP32 forces of the 450 A, 20 mM system against the FP64 oracle
+    (SR_/LR_interactions_lists fast.cpp:1783-1907, stacking_interactions
+    :1911-2141, polymer_constraints :2354-2520, hydrogen_bonds_interactions
+    :2145-2350): norm-wise within 1e-5 and max |dF| / RMS(F) within 1e-4.  The HB
+    term runs with the reference's glibc stream on both sides (same draw count,
+    so the valence resolution takes the same branch)."""
+    h, o = high_salt
+    d = h.d
+    stream = name == "hb"
+    eng = T.Engine(lib, [h], precision=T.PREC_MIXED, rng=T.RNG_STREAM if stream else T.RNG_PHILOX)
+    if stream:
+        raw = O.glibc_stream(4242, 20000)
+        eng.set_stream(raw)
+        o.set_state(d["pos"], d["pos_old"], d["vel"])
+        o.set_stream(raw)
     f, _ = eng.eval(mask)
     _, fo, _ = o.eval(mask)
-    print("rel", rel(f, fo))
-    assert rel(f, fo) < 1e-5
+    rms = np.sqrt(np.mean(np.sum(fo ** 2, axis=1)))
+    r_norm, r_max = rel(f, fo), np.abs(f - fo).max() / rms
+    print(f"high salt {name}: |dF|/|F| = {r_norm:.3e}, max|dF|/RMS(F) = {r_max:.3e}")
+    if stream:
+        assert eng.cursor() == o.cursor() and eng.cursor() > 0
+    assert np.linalg.norm(fo) > 0
+    assert r_norm < 1e-5
+    assert r_max < 1e-4
     eng.close()
 
 
