@@ -11,9 +11,16 @@ differ in ion count, so no exchange moves are
 attempted between them (an exchange needs two copies of one system).
 
 Printed: one JSON line.  value = aggregate replica-steps per second of the
-batch.  Beside it: the plain FP64 one-replica-per-launch path on the 3,659
-particle 5 mM system (no graph, FP64 forces), the roofline of the dominant
-kernel from HIP-event timings, and the FP64 CPU oracle timed on this host.
+batch, its timed steps replayed from a hipGraph captured before the timer
+(launch_mode says how the timed steps ran).  Beside it: the plain FP64
+one-replica-per-launch path on the 3,659 particle 5 mM system (no graph, FP64
+forces), the roofline of the dominant kernel from HIP-event timings, and the
+reference's own step loop (compiled by oracle/Makefile) on this host's cores:
+the 16 configurations as concurrent single-rank processes, one after another
+on one core, and one configuration under its own MPI decomposition.
+
+--workload highsalt measures one 16SCD replica at 20 mM MgCl2 + 100 mM KCl in a
+450 A box (14,957 particles) with the per-term error of its FP32 forces.
 
 --workload chain1500 measures the large synthetic chain instead: one replica
 of a seeded 1,500-nt random-sequence chain (self-avoiding walk in a sphere of
@@ -26,6 +33,7 @@ import argparse
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 import time
@@ -93,27 +101,106 @@ def batched_params(n_rep=16):
     return [(0.5e-3 * 40.0 ** (k / 15.0), 350.0 + 100.0 * k / 15.0, k + 1) for k in range(n_rep)]
 
 
-def cpu_baseline_reference(steps_each):
+REF_EXE = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+REF_DATA = os.path.join(REPO, "16s-central-folding_amd", "data", "16SCD")
+
+
+def _ref_cmd(c_mg, box, seed, steps, c_k=0.050):
+    return [REF_EXE, "-D0.850", "-E3.975", "-d6.5", "-T0.596", f"-c{c_mg:.10f}", f"-f{c_k:.3f}", f"-s{seed}",
+            f"-L{box:.6f}", f"-N{steps}", "-P"]
+
+
+def _ref_rate(out):
+    """steps/s of the reference's timed step loop from the harness's output line."""
+    m = re.search(r"steps (\d+) seconds ([0-9.eE+-]+)", out)
+    if not m:
+        raise RuntimeError("unexpected reference harness output: " + out[-200:])
+    return int(m.group(1)) / float(m.group(2))
+
+
+def host_cores():
+    """Host cores this process may use, at most 16 (the CPU share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_reference(steps_one, steps_all, configs=None):
     """The reference's own step sequence (its translation unit compiled by
-    oracle/Makefile into oracle/_ref/ref_harness) timed on one host core for
-    `steps_each` steps of each of the 16 replica configurations; replica-steps/s.
-    None when that binary was not built."""
-    exe = os.path.join(REPO, "oracle", "_ref", "ref_harness")
-    if not os.access(exe, os.X_OK):
+    oracle/Makefile into oracle/_ref/ref_harness, one MPI rank per process) on
+    the replica configurations `configs` ((c_Mg, box, seed, c_K); default the 16
+    of the batch).  Returns None when that binary was not built, else a dict:
+      one_core   replica-steps/s with the configurations run one after another
+                 on one core (`steps_one` steps each);
+      all_cores  replica-steps/s with every configuration its own single-rank
+                 process, all at once on up to host_cores() cores (`steps_all`
+                 steps each; the reference runs its replicas as separate jobs).
+                 With C >= n configurations it is the sum of the concurrent
+                 rates, with fewer cores C times their harmonic mean (the
+                 configurations packed onto C cores);
+      mpi        steps/s of the first configuration under the reference's own
+                 intra-replica decomposition (deterministic_forces_MPI,
+                 fast.cpp:2524-2586, MPI_Allreduce :2572) for 1, 2, 4, ... ranks
+                 (mpiexec of the installed MPICH), or None without mpiexec."""
+    if not os.access(REF_EXE, os.X_OK):
         return None
-    data = os.path.join(REPO, "16s-central-folding_amd", "data", "16SCD")
-    tot_steps, tot_s = 0, 0.0
-    for c_mg, box, seed in batched_params():
-        cmd = [exe, "-D0.850", "-E3.975", "-d6.5", "-T0.596", f"-c{c_mg:.10f}", "-f0.050", f"-s{seed}",
-               f"-L{box:.6f}", f"-N{steps_each}", "-P"]
-        out = subprocess.run(cmd, cwd=data, capture_output=True, text=True, timeout=600, check=True).stdout
-        m = re.search(r"steps (\d+) seconds ([0-9.eE+-]+)", out)
-        if not m:
-            raise RuntimeError("unexpected reference harness output: " + out[-200:])
-        tot_steps += int(m.group(1))
-        tot_s += float(m.group(2))
-        log(f"reference CPU: c_Mg {c_mg * 1e3:.2f} mM box {box:.1f} A, {m.group(1)} steps in {float(m.group(2)):.2f} s")
-    return tot_steps / tot_s
+    configs = configs or [(c, b, s, 0.050) for c, b, s in batched_params()]
+    run = lambda cmd: subprocess.run(cmd, cwd=REF_DATA, capture_output=True, text=True, timeout=900,
+                                     check=True).stdout
+    rates_one = []
+    for c_mg, box, seed, c_k in configs:
+        rates_one.append(_ref_rate(run(_ref_cmd(c_mg, box, seed, steps_one, c_k))))
+        log(f"reference CPU, one core: c_Mg {c_mg * 1e3:.2f} mM box {box:.1f} A: {rates_one[-1]:.0f} steps/s")
+    one_core = len(configs) / sum(1.0 / r for r in rates_one)
+    cores = host_cores()
+    rates_all = []
+    pending = list(configs)
+    while pending:
+        batch, pending = pending[:cores], pending[cores:]
+        procs = [subprocess.Popen(_ref_cmd(c_mg, box, seed, steps_all, c_k), cwd=REF_DATA, stdout=subprocess.PIPE,
+                                  stderr=subprocess.DEVNULL, text=True) for c_mg, box, seed, c_k in batch]
+        for p in procs:
+            out, _ = p.communicate(timeout=900)
+            if p.returncode != 0:
+                raise RuntimeError("reference harness failed")
+            rates_all.append(_ref_rate(out))
+    if cores >= len(configs):
+        all_cores = sum(rates_all)
+    else:
+        all_cores = cores * len(rates_all) / sum(1.0 / r for r in rates_all)
+    log(f"reference CPU, {min(cores, len(configs))} concurrent processes: {all_cores:.0f} replica-steps/s")
+    mpi = None
+    mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.access("/opt/conda/bin/mpiexec", os.X_OK)
+                                           else None)
+    if mpiexec:
+        mpi = {}
+        c_mg, box, seed, c_k = configs[0]
+        n = 1
+        while n <= cores:
+            try:
+                out = run([mpiexec, "-n", str(n)] + _ref_cmd(c_mg, box, seed, steps_one, c_k))
+                mpi[str(n)] = min(_ref_rate(line) for line in out.splitlines() if line.startswith("steps"))
+            except (subprocess.SubprocessError, RuntimeError, ValueError) as e:
+                log(f"mpiexec -n {n}: {e}")
+                break
+            log(f"reference CPU, mpiexec -n {n}: {mpi[str(n)]:.0f} steps/s")
+            n *= 2
+    return {"one_core": one_core, "all_cores": all_cores, "cores": min(cores, len(configs)), "mpi": mpi,
+            "steps_one": steps_one, "steps_all": steps_all}
+
+
+def reference_cpu_line(ref, what):
+    """cpu_baseline entry from cpu_baseline_reference's result."""
+    return {"value": ref["all_cores"], "unit": "steps/s", "cores": ref["cores"], "kind": "reference",
+            "sample": f"the reference's step loop (fast.cpp:6380-6386, g++ -O3, one MPI rank per process): "
+                      f"{what} as {ref['cores']} concurrent processes, {ref['steps_all']} steps each "
+                      f"(each replica its own job, as the reference runs its replicas)",
+            "one_core": {"value": ref["one_core"], "cores": 1,
+                         "sample": f"the same configurations one after another on one core, "
+                                   f"{ref['steps_one']} steps each"},
+            "mpi_ranks_one_replica": ref["mpi"]}
 
 
 def step_bytes(eng, handles):
@@ -195,14 +282,7 @@ def run_chain(args, lib):
     log(f"built the synthetic chain: {d['n_particles']} particles")
     h = T.SystemHandle(d)
     launch, tuning = tune(lib, [h], args)
-    eng = T.Engine(lib, [h], precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
-                   use_graph=0 if args.no_graph else 1, **launch)
-    eng.step(args.warmup)
-    log(f"warmup {args.warmup} steps done")
-    t0 = time.perf_counter()
-    eng.step(args.steps)
-    dt = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {dt:.2f} s")
+    eng, dt, mode = timed_run(lib, [h], launch, args)
     value = args.steps / dt
     kt, dominant, avg_ms, bytes_launch, achieved = {}, None, None, None, None
     if args.profile_steps > 0:
@@ -243,6 +323,7 @@ def run_chain(args, lib):
                                "stacks on ~60% of adjacent pairs, 400 HB candidates",
                    "replicas": 1, "particles": d["n_particles"], "precision": "mixed", "rng": "philox",
                    "launch": launch, "autotune": tuning},
+        "launch_mode": mode,
         "ns_per_day": value * FS_PER_STEP * 86400 * 1e-6,
         "roofline": None if dominant is None else {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -258,6 +339,80 @@ def run_chain(args, lib):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+TERM_NAMES = (("stack", T.TERM_STACK), ("backbone", T.TERM_BACKBONE), ("sr", T.TERM_SR), ("lr", T.TERM_LR),
+              ("hb", T.TERM_HB))
+
+
+def per_term_errors(lib, h, seed=2024):
+    """FP32 (mixed-precision) forces of each term against the FP64 engine at the
+    same state, same Philox draws (so the HB valence resolution takes the same
+    branch): |dF| / |F| and max |dF| / RMS(F).  The FP64 engine equals the FP64
+    CPU oracle to 1e-10 per term (tests/test_gpu_parity.py)."""
+    e64 = T.Engine(lib, [h], precision=T.PREC_FP64, rng=T.RNG_PHILOX, seed=seed, use_graph=0)
+    e32 = T.Engine(lib, [h], precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=seed, use_graph=0)
+    out = {}
+    for name, mask in TERM_NAMES:
+        f64, _ = e64.eval(mask)
+        f32, _ = e32.eval(mask)
+        rms = float(np.sqrt(np.mean(np.sum(f64 ** 2, axis=1))))
+        out[name] = {"rel_norm": float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64)),
+                     "max_over_rms": float(np.abs(f32 - f64).max() / rms)}
+    e64.close()
+    e32.close()
+    return out
+
+
+def run_highsalt(args, lib):
+    """One replica of the 16SCD chain at high salt: 450 A box, 20 mM MgCl2 + 100
+    mM KCl (14,957 particles), FP32 forces / FP64 state, Philox noise; steps/s,
+    ns/day and the per-term force error of the FP32 arithmetic."""
+    d, _ = T.build_16scd(lib, T=0.596, c_mg=0.020, c_k=0.100, seed=1, box=450.0)
+    h = T.SystemHandle(d)
+    log(f"built the high-salt system: {d['n_particles']} particles")
+    errs = per_term_errors(lib, h)
+    log(f"per-term FP32 force error: {errs}")
+    launch, tuning = tune(lib, [h], args)
+    eng, dt, mode = timed_run(lib, [h], launch, args)
+    value = args.steps / dt
+    kt = {}
+    if args.profile_steps > 0:
+        eng.set_profiling(True)
+        eng.step(args.profile_steps)
+        kt = eng.kernel_times()
+        eng.set_profiling(False)
+    b_step = step_bytes(eng, [h])
+    eng.close()
+    cpu = None
+    if not args.no_cpu:
+        ref = cpu_baseline_reference(10 * args.cpu_steps, 10 * args.cpu_steps, [(0.020, 450.0, 1, 0.100)])
+        if ref is not None:
+            best = max([("1", ref["one_core"])] + list((ref["mpi"] or {}).items()), key=lambda kv: kv[1])
+            cpu = {"value": best[1], "unit": "steps/s", "cores": int(best[0]), "kind": "reference",
+                   "sample": f"the reference's step loop (fast.cpp:6380-6386, g++ -O3) on this system, "
+                             f"{ref['steps_one']} steps, the fastest of one rank and mpiexec -n 2, 4, ... of its "
+                             f"own MPI decomposition",
+                   "one_core": {"value": ref["one_core"], "cores": 1},
+                   "mpi_ranks_one_replica": ref["mpi"]}
+    print(json.dumps({
+        "metric": "Langevin steps/s, 16SCD at 20 mM MgCl2 + 100 mM KCl (450 A), single replica",
+        "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic: 16SCD chain from the reference PDB, ions placed by the reference's seeded rule",
+        "config": {"workload": "16SCD, 450 A box, 20 mM MgCl2 + 100 mM KCl, T=0.596", "replicas": 1,
+                   "particles": d["n_particles"], "precision": "mixed", "rng": "philox", "launch": launch,
+                   "autotune": tuning},
+        "launch_mode": mode,
+        "ns_per_day": value * FS_PER_STEP * 86400 * 1e-6,
+        "per_term_force_error_fp32_vs_fp64": errs,
+        "roofline": None,
+        "roofline_step": {"bound": "hbm", "bytes_per_step": b_step, "achieved": b_step * value / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
+        "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }))
 
 
 def run_force_match(args, lib, n_snap=256):
@@ -307,7 +462,7 @@ def run_force_match(args, lib, n_snap=256):
         "metric": "force-matching iterations/s (loss + gradients, 256 snapshots of 16SCD at 5 mM)",
         "value": value, "unit": "iterations/s", "n_gpus": 1, "steps": iters, "warmup": max(1, args.warmup // 100),
         "ms_per_step": dt / iters * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 forces and tangents / f64 sums",
+        "dtype": "f32 forces / f64 tangents and sums",
         "data": "synthetic: 16SCD beads from the reference PDB + N(0, 0.5^2) noise, ions by the reference's rule",
         "config": {"workload": "256 snapshots x 3,659 particles, targets from parameters scaled by 1 +- 10 %",
                    "snapshots": n_snap, "particles": n, "precision": "mixed", "rng": "philox"},
@@ -356,6 +511,35 @@ def run_remd(args, lib, n_rep=16, every=1000):
     }))
 
 
+def timed_run(lib, handles, launch, args, **engine_kw):
+    """Warm-up, then exactly args.steps timed steps of an Engine over `handles`.
+    The hipGraph the timed steps replay (its length the tuned one, or the step
+    count when that is shorter) is captured before the timer starts; the launch
+    counts of the timed call say how the steps ran.  Returns (engine, seconds,
+    launch mode)."""
+    launch = dict(launch)
+    use_graph = 0 if args.no_graph else 1
+    if use_graph:
+        launch["graph_steps"] = max(1, min(launch["graph_steps"], args.steps))
+    kw = dict(precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024)
+    kw.update(engine_kw)
+    eng = T.Engine(lib, handles, use_graph=use_graph, **kw, **launch)
+    eng.step(args.warmup)
+    log(f"warmup {args.warmup} steps done")
+    if use_graph:
+        eng.prepare_graph(launch["graph_steps"])
+    c0 = eng.launch_counts()
+    t0 = time.perf_counter()
+    eng.step(args.steps)          # returns after a stream synchronize
+    dt = time.perf_counter() - t0
+    c1 = eng.launch_counts()
+    mode = {k: c1[k] - c0[k] for k in c0}
+    mode["graph_length"] = launch["graph_steps"] if use_graph else 0
+    mode["mode"] = "graph" if mode["eager_steps"] == 0 else ("eager" if mode["graph_steps"] == 0 else "mixed")
+    log(f"timed {args.steps} steps in {dt:.3f} s ({mode})")
+    return eng, dt, mode
+
+
 def tune(lib, handles, args):
     """Launch shapes for the timed run: tis_autotune's choice on a scratch copy of
     the batch (its trials are outside the timed region), or the defaults."""
@@ -378,12 +562,16 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched16", "chain1500", "forcematch256", "remd16"], default="batched16")
+    ap.add_argument("--workload", choices=["batched16", "chain1500", "forcematch256", "remd16", "highsalt"],
+                    default="batched16")
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=250)
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--plain-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=200, help="CPU steps per replica for cpu_baseline")
+    ap.add_argument("--cpu-steps", type=int, default=200,
+                    help="CPU steps per replica configuration, one core (cpu_baseline.one_core)")
+    ap.add_argument("--cpu-steps-all", type=int, default=1000,
+                    help="CPU steps per replica configuration, concurrent processes (cpu_baseline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches (for counter collection)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of tis_autotune's")
@@ -410,18 +598,14 @@ def main():
     if args.workload == "remd16":
         run_remd(args, lib)
         return
+    if args.workload == "highsalt":
+        run_highsalt(args, lib)
+        return
     systems = batched_systems(lib)
     log(f"built {len(systems)} replicas")
     handles = [T.SystemHandle(d) for d in systems]
     launch, tuning = tune(lib, handles, args)
-    eng = T.Engine(lib, handles, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024,
-                   use_graph=0 if args.no_graph else 1, **launch)
-    eng.step(args.warmup)
-    log(f"warmup {args.warmup} steps done")
-    t0 = time.perf_counter()
-    eng.step(args.steps)          # returns after a stream synchronize
-    dt = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {dt:.2f} s")
+    eng, dt, mode = timed_run(lib, handles, launch, args)
     n_rep = len(handles)
     value = n_rep * args.steps / dt
     # per-kernel device times over a separate event-timed run of the same batch
@@ -453,11 +637,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg")
-        v = cpu_baseline_reference(args.cpu_steps)
-        if v is not None:
-            cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "reference",
-                   "sample": f"the reference's step loop (fast.cpp:6380-6386, g++ -O3, one MPI rank), "
-                             f"{args.cpu_steps} steps of each of the 16 replica configurations, one core"}
+        ref = cpu_baseline_reference(args.cpu_steps, args.cpu_steps_all)
+        if ref is not None:
+            cpu = reference_cpu_line(ref, "the 16 replica configurations")
         else:
             v = cpu_baseline(systems, args.cpu_steps)
             cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
@@ -480,6 +662,7 @@ def main():
                                "box 350+100k/15 A, T=0.596, independent (no exchange)",
                    "replicas": n_rep, "particles": int(sum(d["n_particles"] for d in systems)),
                    "precision": "mixed", "rng": "philox", "launch": launch, "autotune": tuning},
+        "launch_mode": mode,
         "ns_per_day_aggregate": value * FS_PER_STEP * 86400 * 1e-6,
         "plain_fp64_single": None if plain is None else {
             "value": plain, "unit": "steps/s", "particles": d1["n_particles"],
