@@ -16,9 +16,11 @@
 
 namespace tis {
 
-// v = F - F^target per particle into D.jv, and sum |v|^2 into *loss.
+// v = F - F^target per particle into D.jv, and sum |v|^2 into *loss.  The forces
+// go to forces (double4 per particle) and / or forces3 (packed xyz); either may
+// be null.
 __global__ void __launch_bounds__(kBlock) k_fm_residual(Dev D, const double *target, double4 *forces,
-                                                          double *loss) {
+                                                          double *forces3, double *loss) {
     __shared__ double red[8];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
@@ -26,7 +28,12 @@ __global__ void __launch_bounds__(kBlock) k_fm_residual(Dev D, const double *tar
         const double4 a = D.f_sr[p], b = D.f_lr[p], c = D.f_mb[p];
         const double tx = target[3 * (size_t)p], ty = target[3 * (size_t)p + 1], tz = target[3 * (size_t)p + 2];
         const double fx = a.x + b.x + c.x, fy = a.y + b.y + c.y, fz = a.z + b.z + c.z;
-        forces[p] = make_double4(fx, fy, fz, 0.0);
+        if (forces) forces[p] = make_double4(fx, fy, fz, 0.0);
+        if (forces3) {
+            forces3[3 * (size_t)p] = fx;
+            forces3[3 * (size_t)p + 1] = fy;
+            forces3[3 * (size_t)p + 2] = fz;
+        }
         const double vx = fx - tx, vy = fy - ty, vz = fz - tz;
         D.jv[p] = make_double4(vx, vy, vz, 0.0);
         s = vx * vx + vy * vy + vz * vz;
@@ -109,10 +116,19 @@ __global__ void __launch_bounds__(kBlock) k_jvp_pairs(Dev D, double *grad_ion_r)
     }
 }
 
+// Per-term gradient value of term g of replica r: stored at its global index, or
+// (local) added at its index within the replica, which sums replicas of one system.
+TIS_DEV void put_term_grad(double *dst, int g, int g0, bool local, double v) {
+    if (local) unsafeAtomicAdd(&dst[g - g0], v);
+    else dst[g] = v;
+}
+
 // Stacks (one workgroup per replica, after the tertiary-formation flags of the
 // step) and bonded terms (one thread per term) with derivatives: tangents into
-// D.jf; per stack, v . F_stack / E_stack into g_st3 / g_st2 (global stack index).
-__global__ void __launch_bounds__(kBlock) k_jvp_multibody(Dev D, int nb_bonded, double *g_st3, double *g_st2) {
+// D.jf; per stack, v . F_stack / E_stack into g_st3 / g_st2 (global stack index,
+// or summed at the replica-local index when local).
+__global__ void __launch_bounds__(kBlock) k_jvp_multibody(Dev D, int nb_bonded, double *g_st3, double *g_st2,
+                                                            int local) {
     if ((int)blockIdx.x < D.n_rep) {
         const int r = blockIdx.x;
         const RepParams &P = D.rp[r];
@@ -135,12 +151,12 @@ __global__ void __launch_bounds__(kBlock) k_jvp_multibody(Dev D, int nb_bonded, 
             if (k < P.nst3) {
                 const int g = P.st30 + k;
                 tertiary_stack<Dual>(D, D.jf, g, &vd);
-                g_st3[g] = vd / D.st3_E[g];
+                put_term_grad(g_st3, g, P.st30, local, vd / D.st3_E[g]);
             } else {
                 const int m = k - P.nst3, g = P.st20 + m;
-                if (off[m]) { g_st2[g] = 0.0; continue; }
+                if (off[m]) { put_term_grad(g_st2, g, P.st20, local, 0.0); continue; }
                 secondary_stack<Dual>(D, D.jf, g, &vd);
-                g_st2[g] = vd / D.st2_rE[g].y;
+                put_term_grad(g_st2, g, P.st20, local, vd / D.st2_rE[g].y);
             }
         }
         return;
@@ -174,8 +190,9 @@ __global__ void __launch_bounds__(kBlock) k_jvp_multibody(Dev D, int nb_bonded, 
 }
 
 // HB candidates kept by the resolution, with derivatives: tangents into D.jf and
-// v . F_hb / hb_E into g_hb (global candidate index).  Grid (kActMax / kBlock, replicas).
-__global__ void __launch_bounds__(kBlock) k_jvp_hb(Dev D, double *g_hb) {
+// v . F_hb / hb_E into g_hb (global candidate index, or summed at the
+// replica-local index when local).  Grid (kActMax / kBlock, replicas).
+__global__ void __launch_bounds__(kBlock) k_jvp_hb(Dev D, double *g_hb, int local) {
     const int r = blockIdx.y;
     const RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
@@ -188,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) k_jvp_hb(Dev D, double *g_hb) {
     const Dual e = Dual(D.hb_E[g]) * exp(-q);
     double vd = 0.0;
     sextet_forces<Dual>(D, D.jf, D.hb_b + 6 * g, D.hb_g + 6 * g, -e, psi0, Dual(5.0), Dual(1.5), Dual(0.15), &vd);
-    g_hb[g] = vd / D.hb_E[g];
+    put_term_grad(g_hb, g, P.hb0, local, vd / D.hb_E[g]);
 }
 
 }  // namespace tis
@@ -215,6 +232,27 @@ __global__ void __launch_bounds__(kBlock) k_sum_replicas(const double *src, int 
     double s = 0.0;
     for (int r = 0; r < n_rep; ++r) s += src[(size_t)r * n + k];
     dst[k] = scale * s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_zero(double *dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = 0.0;
+}
+
+// Outputs of the allocation-free force matching from its working sums: s[0] sum
+// |v|^2, s[1..3] ion-radius sums, and the per-term sums shb, sst3, sst2 of the n_hb,
+// n_st3, n_st2 terms of one replica; the gradients are scaled by sc = 2 / M and
+// the loss by 1 / M.  Null outputs are skipped.
+__global__ void __launch_bounds__(kBlock) k_fm_finish(const double *s, const double *shb, const double *sst3,
+                                                       const double *sst2, int n_hb, int n_st3, int n_st2, double sc,
+                                                       double inv_m, double *loss, double *g_ion, double *g_hb,
+                                                       double *g_st3, double *g_st2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && loss) loss[0] = s[0] * inv_m;
+    if (k < 3 && g_ion) g_ion[k] = sc * s[1 + k];
+    if (k < n_hb && g_hb) g_hb[k] = sc * shb[k];
+    if (k < n_st3 && g_st3) g_st3[k] = sc * sst3[k];
+    if (k < n_st2 && g_st2) g_st2[k] = sc * sst2[k];
 }
 
 // Parameters of replica 0's region copied to every replica (replicas of one

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/tis_gpu.h"
@@ -54,6 +55,34 @@ void upload(T *dst, const std::vector<T> &src) {
     if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
+}  // namespace
+
+// Working memory of tis_force_match_noalloc, in the code object so that the
+// entry point allocates nothing: its loss and ion-radius sums and the per-term
+// gradient sums (4 + n_hb + n_st3 + n_st2 doubles, at most kScratchDoubles,
+// 1 MiB).  Each device holds its own copy until the process ends; host code
+// reaches it only through the address scratch_on_device() looked up for that
+// device (the name itself is a host-side stand-in).
+constexpr long long kScratchDoubles = 1LL << 17;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (the current device), looked up
+// once per device.
+double *scratch_on_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw DeviceError{"device index out of range for the scratch table"};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        double *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
 }  // namespace
 
 struct tis_ctx {
@@ -101,6 +130,7 @@ struct tis_ctx {
     double4 *fm_forces = nullptr;
     double *fm_t3 = nullptr, *fm_out3 = nullptr, *fm_small = nullptr;   // packed target / outputs, parameter staging
     double *fm_loss = nullptr, *fm_ion = nullptr, *fm_ghb = nullptr, *fm_gst3 = nullptr, *fm_gst2 = nullptr;
+    double *scratch_dev = nullptr;   // this device's `scratch` (tis_force_match_noalloc)
 
     ~tis_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -313,6 +343,33 @@ struct tis_ctx {
         timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
     }
 
+    // Force matching on stream st (tis_force_match, tis_force_match_noalloc):
+    // forward, the step's forces with a fresh long-range term; residual v = F -
+    // target into D.jv with its squared sum added to *loss; backward, force
+    // tangents along v into D.jf and the parameter sums (ion radii into ion[0..2],
+    // per-term values at global indices, or summed at replica-local ones when
+    // local).  Multi-body forces are cleared at the end.
+    void launch_force_match(hipStream_t st, const double *target, double4 *forces4, double *forces3, double *loss,
+                            double *ion, double *ghb, double *gst3, double *gst2, int local) {
+        const int N = D.n_tot;
+        k_zero_fmb<<<grid(N), kBlock, 0, st>>>(D);
+        launch_hb(st, 0);
+        const int nmb = n_rep + nb_bonded, npr = pair_grid();
+        if (D.precision == TIS_PREC_FP64) {
+            k_force<double, 0><<<nmb, kBlock, 0, st>>>(D, nb_bonded, 31u, kEvalForce);
+            k_force<double, 1><<<npr, pair_block, 0, st>>>(D, nb_bonded, 31u, kEvalForce);
+        } else {
+            k_force<float, 0><<<nmb, kBlock, 0, st>>>(D, nb_bonded, 31u, kEvalForce);
+            k_force<float, 1><<<npr, pair_block, 0, st>>>(D, nb_bonded, 31u, kEvalForce);
+        }
+        k_fm_residual<<<grid(N), kBlock, 0, st>>>(D, target, forces4, forces3, loss);
+        k_jvp_pairs<<<grid(kJvpLanes * N), kBlock, 0, st>>>(D, ion);
+        k_jvp_multibody<<<nmb, kBlock, 0, st>>>(D, nb_bonded, gst3, gst2, local);
+        k_jvp_hb<<<dim3((kActMax + kBlock - 1) / kBlock, n_rep), kBlock, 0, st>>>(D, ghb, local);
+        k_zero_fmb<<<grid(N), kBlock, 0, st>>>(D);
+        HIP_TRY(hipGetLastError());
+    }
+
     RepState get_rs(int r) {
         RepState s;
         HIP_TRY(hipMemcpy(&s, D.rs + r, sizeof s, hipMemcpyDeviceToHost));
@@ -321,8 +378,9 @@ struct tis_ctx {
     void put_rs(int r, const RepState &s) {
         HIP_TRY(hipMemcpy(D.rs + r, &s, sizeof s, hipMemcpyHostToDevice));
     }
+    std::vector<RepState> rs_host;    // n_rep, sized at creation (check_flags allocates nothing)
     void check_flags() {
-        std::vector<RepState> s(n_rep);
+        std::vector<RepState> &s = rs_host;
         HIP_TRY(hipMemcpy(s.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
         for (int r = 0; r < n_rep; ++r) {
             if (s[r].overflow & 1) throw Fail{TIS_EOVERFLOW, "neighbour slot overflow (more SR/LR partners than the slot set up at creation) in replica " + std::to_string(r)};
@@ -485,6 +543,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
     if (!c) return fail(Fail{TIS_ENOMEM, "tis_create: out of memory"});
     try {
         c->n_rep = n_rep;
+        c->rs_host.resize(n_rep);
         if (opt) c->opt = *opt;
         else tis_default_options(&c->opt);
         const tis_options &o = c->opt;
@@ -498,6 +557,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         HIP_TRY(hipGetDeviceCount(&dev_count));
         if (dev_count <= 0) throw Fail{TIS_EDEVICE, "no HIP device"};
         HIP_TRY(hipGetDevice(&c->dev_id));
+        c->scratch_dev = scratch_on_device(c->dev_id);
         HIP_TRY(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&c->s_hb, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&c->s_mb, hipStreamNonBlocking));
@@ -1136,24 +1196,8 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
         HIP_TRY(hipMemsetAsync(c->fm_loss, 0, sizeof(double), st));
         HIP_TRY(hipMemsetAsync(c->fm_ion, 0, 3 * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(c->fm_ghb, 0, std::max<size_t>(1, c->h_hbE.size()) * sizeof(double), st));
-        // forward: the step's forces with a fresh long-range term
-        k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
-        c->launch_hb(st, 0);
-        const int nmb = c->n_rep + c->nb_bonded, npr = c->pair_grid();
-        if (D.precision == TIS_PREC_FP64) {
-            k_force<double, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-            k_force<double, 1><<<npr, c->pair_block, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-        } else {
-            k_force<float, 0><<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-            k_force<float, 1><<<npr, c->pair_block, 0, st>>>(D, c->nb_bonded, 31u, kEvalForce);
-        }
-        k_fm_residual<<<c->grid(N), kBlock, 0, st>>>(D, c->fm_t3, c->fm_forces, c->fm_loss);
-        // backward: force tangents along v = F - target, parameter sums
-        k_jvp_pairs<<<c->grid(kJvpLanes * N), kBlock, 0, st>>>(D, c->fm_ion);
-        k_jvp_multibody<<<nmb, kBlock, 0, st>>>(D, c->nb_bonded, c->fm_gst3, c->fm_gst2);
-        k_jvp_hb<<<dim3((kActMax + kBlock - 1) / kBlock, c->n_rep), kBlock, 0, st>>>(D, c->fm_ghb);
-        k_zero_fmb<<<c->grid(N), kBlock, 0, st>>>(D);
-        HIP_TRY(hipGetLastError());
+        c->launch_force_match(st, c->fm_t3, c->fm_forces, nullptr, c->fm_loss, c->fm_ion, c->fm_ghb, c->fm_gst3,
+                              c->fm_gst2, 0);
         HIP_TRY(hipStreamSynchronize(st));
         c->check_flags();
         const double M = 3.0 * N, sc = 2.0 / M;
@@ -1188,6 +1232,48 @@ extern "C" int tis_force_match(tis_ctx *c, const double *target, double *loss, d
             HIP_TRY(hipMemcpy(g3, c->fm_ion, 3 * sizeof(double), hipMemcpyDeviceToHost));
             for (int t = 0; t < 3; ++t) grad_ion_R[t] = sc * g3[t];
         }
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_force_match_noalloc(tis_ctx *c, const double *target, double *loss, double *forces,
+                                       double *grad_pos, double *grad_hb_E, double *grad_st3_E, double *grad_st2_E,
+                                       double *grad_ion_R) {
+    if (!c || !target || !loss) return fail(Fail{TIS_EINVAL, "tis_force_match_noalloc: bad arguments"});
+    if ((grad_hb_E || grad_st3_E || grad_st2_E) && !c->all_same)
+        return fail(Fail{TIS_EINVAL, "tis_force_match_noalloc: per-term gradients need replicas of one system"});
+    // sums at replica-local term indices: regions as large as the largest
+    // replica's term counts (per-term outputs need one system, where all agree)
+    const RepParams &P0 = c->rp[0];
+    int mhb = 0, mst3 = 0, mst2 = 0;
+    for (const RepParams &q : c->rp) {
+        mhb = std::max(mhb, q.nhb);
+        mst3 = std::max(mst3, q.nst3);
+        mst2 = std::max(mst2, q.nst2);
+    }
+    const long long need = 4LL + mhb + mst3 + mst2;
+    if (need > kScratchDoubles)
+        return fail(Fail{TIS_EINVAL, "tis_force_match_noalloc: " + std::to_string(need) +
+                                     " working sums (4 + HB candidates + stacks) exceed the " +
+                                     std::to_string(kScratchDoubles) + " of the device scratch"});
+    return guarded([&]() -> int {
+        Dev &D = c->D;
+        const int N = D.n_tot;
+        hipStream_t st = c->s_main;
+        double *w = c->scratch_dev;
+        // every call starts from zero sums (the array keeps the last call's values)
+        k_fill_zero<<<(int)((need + kBlock - 1) / kBlock), kBlock, 0, st>>>(w, need);
+        double *whb = w + 4, *wst3 = whb + mhb, *wst2 = wst3 + mst3;
+        c->launch_force_match(st, target, nullptr, forces, w, w + 1, whb, wst3, wst2, 1);
+        const double M = 3.0 * N, sc = 2.0 / M;
+        if (grad_pos) k_pack3<<<c->grid(N), kBlock, 0, st>>>(D.jf, N, sc, grad_pos);
+        const int nfin = std::max({3, P0.nhb, P0.nst3, P0.nst2});
+        k_fm_finish<<<(nfin + kBlock - 1) / kBlock, kBlock, 0, st>>>(w, whb, wst3, wst2, P0.nhb, P0.nst3, P0.nst2, sc,
+                                                                      1.0 / M, loss, grad_ion_R, grad_hb_E, grad_st3_E,
+                                                                      grad_st2_E);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));
+        c->check_flags();
         return TIS_OK;
     });
 }

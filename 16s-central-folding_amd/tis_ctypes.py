@@ -185,6 +185,7 @@ def load_library(path=LIB_PATH):
                                                C.c_int]
         lib.tis_get_walkers.argtypes = [C.c_void_p, _ip, C.c_int]
         lib.tis_force_match.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.tis_force_match_noalloc.argtypes = [C.c_void_p] + [C.c_void_p] * 8
         lib.tis_set_force_field.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         lib.tis_autotune.argtypes = [C.POINTER(TisSystem), C.c_int, C.POINTER(Options), C.c_int, C.POINTER(Tuning)]
         lib.tis_observe.argtypes = [C.c_void_p, C.c_longlong, C.c_double]

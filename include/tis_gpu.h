@@ -328,6 +328,19 @@ int tis_get_walkers(tis_ctx *ctx, int *walker, int n_rep);
  * current state (tis_set_state rebuilds them). */
 int tis_force_match(tis_ctx *ctx, const double *target, double *loss, double *forces, double *grad_pos,
                     double *grad_hb_E, double *grad_st3_E, double *grad_st2_E, double *grad_ion_R);
+/* tis_force_match without any allocation.  No counterpart in the reference; for
+ * callers that cannot allocate device memory at the call (serving loops, code
+ * that forbids hipMalloc).  Every pointer is DEVICE memory the caller owns, laid
+ * out as for tis_force_match (target, forces, grad_pos 3 * N_total doubles; loss
+ * 1; per-term gradients n_hb, n_st3, n_st2 of one replica; grad_ion_R 3); target
+ * and loss are required, the others may be NULL.  The working sums live in a
+ * fixed array of the library's code object, 1 MiB per device (4 + n_hb + n_st3 +
+ * n_st2 doubles at most 131,072, else TIS_EINVAL), held until the process ends
+ * and shared by every context on that device: calls that overlap on one device
+ * race.  The call returns after its work on the context's stream is done. */
+int tis_force_match_noalloc(tis_ctx *ctx, const double *target, double *loss, double *forces, double *grad_pos,
+                            double *grad_hb_E, double *grad_st3_E, double *grad_st2_E, double *grad_ion_R);
+
 /* Replaces force-field parameters in every replica: HB strengths (n_hb), tertiary
  * and secondary stack energies (n_st3, n_st2) and the LJ radii of Mg, Cl, K (which
  * also set their friction, gamma = 0.555 R, as read_maxi_key does, fast.cpp:2897).
