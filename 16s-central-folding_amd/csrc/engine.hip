@@ -100,8 +100,26 @@ struct tis_ctx {
     std::vector<long long> stream_cap;
     long long stream_total = 0;
     bool graph_ok = false;
-    hipGraphExec_t graph = nullptr;
+    // one captured graph per parity of the state buffers it starts from (a graph
+    // of an odd number of steps ends on the other parity)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_steps = 0;
+    // state buffers: two of each, `par` the one holding the current state; every
+    // step writes the other and the host swaps D's pointers after launching it
+    double4 *buf_pos = nullptr, *buf_old = nullptr, *buf_vel = nullptr, *buf_fmb = nullptr;
+    int par = 0;
+    void set_parity(int p) {
+        par = p;
+        const size_t n = (size_t)D.n_tot;
+        D.pos = buf_pos + p * n; D.pos_nx = buf_pos + (1 - p) * n;
+        D.old = buf_old + p * n; D.old_nx = buf_old + (1 - p) * n;
+        D.vel = buf_vel + p * n; D.vel_nx = buf_vel + (1 - p) * n;
+        D.f_mb = buf_fmb + p * n; D.f_mb_nx = buf_fmb + (1 - p) * n;
+    }
+    void drop_graphs() {
+        for (hipGraphExec_t &g : graph)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
     // launches of tis_step: graph replays, steps replayed from graphs, steps
     // launched one by one
     long long n_graph_launch = 0, n_graph_step = 0, n_eager_step = 0;
@@ -133,7 +151,7 @@ struct tis_ctx {
     double *scratch_dev = nullptr;   // this device's `scratch` (tis_force_match_noalloc)
 
     ~tis_ctx() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        drop_graphs();
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         for (void *p : owned) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -228,14 +246,21 @@ struct tis_ctx {
     // The groups' energy sums and serial depths are reduced by k_hbsum, or in a
     // step (mode bit 0) by k_commit.
     void launch_hb(hipStream_t st, int mode) {
+        launch_hb_energy(st);
+        launch_hb_resolution(st, mode);
+    }
+    void launch_hb_energy(hipStream_t st) {
+        const dim3 ge((kActMax + kBlock - 1) / kBlock, n_rep);
+        timed("k_hbenergy", st, [&] {
+            if (D.precision == TIS_PREC_FP64) k_hbenergy<double><<<ge, kBlock, 0, st>>>(D);
+            else k_hbenergy<float><<<ge, kBlock, 0, st>>>(D);
+        });
+    }
+    void launch_hb_resolution(hipStream_t st, int mode) {
         timed("k_hbres", st, [&] {
             if (D.precision == TIS_PREC_FP64) k_hbres<double><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
             else k_hbres<float><<<dim3(hb_groups, n_rep), kResBlock, kHbLds, st>>>(D, mode);
         });
-        if (D.precision == TIS_PREC_FP64)
-            timed("k_hbforce64", st, [&] {
-                k_hbforce64<<<dim3((kActMax + kBlock - 1) / kBlock, n_rep), kBlock, 0, st>>>(D);
-            });
         if (!(mode & 1)) k_hbsum<<<grid(n_rep), kBlock, 0, st>>>(D);
     }
 
@@ -317,9 +342,10 @@ struct tis_ctx {
     void launch_step() {
         HIP_TRY(hipEventRecord(ev_fork, s_main));
         HIP_TRY(hipStreamWaitEvent(s_hb, ev_fork, 0));
-        HIP_TRY(hipStreamWaitEvent(s_mb, ev_fork, 0));
         if (!D.hb_lag) launch_rebuild_hb(s_hb, 0);
-        launch_hb(s_hb, 1);
+        HIP_TRY(hipStreamWaitEvent(s_mb, ev_fork, 0));
+        launch_hb_energy(s_hb);
+        launch_hb_resolution(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
             const int nmb = n_rep * D.st_slices + nb_bonded;
@@ -340,7 +366,8 @@ struct tis_ctx {
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));
-        timed("k_commit", s_main, [&] { k_commit<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
+        timed("k_commit", s_main, [&] { k_commit<<<1, kBlock, 0, s_main>>>(D); });
+        set_parity(par ^ 1);   // the new state is in the other buffers
     }
 
     // Force matching on stream st (tis_force_match, tis_force_match_noalloc):
@@ -608,12 +635,13 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         auto &own = c->owned;
         D.rp = dalloc<RepParams>(own, n_rep);
         D.rs = dalloc<RepState>(own, n_rep);
-        D.pos = dalloc<double4>(own, 2 * (size_t)N);
-        D.old = dalloc<double4>(own, 2 * (size_t)N);
-        D.vel = dalloc<double4>(own, 2 * (size_t)N);
+        c->buf_pos = dalloc<double4>(own, 2 * (size_t)N);
+        c->buf_old = dalloc<double4>(own, 2 * (size_t)N);
+        c->buf_vel = dalloc<double4>(own, 2 * (size_t)N);
+        c->buf_fmb = dalloc<double4>(own, 2 * (size_t)N);
+        c->set_parity(0);
         D.f_sr = dalloc<double4>(own, N);
         D.f_lr = dalloc<double4>(own, N);
-        D.f_mb = dalloc<double4>(own, N);
         D.ptype = dalloc<int>(own, N);
         D.prep = dalloc<int>(own, N);
         D.cell_of = dalloc<int>(own, N);
@@ -776,6 +804,9 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         const size_t nhb_tot = hbE.size(), nsite_tot = sv.size();
         D.hb_status = dalloc<signed char>(own, nhb_tot);
         D.hb_out = dalloc<signed char>(own, nhb_tot);
+        D.hb_en = dalloc<double>(own, (size_t)kActMax * n_rep);
+        D.hb_gate = dalloc<unsigned char>(own, (size_t)kActMax * n_rep);
+        D.hb_frc = dalloc<double>(own, 18 * (size_t)kActMax * n_rep);
         D.hb_iter_part = dalloc<int>(own, (size_t)kMaxHbGroups * n_rep);
         D.hb_e_part = dalloc<double>(own, 2 * (size_t)kMaxHbGroups * n_rep);
         D.tab_hb = (long long)nhb_tot + n_rep;
@@ -784,7 +815,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.act_hb = dalloc<int>(own, 2 * (size_t)D.tab_hb);
         D.act_tmp = dalloc<int>(own, nhb_tot);
         D.act_flag = dalloc<unsigned char>(own, nhb_tot);
-        D.asite_id = dalloc<int>(own, 2 * (size_t)D.tab_site);
+        D.asite_val = dalloc<int>(own, 2 * (size_t)D.tab_site);
         D.asite_off = dalloc<int>(own, 2 * (size_t)D.tab_site);
         D.asite_list = dalloc<int>(own, 2 * (size_t)D.tab_inc);
         D.ahb_inc_off = dalloc<int>(own, 2 * (size_t)D.tab_hb);
@@ -987,13 +1018,18 @@ extern "C" int tis_set_rng_epoch(tis_ctx *c, int rep, unsigned epoch) {
 
 // Captures and instantiates the graph of opt.graph_steps steps (launching
 // nothing) unless it exists.
+// Captures and instantiates the graph of opt.graph_steps steps starting from the
+// current parity of the state buffers (launching nothing) unless it exists.
+// Capture issues no work, so the parity is restored afterwards.
 static void capture_graph(tis_ctx *c) {
-    if (c->graph) return;
+    const int p0 = c->par;
+    if (c->graph[p0]) return;
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < c->opt.graph_steps; ++s) c->launch_step();
     HIP_TRY(hipStreamEndCapture(c->s_main, &g));
-    HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    c->set_parity(p0);
+    HIP_TRY(hipGraphInstantiate(&c->graph[p0], g, nullptr, nullptr, 0));
     HIP_TRY(hipGraphDestroy(g));
     c->graph_steps = c->opt.graph_steps;
 }
@@ -1002,10 +1038,7 @@ extern "C" int tis_prepare_graph(tis_ctx *c, int graph_steps) {
     if (!c || graph_steps < 0) return fail(Fail{TIS_EINVAL, "tis_prepare_graph: bad arguments"});
     return guarded([&]() -> int {
         HIP_TRY(hipStreamSynchronize(c->s_main));
-        if (c->graph && (graph_steps != c->graph_steps || !c->opt.use_graph || graph_steps == 0)) {
-            HIP_TRY(hipGraphExecDestroy(c->graph));
-            c->graph = nullptr;
-        }
+        if (graph_steps != c->graph_steps || !c->opt.use_graph || graph_steps == 0) c->drop_graphs();
         c->opt.graph_steps = graph_steps;
         if (graph_steps > 0 && c->opt.use_graph && !c->profiling) capture_graph(c);
         return TIS_OK;
@@ -1041,7 +1074,9 @@ static void run_steps(tis_ctx *c, long long n_steps) {
     if (c->opt.use_graph && c->opt.graph_steps > 0 && n_steps - done >= c->opt.graph_steps) {
         capture_graph(c);
         while (n_steps - done >= c->graph_steps) {
-            HIP_TRY(hipGraphLaunch(c->graph, c->s_main));
+            capture_graph(c);   // a graph for this parity (after an odd graph or remainder)
+            HIP_TRY(hipGraphLaunch(c->graph[c->par], c->s_main));
+            c->set_parity(c->par ^ (c->graph_steps & 1));
             done += c->graph_steps;
             c->n_graph_launch += 1;
             c->n_graph_step += c->graph_steps;
@@ -1088,10 +1123,7 @@ static void apply_launch(tis_ctx *c, const tis_options &o) {
     c->D.pair_lanes = o.pair_lanes;
     c->pair_block = o.pair_block;
     c->list_waves = o.list_waves;
-    if (c->graph) {
-        HIP_TRY(hipGraphExecDestroy(c->graph));
-        c->graph = nullptr;
-    }
+    c->drop_graphs();
 }
 
 extern "C" int tis_autotune(const tis_system *reps, int n_rep, tis_options *opt, int trial_steps, tis_tuning *report) {
@@ -1573,7 +1605,7 @@ extern "C" int tis_set_uniform_stream(tis_ctx *c, int rep, const int32_t *raw, l
             c->stream_total = total;
             c->stream_cap[rep] = n;
             HIP_TRY(hipMemcpy(c->D.stream_base, base.data(), c->n_rep * sizeof(long long), hipMemcpyHostToDevice));
-            if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+            c->drop_graphs();
         }
         long long base_r;
         HIP_TRY(hipMemcpy(&base_r, c->D.stream_base + rep, sizeof base_r, hipMemcpyDeviceToHost));

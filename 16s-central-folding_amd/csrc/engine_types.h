@@ -83,9 +83,13 @@ struct Dev {
     int st_slices;                // stack workgroups per replica in a step
     RepParams *rp;
     RepState *rs;
-    // particles (global index)
+    // particles (global index).  The state and the multi-body forces have two
+    // buffers each: a step reads pos / old / vel / f_mb and writes the new state
+    // to the *_nx buffers (clearing f_mb_nx for the next step); the host then
+    // swaps the pointers
     double4 *pos, *old, *vel;
-    double4 *f_sr, *f_lr, *f_mb;
+    double4 *pos_nx, *old_nx, *vel_nx;
+    double4 *f_sr, *f_lr, *f_mb, *f_mb_nx;
     double4 *jv, *jf;             // force matching: position tangent in, force tangent out
     int *ptype;                   // type | cls << 4 | charged << 8
     int *prep;                    // replica of each particle
@@ -124,10 +128,15 @@ struct Dev {
     int *hb_iter_part;            // per replica and resolution group: serial depth of its wavefronts
     double *hb_e_part;            // per replica and resolution group: its E2, E3 HB sums
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off
+    // per replica and live index (r * kActMax + a), written by k_hbenergy: energy,
+    // gate flag and the six bead forces (18 values of the force precision)
+    double *hb_en;
+    unsigned char *hb_gate;
+    void *hb_frc;
     int *act_hb;                  // per replica region [hb0, hb0 + nhb): live candidate ids
     int *act_tmp;                 // rebuild scratch: each status chunk's live ids, packed
     unsigned char *act_flag;      // region [hb0, hb0 + nhb): gate / resolution outcome
-    int *asite_id;                // region [site0, ...): site of each active site
+    int *asite_val;               // region [site0, ...): valence of each active site
     int *asite_off;               // region [site0 + rep, ...]: CSR offsets (n_asite + 1)
     int *asite_list;              // region [inc0, ...): live indices per active site, static order
     int *ahb_inc_off;             // region [hb0 + rep, ...]: CSR offsets per live candidate

@@ -55,27 +55,22 @@ TIS_DEV int wave_incl_scan(int x) {
 
 
 // Ordered exclusive prefix of a per-thread count across the block; *total gets
-// the block sum.  scratch must hold blockDim/64 + 1 ints.  All threads call.
+// the block sum.  scratch must hold blockDim/64 ints.  All threads call.
 TIS_DEV int block_excl_sum(int v, int *scratch, int *total) {
-    int lane = lane_id(), w = wave_id(), nw = (blockDim.x + 63) >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const int lane = lane_id(), w = wave_id(), nw = (blockDim.x + 63) >> 6;
+    const int x = wave_incl_scan(v);
     if (lane == 63) scratch[w] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int i = 0; i < nw; ++i) { int t = scratch[i]; scratch[i] = s; s += t; }
-        scratch[nw] = s;
+    // every thread adds the wavefront totals itself (independent broadcast reads)
+    int before = 0, all = 0;
+    for (int i = 0; i < nw; ++i) {
+        const int t = scratch[i];
+        before += i < w ? t : 0;
+        all += t;
     }
-    __syncthreads();
-    int res = scratch[w] + x - v;
-    *total = scratch[nw];
-    __syncthreads();
-    return res;
+    *total = all;
+    __syncthreads();   // the caller may reuse scratch at once
+    return before + x - v;
 }
 
 // Philox key of replica r: (seed low word mixed with the replica, seed high word
@@ -371,11 +366,11 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
 // ------------------------------------------------------------ HB rebuild
 // One buffer of the HB rebuild tables (the arrays hold two, tab_* elements apart).
 struct HbTab {
-    int *act_hb, *asite_id, *asite_off, *asite_list, *ahb_inc_off, *ahb_inc;
+    int *act_hb, *asite_val, *asite_off, *asite_list, *ahb_inc_off, *ahb_inc;
     int4 *act_rec;   // per listed candidate: (id, beads 0-2), (beads 3-5, 0)
 };
 TIS_DEV HbTab hb_tab(const Dev &D, int b) {
-    return {D.act_hb + b * D.tab_hb,      D.asite_id + b * D.tab_site,    D.asite_off + b * D.tab_site,
+    return {D.act_hb + b * D.tab_hb,      D.asite_val + b * D.tab_site,    D.asite_off + b * D.tab_site,
             D.asite_list + b * D.tab_inc, D.ahb_inc_off + b * D.tab_hb, D.ahb_inc + b * D.tab_inc,
             D.act_rec + 2 * b * D.tab_hb};
 }
@@ -413,7 +408,7 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
 }
 
 // k_hbsites: one workgroup per rebuilt replica.  Active sites (with at least one
-// live candidate) in site order, each with its live candidates in the order of
+// live candidate) in site order, with their valences, each with its live candidates in the order of
 // the static site list HB_ATOM (restored from the precomputed rank of every
 // incidence), and every live candidate's incidences as active-site indices.
 // Dynamic LDS: 2 * nsite ints.  lag: write the buffer the resolution is not
@@ -483,7 +478,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
         if (cnt > 0) {
             site_map[s] = rank;
             if (rank < kASiteMax) {
-                T.asite_id[P.site0 + rank] = P.site0 + s;
+                T.asite_val[P.site0 + rank] = D.site_val[P.site0 + s];
                 T.asite_off[P.site0 + r + rank] = off;
                 aoff_l[rank] = off;
             }
@@ -620,13 +615,13 @@ TIS_DEV void hb_geometry(const Dev &D, int g, R *q_out, R *psi0, V3<R> *d_unused
 // Forces of the six-bead Gaussian/Lorentzian-type term: with w the coefficient
 // (-E for HB, -E/(1+Q) for stacks), each harmonic piece k (y - y0)^2 is applied
 // with k scaled by w (stacking_*_force calls, fast.cpp:2043-2055, 2333-2345).
-// With vdotf, also returns the sum over the six beads of (force value) . (position
-// tangent), the derivative of v . F along a factor that scales the whole term.
+// sextet_force_parts sums the six beads' forces per bead into fb[0..5];
+// sextet_forces_at adds them to F and, with vdotf, also returns the sum over the
+// six beads of (force value) . (position tangent), the derivative of v . F along
+// a factor that scales the whole term.
 template <class R, class Pt>
-TIS_DEV void sextet_forces_at(double4 *F, const int *b, const Pt *p, const double *geo, R w, const R psi0[3],
-                              R kr, R kth, R kpsi, double *vdotf = nullptr) {
-    // the six beads' forces are summed here, then added once each
-    V3<R> fb[6];
+TIS_DEV void sextet_force_parts(V3<R> fb[6], const Pt *p, const double *geo, R w, const R psi0[3], R kr, R kth,
+                                R kpsi) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) fb[q] = V3<R>{(R)0, (R)0, (R)0};
     // bond (b1, b4)
@@ -666,6 +661,13 @@ TIS_DEV void sextet_forces_at(double4 *F, const int *b, const Pt *p, const doubl
 #pragma unroll
         for (int c = 0; c < 4; ++c) fb[quad[a][c]] = fb[quad[a][c]] + g[c] * coef;
     }
+}
+template <class R, class Pt>
+TIS_DEV void sextet_forces_at(double4 *F, const int *b, const Pt *p, const double *geo, R w, const R psi0[3],
+                              R kr, R kth, R kpsi, double *vdotf = nullptr) {
+    // the six beads' forces are summed first, then added once each
+    V3<R> fb[6];
+    sextet_force_parts<R>(fb, p, geo, w, psi0, kr, kth, kpsi);
 #pragma unroll
     for (int q = 0; q < 6; ++q) atomic_add_force(F, b[q], fb[q]);
     if (vdotf) {
@@ -763,14 +765,10 @@ TIS_DEV void wave_lds_sync() {
 // shorter one evaluates the tests on its path.  Draws come from an LDS window of `win` values that the wavefront
 // refills.  Sets *ovf on a site with more than 64 live candidates or an
 // inconsistent table.
-TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack,
-                          const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
-                          const double *E, double T, double *ubuf, int win, const HbDraws &dr, int comp,
-                          int *iters_out, long long *base_out, bool *ovf) {
+// Pairs (i, c), c < i, of the all-at-once walk evaluated by this lane in passes
+// 0..3: triangular index t = 64 p + lane = i (i - 1) / 2 + c.
+TIS_DEV void walk_pairs(int pi[4], int pc[4]) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    // pairs (i, c) of passes 0..3 handled by this lane
-    int pi[4], pc[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int t = 64 * p + lane;
@@ -780,6 +778,78 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         pi[p] = i;
         pc[p] = t - i * (i - 1) / 2;
     }
+}
+
+// One pass of the process at the picked site, whose candidate list is pk =
+// offset | length << 16 in alist: its live candidates compacted to lanes
+// 0..nK-1 in static order, the shuffle (step i swaps i with ceil(u_{1+i} nK) -
+// 1, targets computed at once) and the walk (k moves to i when u_{nK+i} <
+// exp((E_i - E_k)/T)).  Returns the candidate to switch off, with *nK_out, or -1
+// for an inconsistent table (more than 64 entries, no live candidate).
+TIS_DEV int resolve_site_pass(int pk, const short *alist, const unsigned char *flag, const double *E,
+                              const double *u, double T, double invT, const int pi[4], const int pc[4],
+                              int *nK_out) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const int o0 = pk & 0xffff, len = pk >> 16;
+    if (len > 64) return -1;
+    int bond = -1;
+    bool keep = false;
+    if (lane < len) { bond = alist[o0 + lane]; keep = flag[bond] == FLAG_LIVE; }
+    const unsigned long long km = __ballot(keep);
+    const int nK = __popcll(km);
+    if (nK == 0) return -1;
+    // a kept lane sends its bond to lane pos, the others to lane 63, which is
+    // never read (nK == 64 only if every lane keeps)
+    const int pos = __popcll(km & lt_mask);
+    int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
+    const int Jl = lane < nK ? ceil_index(u[1 + lane], nK) - 1 : 0;
+    for (int i = 0; i < nK; ++i) {
+        const int tj = readlane_i(Jl, i);
+        const int bi = readlane_i(bv, i), bj = readlane_i(bv, tj);
+        if (lane == i) bv = bj;
+        else if (lane == tj) bv = bi;
+    }
+    const double Ev = lane < nK ? E[bv] : 0.0;
+    int k = 0;
+    const int npair = nK * (nK - 1) / 2;
+    if (nK > kWalkSeq && npair <= 256) {
+        unsigned long long m[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (64 * p < npair) {
+                const int i = pi[p] < nK ? pi[p] : 0, c = pi[p] < nK ? pc[p] : 0;
+                const double Ei = shfl_d(Ev, i), Ec = shfl_d(Ev, c);
+                const bool acc = pi[p] < nK && hb_accept(u[nK + i], Ei - Ec, T, invT);
+                m[p] = __ballot(acc);
+            }
+        }
+        for (int i = 1; i < nK; ++i) {
+            const int t = i * (i - 1) / 2 + k;
+            const unsigned long long mt = t < 128 ? (t < 64 ? m[0] : m[1]) : (t < 192 ? m[2] : m[3]);
+            if ((mt >> (t & 63)) & 1ull) k = i;
+        }
+    } else {
+        // a short walk: only the tests on its path, draws loaded at once
+        const double ul = lane < nK ? u[nK + lane] : 0.0;
+        double Ek = readlane_d(Ev, 0);
+        for (int i = 1; i < nK; ++i) {
+            const double Ei = readlane_d(Ev, i);
+            if (hb_accept(readlane_d(ul, i), Ei - Ek, T, invT)) { k = i; Ek = Ei; }
+        }
+    }
+    *nK_out = nK;
+    return readlane_i(bv, k);
+}
+
+TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack,
+                          const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
+                          const double *E, double T, double *ubuf, int win, const HbDraws &dr, int comp,
+                          int *iters_out, long long *base_out, bool *ovf) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    int pi[4], pc[4];
+    walk_pairs(pi, pc);
     long long base = 0, win_lo = 0;
     int iters = 0;
     const double invT = 1.0 / T;
@@ -808,57 +878,9 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         if (hb == 0) { *ovf = true; break; }
         const int ci = c0 + 64 * L + __builtin_ctzll(hb);
         if (ci >= nA0) { *ovf = true; break; }
-        // ---- its live candidates, compacted to lanes 0..nK-1 in static order
-        const int pk = apack[ci], o0 = pk & 0xffff, len = pk >> 16;
-        if (len > 64) { *ovf = true; break; }
-        int bond = -1;
-        bool keep = false;
-        if (lane < len) { bond = alist[o0 + lane]; keep = flag[bond] == FLAG_LIVE; }
-        const unsigned long long km = __ballot(keep);
-        const int nK = __popcll(km);
-        if (nK == 0) { *ovf = true; break; }
-        // a kept lane sends its bond to lane pos, the others to lane 63, which is
-        // never read (nK == 64 only if every lane keeps)
-        const int pos = __popcll(km & lt_mask);
-        int bv = __builtin_amdgcn_ds_permute((keep ? pos : 63) << 2, keep ? bond : -1);
-        // ---- shuffle: step i swaps i with ceil(u_{1+i} nK) - 1; targets computed at once
-        const int Jl = lane < nK ? ceil_index(u[1 + lane], nK) - 1 : 0;
-        for (int i = 0; i < nK; ++i) {
-            const int tj = readlane_i(Jl, i);
-            const int bi = readlane_i(bv, i), bj = readlane_i(bv, tj);
-            if (lane == i) bv = bj;
-            else if (lane == tj) bv = bi;
-        }
-        const double Ev = lane < nK ? E[bv] : 0.0;
-        // ---- walk: k moves to i when u_{nK+i} < exp((E_i - E_k)/T)
-        int k = 0;
-        const int npair = nK * (nK - 1) / 2;
-        if (nK > kWalkSeq && npair <= 256) {
-            unsigned long long m[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                if (64 * p < npair) {
-                    const int i = pi[p] < nK ? pi[p] : 0, c = pi[p] < nK ? pc[p] : 0;
-                    const double Ei = shfl_d(Ev, i), Ec = shfl_d(Ev, c);
-                    const bool acc = pi[p] < nK && hb_accept(u[nK + i], Ei - Ec, T, invT);
-                    m[p] = __ballot(acc);
-                }
-            }
-            for (int i = 1; i < nK; ++i) {
-                const int t = i * (i - 1) / 2 + k;
-                const unsigned long long mt = t < 128 ? (t < 64 ? m[0] : m[1]) : (t < 192 ? m[2] : m[3]);
-                if ((mt >> (t & 63)) & 1ull) k = i;
-            }
-        } else {
-            // a short walk: only the tests on its path, draws loaded at once
-            const double ul = lane < nK ? u[nK + lane] : 0.0;
-            double Ek = readlane_d(Ev, 0);
-            for (int i = 1; i < nK; ++i) {
-                const double Ei = readlane_d(Ev, i);
-                if (hb_accept(readlane_d(ul, i), Ei - Ek, T, invT)) { k = i; Ek = Ei; }
-            }
-        }
-        const int off = readlane_i(bv, k);
+        int nK = 0;
+        const int off = resolve_site_pass(apack[ci], alist, flag, E, u, T, invT, pi, pc, &nK);
+        if (off < 0) { *ovf = true; break; }
         base += 2 * nK;
         // ---- switch it off; sites whose excess reaches 0 leave the mask
         if (lane == 0) flag[off] = FLAG_OFF;
@@ -879,6 +901,63 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
             cm &= cm - 1ull;
             const int bb = readlane_i(b, l);
             if (lane == (bb >> 6)) w &= ~(1ull << (bb & 63));
+        }
+        wave_lds_sync();
+    }
+    if (base > dr.avail()) *ovf = true;
+    *iters_out = iters;
+    *base_out = base;
+}
+
+// resolve_wave for a component of at most 64 over-subscribed sites, with their
+// state in registers: lane t < nm holds the component's t-th site in index order
+// (mci), its excess (ex) and its candidate list (pk).  The same draws, picks and
+// decisions as resolve_wave on the component's mask: the j-th over-subscribed
+// site in index order is the j-th lane whose excess is positive, and every
+// over-subscribed site of a switched-off candidate is a member (a component
+// joins all the sites its live candidates touch), so the candidate's incidences
+// lower lane excesses only (once per incidence, multiplicity kept).
+TIS_DEV void resolve_small(int nm, int mci, int ex, int pk, const short *alist, const short *hoff,
+                           const short *hinc, unsigned char *flag, const double *E, double T, double *ubuf,
+                           int win, const HbDraws &dr, int comp, int *iters_out, long long *base_out, bool *ovf) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    int pi[4], pc[4];
+    walk_pairs(pi, pc);
+    long long base = 0, win_lo = 0;
+    int iters = 0;
+    const double invT = 1.0 / T;
+    int exv = lane < nm ? ex : 0;
+    dr.fill(ubuf, 0, win, lane, 64, comp);
+    wave_lds_sync();
+    for (;;) {
+        const unsigned long long om = __ballot(exv > 0);
+        const int nA = __popcll(om);
+        if (nA == 0) break;
+        if (++iters > 1000000) { *ovf = true; break; }   // each pass removes >= 1 excess unit
+        if (base + 128 > win_lo + win) {
+            win_lo = base & ~3ll;
+            dr.fill(ubuf, win_lo, win, lane, 64, comp);
+            wave_lds_sync();
+        }
+        const double *u = ubuf + (base - win_lo);
+        // ---- the j-th over-subscribed site: the lane holding it
+        const int j = ceil_index(u[0], nA) - 1;
+        const unsigned long long hit = __ballot(((om >> lane) & 1ull) && __popcll(om & lt_mask) == j);
+        if (hit == 0) { *ovf = true; break; }
+        const int L = __builtin_ctzll(hit);
+        int nK = 0;
+        const int off = resolve_site_pass(readlane_i(pk, L), alist, flag, E, u, T, invT, pi, pc, &nK);
+        if (off < 0) { *ovf = true; break; }
+        base += 2 * nK;
+        // ---- switch it off and lower the excess of its sites
+        if (lane == 0) flag[off] = FLAG_OFF;
+        const int h0 = hoff[off], hn = hoff[off + 1] - h0;
+        if (hn > 64) { *ovf = true; break; }
+        const int ci2 = lane < hn ? hinc[h0 + lane] : -1;   // over-subscribed index, or -1
+        for (int q = 0; q < hn; ++q) {
+            const int c = readlane_i(ci2, q);
+            if (lane < nm && mci == c) exv -= 1;
         }
         wave_lds_sync();
     }
@@ -930,37 +1009,49 @@ __global__ void __launch_bounds__(kBlock) k_hbsum(Dev D) {
     if (r < D.n_rep) hb_reduce_parts(D, r);
 }
 
-// FP64 forces of the kept HB candidates, one thread each, after k_hbres: its
-// FP64 form does not fuse them (the force arithmetic would spill at 1,024
-// threads per workgroup).  The energy is recomputed by the same expression as
-// in k_hbres; the sums are added to group 0's part.
-__global__ void __launch_bounds__(kBlock) k_hbforce64(Dev D) {
+// Gate, energy and forces of every live HB candidate (hydrogen_bonds_interactions,
+// fast.cpp:2180-2237, forces :2333-2345), one thread per candidate, before the
+// valence resolution decides which are kept: E = E_hb exp(-Q) when |r - r0| <= 2 A
+// for the (k11, k21) pair (gate flag FLAG_LIVE), else 0 (FLAG_GATED); inside the
+// gate the six bead forces of the candidate, to be added by k_hbres if it keeps
+// it.  Computed in precision R once per candidate, for every resolution group.
+// Grid (kActMax / kBlock, replicas); outputs at r * kActMax + live index.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
     const int r = blockIdx.y;
     const RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
-    __shared__ double red[kBlock / 64];
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_act = (S.overflow & 2) ? 0 : S.n_act;
-    if ((int)(blockIdx.x * blockDim.x) >= n_act) return;
-    double e2 = 0, e3 = 0;
-    if (a < n_act && D.act_flag[P.hb0 + a] == FLAG_LIVE) {
-        const HbTab T = hb_tab(D, S.tab);
-        const int4 ra = T.act_rec[2 * (P.hb0 + a)], rb = T.act_rec[2 * (P.hb0 + a) + 1];
-        const int g = ra.x, bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
-        const double4 pt[6] = {D.pos[bd[0]], D.pos[bd[1]], D.pos[bd[2]], D.pos[bd[3]], D.pos[bd[4]], D.pos[bd[5]]};
-        const double *geo = D.hb_g + 6 * g;
-        double q, psi0[3];
-        hb_geometry_at<double>(pt, geo, &q, psi0);
-        const double e = D.hb_E[g] * exp(-q);
-        if (D.hb_dode[g]) e3 = e; else e2 = e;
-        sextet_forces_at<double>(D.f_mb, bd, pt, geo, -e, psi0, 5.0, 1.5, 0.15);
+    if (a >= n_act) return;
+    const HbTab T = hb_tab(D, S.tab);
+    const int4 ra = T.act_rec[2 * (P.hb0 + a)], rb = T.act_rec[2 * (P.hb0 + a) + 1];
+    const int g = ra.x;
+    const double4 pt[6] = {D.pos[ra.y], D.pos[ra.z], D.pos[ra.w], D.pos[rb.x], D.pos[rb.y], D.pos[rb.z]};
+    const double *geo = D.hb_g + 6 * g;
+    const double hE = D.hb_E[g];
+    const size_t o = (size_t)kActMax * r + a;
+    const V3<R> v = diff<R>(pt[1], pt[4]);
+    const R rr = sqrt(dot(v, v));
+    double e = 0.0;
+    unsigned char f = FLAG_GATED;
+    if (!(fabs(rr - (R)geo[0]) > (R)2)) {
+        R q, psi0[3];
+        hb_geometry_at<R>(pt, geo, &q, psi0);
+        e = (double)((R)hE * exp(-q));
+        f = FLAG_LIVE;
+        V3<R> fb[6];
+        sextet_force_parts<R>(fb, pt, geo, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+        R *out = (R *)D.hb_frc + 18 * o;
+#pragma unroll
+        for (int q6 = 0; q6 < 6; ++q6) {
+            out[3 * q6] = fb[q6].x;
+            out[3 * q6 + 1] = fb[q6].y;
+            out[3 * q6 + 2] = fb[q6].z;
+        }
     }
-    const double s2 = block_sum_double(e2, red);
-    const double s3 = block_sum_double(e3, red);
-    if (threadIdx.x == 0) {
-        if (s2 != 0.0) unsafeAtomicAdd(&D.hb_e_part[2 * kMaxHbGroups * r], s2);
-        if (s3 != 0.0) unsafeAtomicAdd(&D.hb_e_part[2 * kMaxHbGroups * r + 1], s3);
-    }
+    D.hb_en[o] = e;
+    D.hb_gate[o] = f;
 }
 
 // Energies, gates and valence resolution of the live HB candidates; writes the
@@ -1025,11 +1116,9 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     if (D.rng == 1 && grp > 0) return;
     if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    // ---- the rebuild tables, site valences, candidate energies and gates.  A
-    // thread issues all its loads of one level before it uses any: level 1 the
-    // tables, the live candidates' ids and the active sites' ids; level 2 the
-    // candidates' gate beads and r0 and the sites' valences; level 3 the gate
-    // positions, then the geometry of the candidates inside the gate.
+    // ---- the rebuild tables with the active sites' valences, and the candidate
+    // energies and gates of k_hbenergy: one level of loads, all issued before any
+    // is used.
     const int n_hinc = T.ahb_inc_off[P.hb0 + r + n_act];
     if (n_hinc > kAIncMax) {                      // inconsistent rebuild tables
         if (tid == 0) atomicOr(&S.overflow, 2);
@@ -1038,17 +1127,24 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     constexpr int KA = (kASiteMax + kResBlock) / kResBlock;   // slots per thread: sites (+1)
     constexpr int KC = (kActMax + kResBlock) / kResBlock;     // live candidates (+1)
     constexpr int KI = (kAIncMax + kResBlock - 1) / kResBlock;  // incidences
-    int t_aoff[KA], t_sid[KA], t_hoff[KC], t_alist[KI], t_hinc[KI];
+    int t_aoff[KA], t_val[KA], t_hoff[KC], t_alist[KI], t_hinc[KI];
+    double t_e[KC];
+    unsigned char t_f[KC];
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
         t_aoff[k] = i <= n_as ? T.asite_off[P.site0 + r + i] : 0;
-        t_sid[k] = i < n_as ? T.asite_id[P.site0 + i] : 0;
+        t_val[k] = i < n_as ? T.asite_val[P.site0 + i] : 0;
     }
+    // energies and gates of the live candidates (k_hbenergy)
+    const double *hb_en = D.hb_en + (size_t)kActMax * r;
+    const unsigned char *hb_gate = D.hb_gate + (size_t)kActMax * r;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
         const int i = tid + k * kResBlock;
         t_hoff[k] = i <= n_act ? T.ahb_inc_off[P.hb0 + r + i] : 0;
+        t_e[k] = i < n_act ? hb_en[i] : 0.0;
+        t_f[k] = i < n_act ? hb_gate[i] : (unsigned char)FLAG_GATED;
     }
 #pragma unroll
     for (int k = 0; k < KI; ++k) {
@@ -1056,19 +1152,13 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         t_alist[k] = i < n_ainc ? T.asite_list[P.inc0 + i] : 0;
         t_hinc[k] = i < n_hinc ? T.ahb_inc[P.inc0 + i] : 0;
     }
-    // the thread's first candidate goes through the same levels (later ones, rare
-    // below 1024 live candidates, load in turn)
-    // the thread's first candidate: its record with the tables (level 1), its
-    // bead positions and constants next (level 2)
+    // the thread's first candidate's record with the tables (later ones, rare
+    // below 1024 live candidates, load in turn when their forces are added)
     int4 ra0 = make_int4(0, 0, 0, 0), rb0 = ra0;
     if (tid < n_act) {
         ra0 = T.act_rec[2 * (P.hb0 + tid)];
         rb0 = T.act_rec[2 * (P.hb0 + tid) + 1];
     }
-    const int g0 = ra0.x;
-    int t_val[KA];
-#pragma unroll
-    for (int k = 0; k < KA; ++k) t_val[k] = tid + k * kResBlock < n_as ? D.site_val[t_sid[k]] : 0;
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
@@ -1076,65 +1166,59 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         if (i < n_as) excess[i] = t_val[k];       // the valence, until the excess replaces it
     }
 #pragma unroll
-    for (int k = 0; k < KC; ++k)
-        if (tid + k * kResBlock <= n_act) hoff[tid + k * kResBlock] = (short)t_hoff[k];
+    for (int k = 0; k < KC; ++k) {
+        const int i = tid + k * kResBlock;
+        if (i <= n_act) hoff[i] = (short)t_hoff[k];
+        if (i < n_act) {
+            E[i] = t_e[k];
+            flag[i] = t_f[k];
+        }
+    }
 #pragma unroll
     for (int k = 0; k < KI; ++k) {
         const int i = tid + k * kResBlock;
         if (i < n_ainc) alist[i] = (short)t_alist[k];
         if (i < n_hinc) hinc[i] = (short)t_hinc[k];
     }
-#pragma unroll 1
-    for (int a = tid; a < n_act; a += nt) {
-        int4 ra = ra0, rb = rb0;
-        if (a != tid) {
-            ra = T.act_rec[2 * (P.hb0 + a)];
-            rb = T.act_rec[2 * (P.hb0 + a) + 1];
-        }
-        const int g = ra.x;
-        const double4 pt[6] = {D.pos[ra.y], D.pos[ra.z], D.pos[ra.w], D.pos[rb.x], D.pos[rb.y], D.pos[rb.z]};
-        const double *geo = D.hb_g + 6 * g;
-        const double hE = D.hb_E[g];
-        const V3<R> v = diff<R>(pt[1], pt[4]);
-        const R rr = sqrt(dot(v, v));
-        double e = 0.0;
-        unsigned char f = FLAG_GATED;
-        if (!(fabs(rr - (R)geo[0]) > (R)2)) {
-            R q, psi0[3];
-            hb_geometry_at<R>(pt, geo, &q, psi0);
-            e = (double)((R)hE * exp(-q));
-            f = FLAG_LIVE;
-        }
-        E[a] = e;
-        flag[a] = f;
-    }
     __syncthreads();
     // ---- excess = live incidences - valence per active site; the
     // over-subscribed ones in site order (index ci: candidate list apack[ci]) and
-    // the inverse map A1
-    int nA0 = 0;
-    for (int s0 = 0; s0 < n_as; s0 += nt) {
-        const int s = s0 + tid;
-        int ex = 0;
-        if (s < n_as) {
-            int c = 0;
-            for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
-            ex = c - excess[s];
-        }
-        int tot;
-        const int rank = block_excl_sum(ex > 0, scratch, &tot);
-        if (s < n_as) {
-            A1[s] = (short)(ex > 0 ? nA0 + rank : -1);
-            // excess re-indexed by over-subscribed index (nA0 + rank <= s: no
-            // site of this or a later round is overwritten; this round's reads
-            // are done, block_excl_sum synchronises)
-            if (ex > 0) {
-                apack[nA0 + rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
-                excess[nA0 + rank] = ex;
-                par[nA0 + rank] = nA0 + rank;   // union-find roots of the component form
+    // the inverse map A1.  Each thread takes a contiguous range of sites, so one
+    // scan ranks them in site order.
+    constexpr int KS = (kASiteMax + kResBlock - 1) / kResBlock;   // sites per thread
+    int nA0;
+    {
+        const int per = (n_as + nt - 1) / nt;
+        const int s_lo = min(n_as, tid * per);
+        int exs[KS], mine = 0;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int s = s_lo + k;
+            exs[k] = 0;
+            if (k < per && s < n_as) {
+                int c = 0;
+                for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
+                exs[k] = c - excess[s];
+                mine += exs[k] > 0;
             }
         }
-        nA0 += tot;
+        // excess re-indexed by over-subscribed index (rank <= s); every valence
+        // was read before the scan's barriers
+        int rank = block_excl_sum(mine, scratch, &nA0);
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            const int s = s_lo + k;
+            if (k < per && s < n_as) {
+                const int ex = exs[k];
+                A1[s] = (short)(ex > 0 ? rank : -1);
+                if (ex > 0) {
+                    apack[rank] = (int)aoff[s] | ((int)(aoff[s + 1] - aoff[s]) << 16);
+                    excess[rank] = ex;
+                    par[rank] = rank;   // union-find roots of the component form
+                    ++rank;
+                }
+            }
+        }
     }
     __syncthreads();
     const HbDraws dr(D, r, S);
@@ -1176,25 +1260,34 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             }
         }
         __syncthreads();
-        // roots (smallest index of each component) in order; every index points
-        // at its root afterwards (finds only shorten paths, so they may run
-        // beside each other)
-        int ncomp = 0;
-        for (int c0 = 0; c0 < nA0; c0 += nt) {
-            const int ci = c0 + tid;
-            int rt = -1;
-            if (ci < nA0) {
-                rt = uf_find(vpar, ci);
-                par[ci] = rt;
+        // roots (smallest index of each component) in order, each thread over a
+        // contiguous range of indices; every index points at its root afterwards
+        // (finds only shorten paths, so they may run beside each other)
+        int ncomp;
+        {
+            const int per = (nA0 + nt - 1) / nt;
+            const int c_lo = min(nA0, tid * per);
+            int rts[KS], mine = 0;
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                const int ci = c_lo + k;
+                rts[k] = -1;
+                if (k < per && ci < nA0) {
+                    rts[k] = uf_find(vpar, ci);
+                    par[ci] = rts[k];
+                    mine += rts[k] == ci;
+                }
             }
-            const bool root = ci < nA0 && rt == ci;
-            int tot;
-            const int rank = block_excl_sum(root, scratch, &tot);
-            if (root) {
-                croot[ncomp + rank] = (short)ci;
-                cidx[ci] = (short)(ncomp + rank);
+            int rank = block_excl_sum(mine, scratch, &ncomp);
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                const int ci = c_lo + k;
+                if (k < per && ci < nA0 && rts[k] == ci) {
+                    croot[rank] = (short)ci;
+                    cidx[ci] = (short)rank;
+                    ++rank;
+                }
             }
-            ncomp += tot;
         }
         __syncthreads();
         // ---- each wavefront takes the next component in root order when it is done
@@ -1205,19 +1298,32 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             const int k = grp + ngrp * j;
             if (k >= ncomp) break;
             const int root = croot[k], cfirst = root >> 6, c0 = cfirst << 6;
-            // member mask: word w from the ballot over indices c0 + 64 w .. + 63
+            // member mask (word w from the ballot over indices c0 + 64 w .. + 63)
+            // and, up to 64 members, the members in index order in lanes 0, 1, ...
             unsigned long long m = 0;
+            int nm = 0, mci = -1;
             const int clast = (nA0 - 1) >> 6;
-#pragma unroll 4
             for (int c = cfirst; c <= clast; ++c) {
                 const int ci = 64 * c + lane;
-                const unsigned long long bm = __ballot(ci < nA0 && par[ci] == root);
+                unsigned long long bm = __ballot(ci < nA0 && par[ci] == root);
                 if (lane == c - cfirst) m = bm;
+                while (bm) {
+                    const int b = __builtin_ctzll(bm);
+                    bm &= bm - 1ull;
+                    if (lane == nm) mci = 64 * c + b;
+                    ++nm;
+                }
             }
             int it = 0;
             long long base = 0;
-            resolve_wave(m, excess, c0, nA0, apack, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
-                         &base, &ovf);
+            if (nm <= 64) {
+                const int exm = lane < nm ? excess[mci] : 0, pkm = lane < nm ? apack[mci] : 0;
+                resolve_small(nm, mci, exm, pkm, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it, &base,
+                              &ovf);
+            } else {
+                resolve_wave(m, excess, c0, nA0, apack, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
+                             &base, &ovf);
+            }
             wave_iters += it;
             if (ovf) break;
             int nj = 0;
@@ -1230,10 +1336,9 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         }
     }
     __syncthreads();
-    // ---- outcome and, for the kept candidates, energies and forces
-    // (hydrogen_bonds_interactions, fast.cpp:2333-2345), by the group that ran
-    // the candidate's component
-    constexpr bool kFusedForces = sizeof(R) == sizeof(float);   // FP64: k_hbforce64
+    // ---- outcome and, for the kept candidates, energies and the forces
+    // k_hbenergy computed (hydrogen_bonds_interactions, fast.cpp:2333-2345), by
+    // the group that ran the candidate's component
     double e2 = 0.0, e3 = 0.0;
     for (int a = tid; a < n_act; a += nt) {
         int owner = 0;
@@ -1252,16 +1357,16 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         const int g = ra.x;
         D.hb_out[g] = (signed char)flag[a];
         D.act_flag[P.hb0 + a] = flag[a];
-        if (kFusedForces && flag[a] == FLAG_LIVE) {
+        if (flag[a] == FLAG_LIVE) {
             const int bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
-            const double4 pt[6] = {D.pos[bd[0]], D.pos[bd[1]], D.pos[bd[2]], D.pos[bd[3]], D.pos[bd[4]], D.pos[bd[5]]};
-            const double *geo = D.hb_g + 6 * g;
-            const int dode = D.hb_dode[g];
-            R q, psi0[3];
-            hb_geometry_at<R>(pt, geo, &q, psi0);
+            const R *fr = (const R *)D.hb_frc + 18 * ((size_t)kActMax * r + a);
+            V3<R> fb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fb[q] = V3<R>{fr[3 * q], fr[3 * q + 1], fr[3 * q + 2]};
             const double e = E[a];
-            if (dode) e3 += e; else e2 += e;
-            sextet_forces_at<R>(D.f_mb, bd, pt, geo, (R)(-e), psi0, (R)5, (R)1.5, (R)0.15);
+            if (D.hb_dode[g]) e3 += e; else e2 += e;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) atomic_add_force(D.f_mb, bd[q], fb[q]);
         }
     }
     {
@@ -1695,17 +1800,13 @@ TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S
     }
 }
 
+// The new state goes to the other buffer set (pos_nx, old_nx, vel_nx; every
+// read of this step's state is from the current set, so no particle's write can
+// reach another's read, the recentring bead's included); the cell-ordered copy
+// the pair gather reads is refreshed (its .w, identity and type, stays) and the
+// multi-body force buffer of the next step cleared.
 __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < D.n_rep && D.st_slices > 1) {   // stack energies of the step's slices
-        double s2 = 0, s3 = 0;
-        for (int q = 0; q < D.st_slices; ++q) {
-            s2 += D.st_part[2 * (kMaxStSlices * p + q)];
-            s3 += D.st_part[2 * (kMaxStSlices * p + q) + 1];
-        }
-        D.rs[p].e[EI_E2_ST] = s2;
-        D.rs[p].e[EI_E3_ST] = s3;
-    }
     double d2 = 0.0;   // squared displacement of this thread's particle
     int rp = -1;       // its replica
     if (p < D.n_tot) {
@@ -1730,11 +1831,14 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         const double dx = xn[0] - on[0], dy = xn[1] - on[1], dz = xn[2] - on[2];
         d2 = dx * dx + dy * dy + dz * dz;
         rp = r;
-        // all reads of the old state of this step happen before any write: the
-        // new state goes to the second buffer set, copied back by k_commit
-        D.pos[D.n_tot + p] = make_double4(xn[0], xn[1], xn[2], 0.0);
-        D.old[D.n_tot + p] = make_double4(on[0], on[1], on[2], 0.0);
-        D.vel[D.n_tot + p] = make_double4(vn[0], vn[1], vn[2], 0.0);
+        D.pos_nx[p] = make_double4(xn[0], xn[1], xn[2], 0.0);
+        D.old_nx[p] = make_double4(on[0], on[1], on[2], 0.0);
+        D.vel_nx[p] = make_double4(vn[0], vn[1], vn[2], 0.0);
+        D.f_mb_nx[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+        double4 *c = D.cpos + D.crank[p];
+        c->x = xn[0];
+        c->y = xn[1];
+        c->z = xn[2];
     }
     // largest displacement of the step per replica (d2 >= 0, so its bits order
     // like the values): a wavefront maximum when the wavefront lies in one
@@ -1752,17 +1856,24 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     }
 }
 
-// Copies the second state buffer set into the first (keeps one buffer layout
-// for every other kernel; cheap: 3 x 32 B per particle).  Workgroup 0 also
-// closes the step of every replica: the list-rebuild decision from the largest
-// displacement k_integrate recorded (check_shifts, fast.cpp:1741-1776), the step
-// counter and the LR refresh flag.
+// Closes the step of every replica (one workgroup, a thread per replica): the
+// list-rebuild decision from the largest displacement k_integrate recorded
+// (check_shifts, fast.cpp:1741-1776), the step's stack and HB energy sums, the
+// HB table switch, the step counter and the LR refresh flag.
 __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0) {
+    {
         for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
             RepState &S = D.rs[r];
             const RepParams &P = D.rp[r];
+            if (D.st_slices > 1) {   // stack energies of the step's slices
+                double s2 = 0, s3 = 0;
+                for (int q = 0; q < D.st_slices; ++q) {
+                    s2 += D.st_part[2 * (kMaxStSlices * r + q)];
+                    s3 += D.st_part[2 * (kMaxStSlices * r + q) + 1];
+                }
+                S.e[EI_E2_ST] = s2;
+                S.e[EI_E3_ST] = s3;
+            }
             unsigned long long mb = 0;
             for (int q = 0; q < kMaxSlots; ++q) {
                 mb = mb > S.maxd2[q] ? mb : S.maxd2[q];
@@ -1789,17 +1900,6 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
             S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
         }
     }
-    if (p >= D.n_tot) return;
-    const double4 x = D.pos[D.n_tot + p];
-    D.pos[p] = x;
-    D.old[p] = D.old[D.n_tot + p];
-    D.vel[p] = D.vel[D.n_tot + p];
-    D.f_mb[p] = make_double4(0.0, 0.0, 0.0, 0.0);
-    // the cell-ordered copy the pair gather reads (its .w, identity and type, stays)
-    double4 *c = D.cpos + D.crank[p];
-    c->x = x.x;
-    c->y = x.y;
-    c->z = x.z;
 }
 
 // ------------------------------------------------------------ replica exchange
@@ -1843,7 +1943,7 @@ __global__ void __launch_bounds__(kBlock) k_remd_mark(Dev D, int parity) {
 }
 
 // mode 0: swap the configurations of the pairs in state 1, keeping the old ones
-// in the integrator's second buffer; the configuration moving from slot a to
+// in the integrator's other buffer set; the configuration moving from slot a to
 // slot b has its velocity and x - x_old scaled by sqrt(T_b / T_a).  mode 1:
 // restore the pairs in state 2.  Grid (particle blocks, pairs q, pair (2q +
 // parity, 2q + parity + 1)); the slots are marked for a list rebuild and their
@@ -1855,14 +1955,13 @@ __global__ void __launch_bounds__(kBlock) k_remd_swap(Dev D, int parity, int mod
     if (D.remd_state[ra] != (mode == 0 ? 1 : 2)) return;
     const RepParams &A = D.rp[ra], &B = D.rp[rb];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = D.n_tot;
     if (i < A.n) {
         const int pa = A.p0 + i, pb = B.p0 + i;
         if (mode == 0) {
             const double4 xa = D.pos[pa], oa = D.old[pa], va = D.vel[pa];
             const double4 xb = D.pos[pb], ob = D.old[pb], vb = D.vel[pb];
-            D.pos[N + pa] = xa; D.old[N + pa] = oa; D.vel[N + pa] = va;
-            D.pos[N + pb] = xb; D.old[N + pb] = ob; D.vel[N + pb] = vb;
+            D.pos_nx[pa] = xa; D.old_nx[pa] = oa; D.vel_nx[pa] = va;
+            D.pos_nx[pb] = xb; D.old_nx[pb] = ob; D.vel_nx[pb] = vb;
             const double s_ab = sqrt(B.T / A.T), s_ba = sqrt(A.T / B.T);   // a -> b, b -> a
             D.pos[pa] = xb;
             D.old[pa] = make_double4(xb.x - s_ba * (xb.x - ob.x), xb.y - s_ba * (xb.y - ob.y),
@@ -1873,8 +1972,8 @@ __global__ void __launch_bounds__(kBlock) k_remd_swap(Dev D, int parity, int mod
                                      xa.z - s_ab * (xa.z - oa.z), 0.0);
             D.vel[pb] = make_double4(s_ab * va.x, s_ab * va.y, s_ab * va.z, 0.0);
         } else {
-            D.pos[pa] = D.pos[N + pa]; D.old[pa] = D.old[N + pa]; D.vel[pa] = D.vel[N + pa];
-            D.pos[pb] = D.pos[N + pb]; D.old[pb] = D.old[N + pb]; D.vel[pb] = D.vel[N + pb];
+            D.pos[pa] = D.pos_nx[pa]; D.old[pa] = D.old_nx[pa]; D.vel[pa] = D.vel_nx[pa];
+            D.pos[pb] = D.pos_nx[pb]; D.old[pb] = D.old_nx[pb]; D.vel[pb] = D.vel_nx[pb];
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -244,10 +244,15 @@ def test_high_salt_450A_per_term_mixed(lib, high_salt, name, mask):
     eng.close()
 
 
-def test_graph_replays_match_eager_steps(lib, sys1):
-    """hipGraph replays (50 steps each) give the same trajectory as eager launches."""
+@pytest.mark.parametrize("gsteps", [50, 25, 7])
+def test_graph_replays_match_eager_steps(lib, sys1, gsteps):
+    """hipGraph replays give the same trajectory as eager launches.  Each step
+    writes the other of two state buffers, so a graph of an odd number of steps
+    (25, 7) ends on the other buffer and the next replay uses the graph captured
+    for that one; 50-step calls with a 7-step graph also run an eager remainder
+    of one step between replays."""
     raw = O.glibc_stream(31, 100 * 4200)
-    a = T.Engine(lib, [sys1], rng=T.RNG_STREAM, use_graph=1, graph_steps=50)
+    a = T.Engine(lib, [sys1], rng=T.RNG_STREAM, use_graph=1, graph_steps=gsteps)
     b = T.Engine(lib, [sys1], rng=T.RNG_STREAM, use_graph=0)
     a.set_stream(raw)
     b.set_stream(raw)
@@ -258,6 +263,8 @@ def test_graph_replays_match_eager_steps(lib, sys1):
         pb, _, _ = b.get_state()
         assert a.cursor() == b.cursor()
         assert np.abs(pa - pb).max() < 1e-9
+    cnt = a.launch_counts()
+    assert cnt["graph_steps"] == 2 * (50 // gsteps) * gsteps and cnt["eager_steps"] == 2 * (50 % gsteps)
     a.close()
     b.close()
 
