@@ -96,6 +96,11 @@ struct tis_ctx {
     std::vector<int> n_st3;
     int nb_bonded = 0;
     int n_bonded_terms = 0;
+    // the step's multi-body terms by k_multibody (bead forces summed in LDS):
+    // slices per replica and dynamic LDS bytes; else k_force's step path
+    bool mb_lds = false;
+    int mb_slices = 1;
+    size_t mb_lds_bytes = 0;
     int dev_id = 0;
     std::vector<long long> stream_cap;
     long long stream_total = 0;
@@ -348,6 +353,12 @@ struct tis_ctx {
         launch_hb_resolution(s_hb, 1);
         HIP_TRY(hipEventRecord(ev_join, s_hb));
         timed("k_multibody", s_mb, [&] {
+            if (mb_lds) {
+                const dim3 g(mb_slices, n_rep);
+                if (D.precision == TIS_PREC_FP64) k_multibody<double><<<g, kMbBlock, mb_lds_bytes, s_mb>>>(D);
+                else k_multibody<float><<<g, kMbBlock, mb_lds_bytes, s_mb>>>(D);
+                return;
+            }
             const int nmb = n_rep * D.st_slices + nb_bonded;
             if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
             else k_force<float, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
@@ -828,6 +839,62 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             int mx = 1;
             for (const RepParams &q : c->rp) mx = std::max(mx, ((q.nst3 + 63) & ~63) + q.nst2);
             D.st_slices = std::min(kMaxStSlices, (mx + kBlock - 1) / kBlock);
+        }
+        {   // k_multibody: each replica's multi-body terms by type, sorted by their
+            // lowest bead, cut into slices of the replica's beads; taken when the
+            // largest replica's bead forces fit the LDS of a workgroup
+            int max_beads = 1, max_terms = 1;
+            for (const RepParams &q : c->rp) {
+                max_beads = std::max(max_beads, q.nbeads);
+                max_terms = std::max(max_terms, q.nst3 + q.nst2 + q.nbond + q.nang);
+            }
+            c->mb_lds_bytes = 3 * (size_t)max_beads * sizeof(double);
+            c->mb_slices = std::min(kMaxStSlices, std::max(1, (max_terms + 511) / 512));
+            c->mb_lds = c->mb_lds_bytes <= 120 * 1024;
+            if (c->mb_lds) {
+                const int S = c->mb_slices, nbond_tot = (int)bij.size();
+                std::vector<int> terms, offs((size_t)n_rep * (S + 1) * 4);
+                for (int r = 0; r < n_rep; ++r) {
+                    const RepParams &P = c->rp[r];
+                    std::vector<std::pair<int, int>> lists[4];   // (lowest bead, term id) per type
+                    for (int k = 0; k < P.nst3; ++k) {
+                        const int g = P.st30 + k;
+                        lists[0].push_back({*std::min_element(&st3b[6 * (size_t)g], &st3b[6 * (size_t)g] + 6), g});
+                    }
+                    for (int k = 0; k < P.nst2; ++k) {
+                        const int g = P.st20 + k;
+                        lists[1].push_back({*std::min_element(&st2b[8 * (size_t)g], &st2b[8 * (size_t)g] + 7), g});
+                    }
+                    for (int k = 0; k < P.nbond; ++k) {
+                        const int g = P.bond0 + k;
+                        lists[2].push_back({std::min(bij[g].x, bij[g].y), g});
+                    }
+                    for (int k = 0; k < P.nang; ++k) {
+                        const int g = P.ang0 + k;
+                        lists[3].push_back({std::min({aijk[g].x, aijk[g].y, aijk[g].z}), nbond_tot + g});
+                    }
+                    for (int t = 0; t < 4; ++t) {
+                        std::stable_sort(lists[t].begin(), lists[t].end(),
+                                         [](const auto &a, const auto &b) { return a.first < b.first; });
+                        const int base = (int)terms.size();
+                        for (const auto &e : lists[t]) terms.push_back(e.second);
+                        for (int q = 0; q <= S; ++q) {
+                            const int bead = P.p0 + (int)((long long)q * P.nbeads / S);
+                            const auto it = q == S ? lists[t].end()
+                                                   : std::lower_bound(lists[t].begin(), lists[t].end(), bead,
+                                                                      [](const auto &e, int v) { return e.first < v; });
+                            offs[((size_t)r * (S + 1) + q) * 4 + t] = base + (int)(it - lists[t].begin());
+                        }
+                    }
+                }
+                D.mb_terms = dalloc<int>(own, terms.size()); upload(D.mb_terms, terms);
+                D.mb_off = dalloc<int>(own, offs.size()); upload(D.mb_off, offs);
+                D.st_slices = S;
+                HIP_TRY(hipFuncSetAttribute((const void *)k_multibody<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)c->mb_lds_bytes));
+                HIP_TRY(hipFuncSetAttribute((const void *)k_multibody<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)c->mb_lds_bytes));
+            }
         }
         D.maxwell = dalloc<double>(own, 3000 * (size_t)n_rep);
         D.stream_base = dalloc<long long>(own, n_rep);

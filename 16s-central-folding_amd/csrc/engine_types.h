@@ -112,6 +112,8 @@ struct Dev {
     double *st3_E;
     int2 *st3_dis;                // secondary stacks switched off (replica-local index, -1 none)
     int *st2_b;                   // 8 per stack (7 used)
+    int *mb_terms;                // k_multibody: each replica's terms by type, sorted by lowest bead
+    int *mb_off;                  // k_multibody: 4 offsets into mb_terms per (replica, slice boundary)
     double2 *st2_rE;
     int *hb_b;                    // 6 per HB
     double *hb_g;                 // 6 per HB

@@ -100,6 +100,18 @@ TIS_DEV void atomic_add_force(double4 *f, int i, V3<Dual> v) {
     unsafeAtomicAdd(&f[i].y, v.y.d);
     unsafeAtomicAdd(&f[i].z, v.z.d);
 }
+// Forces of one replica's beads summed in LDS (xyz of bead i - p0 at 3 (i - p0)).
+struct LdsForce {
+    double *l;
+    int p0;
+};
+template <class R>
+TIS_DEV void atomic_add_force(LdsForce f, int i, V3<R> v) {
+    double *a = f.l + 3 * (i - f.p0);
+    atomicAdd(a, (double)v.x);
+    atomicAdd(a + 1, (double)v.y);
+    atomicAdd(a + 2, (double)v.z);
+}
 
 // Position of particle i as the force routines of precision R see it: the FP64
 // position, with its tangent from D.jv for derivatives.
@@ -662,8 +674,8 @@ TIS_DEV void sextet_force_parts(V3<R> fb[6], const Pt *p, const double *geo, R w
         for (int c = 0; c < 4; ++c) fb[quad[a][c]] = fb[quad[a][c]] + g[c] * coef;
     }
 }
-template <class R, class Pt>
-TIS_DEV void sextet_forces_at(double4 *F, const int *b, const Pt *p, const double *geo, R w, const R psi0[3],
+template <class R, class Pt, class FT>
+TIS_DEV void sextet_forces_at(FT F, const int *b, const Pt *p, const double *geo, R w, const R psi0[3],
                               R kr, R kth, R kpsi, double *vdotf = nullptr) {
     // the six beads' forces are summed first, then added once each
     V3<R> fb[6];
@@ -1445,8 +1457,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
 // Tertiary stack g (stacking_interactions, fast.cpp:1911-2141): E / (1 + Q6)
 // with Q6 the weighted squared deviations of one distance, two angles and three
 // dihedrals; returns the energy and adds the forces (see sextet_forces for vdotf).
-template <class R>
-TIS_DEV R tertiary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
+template <class R, class FT>
+TIS_DEV R tertiary_stack(const Dev &D, FT F, int g, double *vdotf) {
     using Pt = typename PtOf<R>::type;
     const int *b = D.st3_b + 6 * g;
     const double *geo = D.st3_g + 6 * g;
@@ -1478,8 +1490,8 @@ TIS_DEV R tertiary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
 // Secondary stack g (stacking_interactions, fast.cpp:2060-2141): E / (1 + Q3)
 // over the B-B distance and two backbone dihedrals; returns the energy and adds
 // the forces (vdotf as in sextet_forces).
-template <class R>
-TIS_DEV R secondary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
+template <class R, class FT>
+TIS_DEV R secondary_stack(const Dev &D, FT F, int g, double *vdotf) {
     using Pt = typename PtOf<R>::type;
     const int *b = D.st2_b + 8 * g;   // P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2
     const double2 rE = D.st2_rE[g];
@@ -1520,8 +1532,8 @@ TIS_DEV R secondary_stack(const Dev &D, double4 *F, int g, double *vdotf) {
 // fast.cpp:2354-2520: bead_bond_force k (r - r0)^2, bead_valence_force
 // k (theta - theta0)^2): adds the forces, returns the energy in *eb or *ea and the
 // end beads (i, k) whose WCA the reference removes; false past the list's end.
-template <class R>
-TIS_DEV bool bonded_term(const Dev &D, double4 *F, int g, int nbond, int nang, int *i_out, int *k_out, double *eb,
+template <class R, class FT>
+TIS_DEV bool bonded_term(const Dev &D, FT F, int g, int nbond, int nang, int *i_out, int *k_out, double *eb,
                          double *ea) {
     if (g < nbond) {
         const int2 ij = D.bond_ij[g];
@@ -1751,6 +1763,92 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
             ec = group_sum(ec);
             if (lr && valid && sub == 0) unsafeAtomicAdd(&S.e[EI_COUL_LR], 0.5 * ec);
         }
+    }
+}
+
+// ------------------------------------------------------------ k_multibody
+// The multi-body terms of a step (stacking_interactions fast.cpp:1911-2141,
+// polymer_constraints :2354-2520) with each replica's bead forces summed in LDS:
+// grid (slices, replicas); slice s owns the terms whose lowest bead lies in its
+// share of the replica's beads (term lists sorted by that bead, D.mb_terms, with
+// per-slice offsets D.mb_off: [(r (S + 1) + s) 4 + type], types tertiary stack,
+// secondary stack, bond, angle), so a bead's contributions meet in one or two
+// workgroups and reach f_mb with one atomic per component and workgroup instead
+// of one per term.  Dynamic LDS: 3 doubles per bead of the largest replica.
+// Energies as k_force in a step: tertiary stacks into st_energy, each slice's
+// E2 / E3 stack sums into st_part (summed by k_commit).
+constexpr int kMbBlock = 256;
+template <class R>
+__global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
+    const int sl = blockIdx.x, r = blockIdx.y, S = gridDim.x;
+    const RepParams &P = D.rp[r];
+    extern __shared__ double lf[];
+    __shared__ unsigned char off[4096];
+    __shared__ double red[kMbBlock / 64];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int k = tid; k < 3 * P.nbeads; k += nt) lf[k] = 0.0;
+    for (int k = tid; k < P.nst2; k += nt) off[k] = 0;
+    __syncthreads();
+    // secondary stacks switched off by formed tertiary stacks (every slice)
+    for (int k = tid; k < P.nst3; k += nt) {
+        const int g = P.st30 + k;
+        const int *b = D.st3_b + 6 * g;
+        V3<R> v = diff<R>(D.pos[b[1]], D.pos[b[4]]);
+        R dr = sqrt(dot(v, v)) - (R)D.st3_g[6 * g];
+        if (dr < (R)10) {
+            const int2 dd = D.st3_dis[g];
+            if (dd.x >= 0) off[dd.x] = 1;
+            if (dd.y >= 0) off[dd.y] = 1;
+        }
+    }
+    __syncthreads();
+    const LdsForce F{lf, P.p0};
+    const int *o0 = D.mb_off + 4 * ((size_t)r * (S + 1) + sl), *o1 = o0 + 4;
+    double e2 = 0.0, e3 = 0.0;
+    for (int q = o0[0] + tid; q < o1[0]; q += nt) {
+        const int g = D.mb_terms[q];
+        const double e = (double)tertiary_stack<R>(D, F, g, nullptr);
+        D.st_energy[g] = e;
+        e3 += e;
+    }
+    for (int q = o0[1] + tid; q < o1[1]; q += nt) {
+        const int g = D.mb_terms[q];
+        if (!off[g - P.st20]) e2 += (double)secondary_stack<R>(D, F, g, nullptr);
+    }
+    const int nbond = D.rp[D.n_rep - 1].bond0 + D.rp[D.n_rep - 1].nbond;
+    const int nang = D.rp[D.n_rep - 1].ang0 + D.rp[D.n_rep - 1].nang;
+    const int nbs = o1[2] - o0[2], nts = nbs + o1[3] - o0[3];
+    for (int t = tid; t < nts; t += nt) {   // the slice's bonds, then its angles
+        const int g = D.mb_terms[t < nbs ? o0[2] + t : o0[3] + t - nbs];
+        int i, k;
+        double eb, ea;
+        bonded_term<R>(D, F, g, nbond, nang, &i, &k, &eb, &ea);
+        if (!D.bsub[g]) continue;
+        // a pair listed by a second bonded term: its WCA removed once more (the
+        // step leaves bonded / 1-3 pairs out of the pair sum; k_force explains)
+        double3 dd;
+        const double r2 = min_image_r2(D.pos[i], D.pos[k], P.L, P.hL, &dd);
+        const int ab = (D.ptype[i] & 15) * kNT + (D.ptype[k] & 15);
+        const R f = -wca_factor<R>(r2, ab, P);
+        V3<R> v = {(R)dd.x * f, (R)dd.y * f, (R)dd.z * f};
+        atomic_add_force(F, i, v);
+        atomic_add_force(F, k, v * (R)-1);
+    }
+    __syncthreads();
+    for (int k = tid; k < P.nbeads; k += nt) {
+        const double x = lf[3 * k], y = lf[3 * k + 1], z = lf[3 * k + 2];
+        if (x != 0.0 || y != 0.0 || z != 0.0) {
+            double4 *f = D.f_mb + P.p0 + k;
+            unsafeAtomicAdd(&f->x, x);
+            unsafeAtomicAdd(&f->y, y);
+            unsafeAtomicAdd(&f->z, z);
+        }
+    }
+    const double s2 = block_sum_double(e2, red);
+    const double s3 = block_sum_double(e3, red);
+    if (tid == 0) {
+        D.st_part[2 * (kMaxStSlices * r + sl)] = s2;
+        D.st_part[2 * (kMaxStSlices * r + sl) + 1] = s3;
     }
 }
 
