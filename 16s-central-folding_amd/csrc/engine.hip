@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -363,6 +364,8 @@ struct tis_ctx {
             if (D.precision == TIS_PREC_FP64) k_force<double, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
             else k_force<float, 0><<<nmb, kBlock, 0, s_mb>>>(D, nb_bonded, 31u, kEvalStep);
         });
+        // the step's random forces (Philox draws), off the integrator's path
+        if (D.rng == TIS_RNG_PHILOX) timed("k_noise", s_mb, [&] { k_noise<<<grid(D.n_tot), kBlock, 0, s_mb>>>(D); });
         HIP_TRY(hipEventRecord(ev_join_mb, s_mb));
         if (D.hb_lag) {   // after the multi-body terms on their stream (the shorter chain)
             launch_rebuild_hb(s_mb, 1);
@@ -377,7 +380,7 @@ struct tis_ctx {
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));
-        timed("k_commit", s_main, [&] { k_commit<<<1, kBlock, 0, s_main>>>(D); });
+        timed("k_commit", s_main, [&] { k_commit<<<grid(64 * n_rep), kBlock, 0, s_main>>>(D); });
         set_parity(par ^ 1);   // the new state is in the other buffers
     }
 
@@ -652,6 +655,8 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         c->buf_fmb = dalloc<double4>(own, 2 * (size_t)N);
         c->set_parity(0);
         D.f_sr = dalloc<double4>(own, N);
+        D.nz = dalloc<double4>(own, N);
+        D.maxd2 = dalloc<unsigned long long>(own, (size_t)n_rep * kMaxSlots * kSlotStride);
         D.f_lr = dalloc<double4>(own, N);
         D.ptype = dalloc<int>(own, N);
         D.prep = dalloc<int>(own, N);
@@ -838,57 +843,78 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             // tertiary stacks padded to whole wavefronts)
             int mx = 1;
             for (const RepParams &q : c->rp) mx = std::max(mx, ((q.nst3 + 63) & ~63) + q.nst2);
-            D.st_slices = std::min(kMaxStSlices, (mx + kBlock - 1) / kBlock);
+            D.st_slices = std::min(16, (mx + kBlock - 1) / kBlock);
         }
         {   // k_multibody: each replica's multi-body terms by type, sorted by their
-            // lowest bead, cut into slices of the replica's beads; taken when the
-            // largest replica's bead forces fit the LDS of a workgroup
-            int max_beads = 1, max_terms = 1;
-            for (const RepParams &q : c->rp) {
-                max_beads = std::max(max_beads, q.nbeads);
-                max_terms = std::max(max_terms, q.nst3 + q.nst2 + q.nbond + q.nang);
-            }
-            c->mb_lds_bytes = 3 * (size_t)max_beads * sizeof(double);
-            c->mb_slices = std::min(kMaxStSlices, std::max(1, (max_terms + 511) / 512));
-            c->mb_lds = c->mb_lds_bytes <= 120 * 1024;
-            if (c->mb_lds) {
-                const int S = c->mb_slices, nbond_tot = (int)bij.size();
-                std::vector<int> terms, offs((size_t)n_rep * (S + 1) * 4);
-                for (int r = 0; r < n_rep; ++r) {
-                    const RepParams &P = c->rp[r];
-                    std::vector<std::pair<int, int>> lists[4];   // (lowest bead, term id) per type
-                    for (int k = 0; k < P.nst3; ++k) {
-                        const int g = P.st30 + k;
-                        lists[0].push_back({*std::min_element(&st3b[6 * (size_t)g], &st3b[6 * (size_t)g] + 6), g});
+            // lowest bead and cut into slices of the replica's beads (one slice per
+            // 256 terms of the largest replica), with the bead window each slice's
+            // local terms reach; taken when the largest window fits the LDS
+            int max_terms = 1;
+            for (const RepParams &q : c->rp) max_terms = std::max(max_terms, q.nst3 + q.nst2 + q.nbond + q.nang);
+            const int S = std::min(kMaxStSlices, std::max(1, (max_terms + 255) / 256)), nbond_tot = (int)bij.size();
+            std::vector<int> terms, offs((size_t)n_rep * (S + 1) * 4), win(2 * (size_t)n_rep * S);
+            int max_win = 1;
+            for (int r = 0; r < n_rep; ++r) {
+                const RepParams &P = c->rp[r];
+                // (lowest bead, highest bead, term id) per type
+                std::vector<std::array<int, 3>> lists[4];
+                auto span = [](const int *b, int n) {
+                    return std::make_pair(*std::min_element(b, b + n), *std::max_element(b, b + n));
+                };
+                for (int k = 0; k < P.nst3; ++k) {
+                    const int g = P.st30 + k;
+                    const auto m = span(&st3b[6 * (size_t)g], 6);
+                    lists[0].push_back({m.first, m.second, g});
+                }
+                for (int k = 0; k < P.nst2; ++k) {
+                    const int g = P.st20 + k;
+                    const auto m = span(&st2b[8 * (size_t)g], 7);
+                    lists[1].push_back({m.first, m.second, g});
+                }
+                for (int k = 0; k < P.nbond; ++k) {
+                    const int g = P.bond0 + k, b2[2] = {bij[g].x, bij[g].y};
+                    const auto m = span(b2, 2);
+                    lists[2].push_back({m.first, m.second, g});
+                }
+                for (int k = 0; k < P.nang; ++k) {
+                    const int g = P.ang0 + k, b3[3] = {aijk[g].x, aijk[g].y, aijk[g].z};
+                    const auto m = span(b3, 3);
+                    lists[3].push_back({m.first, m.second, nbond_tot + g});
+                }
+                for (int q = 0; q < S; ++q) {
+                    win[2 * ((size_t)r * S + q)] = P.p0 + (int)((long long)q * P.nbeads / S);
+                    win[2 * ((size_t)r * S + q) + 1] = P.p0 + (int)((long long)(q + 1) * P.nbeads / S);
+                }
+                for (int t = 0; t < 4; ++t) {
+                    std::stable_sort(lists[t].begin(), lists[t].end(),
+                                     [](const auto &a, const auto &b) { return a[0] < b[0]; });
+                    const int base = (int)terms.size();
+                    for (const auto &e : lists[t]) terms.push_back(e[2]);
+                    for (int q = 0; q <= S; ++q) {
+                        const int bead = P.p0 + (int)((long long)q * P.nbeads / S);
+                        const auto it = q == S ? lists[t].end()
+                                               : std::lower_bound(lists[t].begin(), lists[t].end(), bead,
+                                                                  [](const auto &e, int v) { return e[0] < v; });
+                        offs[((size_t)r * (S + 1) + q) * 4 + t] = base + (int)(it - lists[t].begin());
                     }
-                    for (int k = 0; k < P.nst2; ++k) {
-                        const int g = P.st20 + k;
-                        lists[1].push_back({*std::min_element(&st2b[8 * (size_t)g], &st2b[8 * (size_t)g] + 7), g});
-                    }
-                    for (int k = 0; k < P.nbond; ++k) {
-                        const int g = P.bond0 + k;
-                        lists[2].push_back({std::min(bij[g].x, bij[g].y), g});
-                    }
-                    for (int k = 0; k < P.nang; ++k) {
-                        const int g = P.ang0 + k;
-                        lists[3].push_back({std::min({aijk[g].x, aijk[g].y, aijk[g].z}), nbond_tot + g});
-                    }
-                    for (int t = 0; t < 4; ++t) {
-                        std::stable_sort(lists[t].begin(), lists[t].end(),
-                                         [](const auto &a, const auto &b) { return a.first < b.first; });
-                        const int base = (int)terms.size();
-                        for (const auto &e : lists[t]) terms.push_back(e.second);
-                        for (int q = 0; q <= S; ++q) {
-                            const int bead = P.p0 + (int)((long long)q * P.nbeads / S);
-                            const auto it = q == S ? lists[t].end()
-                                                   : std::lower_bound(lists[t].begin(), lists[t].end(), bead,
-                                                                      [](const auto &e, int v) { return e.first < v; });
-                            offs[((size_t)r * (S + 1) + q) * 4 + t] = base + (int)(it - lists[t].begin());
-                        }
+                    if (t == 0) continue;   // tertiary stacks add to f_mb directly
+                    for (int q = 0; q < S; ++q) {   // window end: the highest bead of the slice's terms
+                        const int a = offs[((size_t)r * (S + 1) + q) * 4 + t] - base;
+                        const int b = offs[((size_t)r * (S + 1) + q + 1) * 4 + t] - base;
+                        int &end = win[2 * ((size_t)r * S + q) + 1];
+                        for (int e = a; e < b; ++e) end = std::max(end, lists[t][e][1] + 1);
                     }
                 }
+                for (int q = 0; q < S; ++q)
+                    max_win = std::max(max_win, win[2 * ((size_t)r * S + q) + 1] - win[2 * ((size_t)r * S + q)]);
+            }
+            c->mb_slices = S;
+            c->mb_lds_bytes = 3 * (size_t)max_win * sizeof(double);
+            c->mb_lds = c->mb_lds_bytes <= 120 * 1024;
+            if (c->mb_lds) {
                 D.mb_terms = dalloc<int>(own, terms.size()); upload(D.mb_terms, terms);
                 D.mb_off = dalloc<int>(own, offs.size()); upload(D.mb_off, offs);
+                D.mb_win = dalloc<int>(own, win.size()); upload(D.mb_win, win);
                 D.st_slices = S;
                 HIP_TRY(hipFuncSetAttribute((const void *)k_multibody<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)c->mb_lds_bytes));
@@ -1023,9 +1049,10 @@ extern "C" int tis_set_state(tis_ctx *c, int rep, const double *pos, const doubl
         all[rep].klok = klok;
         all[rep].progress = 0;
         all[rep].lr_due = 1;
-        for (unsigned long long &m : all[rep].maxd2) m = 0;
         all[rep].overflow = 0;
         HIP_TRY(hipMemcpy(c->D.rs, all.data(), c->n_rep * sizeof(RepState), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(c->D.maxd2 + (size_t)rep * kMaxSlots * kSlotStride, 0,
+                               (size_t)kMaxSlots * kSlotStride * sizeof(unsigned long long), c->s_main));
         c->launch_rebuild();
         HIP_TRY(hipStreamSynchronize(c->s_main));
         HIP_TRY(hipMemcpy(all.data(), c->D.rs, c->n_rep * sizeof(RepState), hipMemcpyDeviceToHost));

@@ -14,7 +14,8 @@ constexpr int kNT = 9;            // parameter types
 constexpr int kMaxCells = 8192;   // cells per replica (LDS counting sort)
 constexpr int kBlock = 256;
 constexpr int kExcl = 8;          // bonded and 1-3 partners per particle
-constexpr int kMaxSlots = 8;      // words of a replica's largest-displacement maximum
+constexpr int kMaxSlots = 32;     // words of a replica's largest-displacement maximum
+constexpr int kSlotStride = 16;   // one 128-byte line per word (unsigned long long units)
 
 // Static parameters of one replica (read-only on the device).
 struct RepParams {
@@ -41,7 +42,6 @@ struct RepParams {
 struct RepState {
     long long klok;               // step about to be executed
     double progress;              // PROGRESS, fast.cpp:1768-1775
-    unsigned long long maxd2[8];  // bits of max |x - x_old|^2 of the current step, in kMaxSlots words
     int need_rebuild;
     int lr_due;
     long long n_rebuilds;
@@ -89,7 +89,10 @@ struct Dev {
     // swaps the pointers
     double4 *pos, *old, *vel;
     double4 *pos_nx, *old_nx, *vel_nx;
-    double4 *f_sr, *f_lr, *f_mb, *f_mb_nx;
+    double4 *f_sr, *f_lr, *f_mb, *f_mb_nx;   // f_mb: RNA beads only (ions' entries stay zero)
+    double4 *nz;                  // random force of the step (Philox draws; k_noise)
+    unsigned long long *maxd2;    // bits of max |x - x_old|^2 of the current step: kMaxSlots words
+                                  // per replica, word q of replica r at (r kMaxSlots + q) kSlotStride
     double4 *jv, *jf;             // force matching: position tangent in, force tangent out
     int *ptype;                   // type | cls << 4 | charged << 8
     int *prep;                    // replica of each particle
@@ -114,6 +117,7 @@ struct Dev {
     int *st2_b;                   // 8 per stack (7 used)
     int *mb_terms;                // k_multibody: each replica's terms by type, sorted by lowest bead
     int *mb_off;                  // k_multibody: 4 offsets into mb_terms per (replica, slice boundary)
+    int *mb_win;                  // k_multibody: bead window (first, end) per (replica, slice)
     double2 *st2_rE;
     int *hb_b;                    // 6 per HB
     double *hb_g;                 // 6 per HB

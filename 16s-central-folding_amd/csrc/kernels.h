@@ -1574,7 +1574,7 @@ TIS_DEV bool bonded_term(const Dev &D, FT F, int g, int nbond, int nang, int *i_
 
 // ------------------------------------------------------------ k_force
 constexpr int kPairLanes = 4;   // default lanes per particle in the pair gather (D.pair_lanes)
-constexpr int kMaxStSlices = 16;  // stack workgroups per replica in a step (D.st_slices)
+constexpr int kMaxStSlices = 64;  // stack workgroups per replica in a step (D.st_slices)
 // ROLE 0 (multi-body): blocks [0, n_rep) stacking of replica b, then the
 // backbone terms.  ROLE 1: pair gather.  The two roles are separate kernels so
 // that the register-heavy stacking code does not lower the occupancy of the
@@ -1772,11 +1772,15 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
 // grid (slices, replicas); slice s owns the terms whose lowest bead lies in its
 // share of the replica's beads (term lists sorted by that bead, D.mb_terms, with
 // per-slice offsets D.mb_off: [(r (S + 1) + s) 4 + type], types tertiary stack,
-// secondary stack, bond, angle), so a bead's contributions meet in one or two
-// workgroups and reach f_mb with one atomic per component and workgroup instead
-// of one per term.  Dynamic LDS: 3 doubles per bead of the largest replica.
-// Energies as k_force in a step: tertiary stacks into st_energy, each slice's
-// E2 / E3 stack sums into st_part (summed by k_commit).
+// secondary stack, bond, angle).  The secondary stacks, bonds and angles of a
+// slice span a few beads past its share, the bead window D.mb_win[2 (r S + s)]
+// (first bead, end); their forces are summed in LDS over that window, so a
+// bead's contributions meet in one or two workgroups and reach f_mb with one
+// atomic per component and workgroup instead of one per term.  Tertiary stacks
+// join distant beads and add theirs to f_mb directly.  Dynamic LDS: 3 doubles
+// per bead of the largest window.  Energies as k_force in a step: tertiary
+// stacks into st_energy, each slice's E2 / E3 stack sums into st_part (summed
+// by k_commit).
 constexpr int kMbBlock = 256;
 template <class R>
 __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
@@ -1786,7 +1790,8 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
     __shared__ unsigned char off[4096];
     __shared__ double red[kMbBlock / 64];
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int k = tid; k < 3 * P.nbeads; k += nt) lf[k] = 0.0;
+    const int w_lo = D.mb_win[2 * (r * S + sl)], w_n = D.mb_win[2 * (r * S + sl) + 1] - w_lo;
+    for (int k = tid; k < 3 * w_n; k += nt) lf[k] = 0.0;
     for (int k = tid; k < P.nst2; k += nt) off[k] = 0;
     __syncthreads();
     // secondary stacks switched off by formed tertiary stacks (every slice)
@@ -1802,12 +1807,12 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
         }
     }
     __syncthreads();
-    const LdsForce F{lf, P.p0};
+    const LdsForce F{lf, w_lo};
     const int *o0 = D.mb_off + 4 * ((size_t)r * (S + 1) + sl), *o1 = o0 + 4;
     double e2 = 0.0, e3 = 0.0;
     for (int q = o0[0] + tid; q < o1[0]; q += nt) {
         const int g = D.mb_terms[q];
-        const double e = (double)tertiary_stack<R>(D, F, g, nullptr);
+        const double e = (double)tertiary_stack<R>(D, D.f_mb, g, nullptr);
         D.st_energy[g] = e;
         e3 += e;
     }
@@ -1835,10 +1840,10 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
         atomic_add_force(F, k, v * (R)-1);
     }
     __syncthreads();
-    for (int k = tid; k < P.nbeads; k += nt) {
+    for (int k = tid; k < w_n; k += nt) {
         const double x = lf[3 * k], y = lf[3 * k + 1], z = lf[3 * k + 2];
         if (x != 0.0 || y != 0.0 || z != 0.0) {
-            double4 *f = D.f_mb + P.p0 + k;
+            double4 *f = D.f_mb + w_lo + k;
             unsafeAtomicAdd(&f->x, x);
             unsafeAtomicAdd(&f->y, y);
             unsafeAtomicAdd(&f->z, z);
@@ -1873,17 +1878,46 @@ TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, const RepState &S, 
     }
 }
 
-// leap_frog_move_atom (fast.cpp:570-642) for particle p; returns the new position
-// and velocity (position before recentring).
-TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double xn[3],
-                      double vn[3], double xs[3]) {
-    const int t = D.ptype[p] & 15;
-    const double4 f0 = D.f_sr[p], f1 = D.f_lr[p], f2 = D.f_mb[p];
+// Random force sigma_type xi of particle p in the current step (full_forces,
+// fast.cpp:2598-2624).  With Philox draws it depends only on the step counter,
+// the particle and the key, so k_noise computes it beside the step's force
+// kernels into D.nz; the reference stream's table is made by the HB resolution
+// of the same step, so that mode forms it in the integrator.
+TIS_DEV double3 random_force(const Dev &D, int r, const RepParams &P, const RepState &S, int p, int t) {
+    if (D.rng == 0) {
+        const double4 z = D.nz[p];
+        return make_double3(z.x, z.y, z.z);
+    }
     double nz[3];
     noise3(D, r, P, S, p, nz);
     const double sg = P.sigma[t] * D.noise_scale;
-    const double f[3] = {f0.x + f1.x + f2.x + nz[0] * sg, f0.y + f1.y + f2.y + nz[1] * sg,
-                         f0.z + f1.z + f2.z + nz[2] * sg};
+    return make_double3(nz[0] * sg, nz[1] * sg, nz[2] * sg);
+}
+
+// k_noise (Philox draws only): D.nz[p] = sigma_type * noise_scale * xi for the
+// step about to run, one thread per particle.
+__global__ void __launch_bounds__(kBlock) k_noise(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.n_tot) return;
+    const int r = D.prep[p], t = D.ptype[p] & 15;
+    const RepParams &P = D.rp[r];
+    double nz[3];
+    noise3(D, r, P, D.rs[r], p, nz);
+    const double sg = P.sigma[t] * D.noise_scale;
+    D.nz[p] = make_double4(nz[0] * sg, nz[1] * sg, nz[2] * sg, 0.0);
+}
+
+// leap_frog_move_atom (fast.cpp:570-642) for particle p; returns the new position
+// and velocity (position before recentring).  Only RNA beads carry multi-body
+// forces (ions' f_mb entries stay zero and are not read).
+TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double xn[3],
+                      double vn[3], double xs[3]) {
+    const int t = D.ptype[p] & 15;
+    const bool bead = p - P.p0 < P.nbeads;
+    const double4 f0 = D.f_sr[p], f1 = D.f_lr[p];
+    const double4 f2 = bead ? D.f_mb[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double3 nz = random_force(D, r, P, S, p, t);
+    const double f[3] = {f0.x + f1.x + f2.x + nz.x, f0.y + f1.y + f2.y + nz.y, f0.z + f1.z + f2.z + nz.z};
     const double4 x = D.pos[p], xo = D.old[p], v = D.vel[p];
     const double xv[3] = {x.x, x.y, x.z}, xov[3] = {xo.x, xo.y, xo.z}, vv[3] = {v.x, v.y, v.z};
     for (int c = 0; c < 3; ++c) vn[c] = P.K1[t] * vv[c] + P.K2[t] * f[c];
@@ -1932,7 +1966,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         D.pos_nx[p] = make_double4(xn[0], xn[1], xn[2], 0.0);
         D.old_nx[p] = make_double4(on[0], on[1], on[2], 0.0);
         D.vel_nx[p] = make_double4(vn[0], vn[1], vn[2], 0.0);
-        D.f_mb_nx[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (p - P.p0 < P.nbeads) D.f_mb_nx[p] = make_double4(0.0, 0.0, 0.0, 0.0);
         double4 *c = D.cpos + D.crank[p];
         c->x = xn[0];
         c->y = xn[1];
@@ -1941,63 +1975,84 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     // largest displacement of the step per replica (d2 >= 0, so its bits order
     // like the values): a wavefront maximum when the wavefront lies in one
     // replica, then one atomic per wavefront into one of kMaxSlots words of the
-    // replica (by wavefront index, so no one word takes every wavefront's atomic)
+    // replica, each on a cache line of its own (by wavefront index, so no one
+    // word or line takes every wavefront's atomic)
     const int r0 = __builtin_amdgcn_readfirstlane(rp);
     const int slot = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kMaxSlots;
     if (__ballot(rp == r0) == ~0ull) {
         double m = d2;
         for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
         if ((threadIdx.x & 63) == 0 && r0 >= 0)
-            atomicMax(&D.rs[r0].maxd2[slot], (unsigned long long)__double_as_longlong(m));
+            atomicMax(&D.maxd2[((size_t)r0 * kMaxSlots + slot) * kSlotStride], (unsigned long long)__double_as_longlong(m));
     } else if (rp >= 0) {
-        atomicMax(&D.rs[rp].maxd2[slot], (unsigned long long)__double_as_longlong(d2));
+        atomicMax(&D.maxd2[((size_t)rp * kMaxSlots + slot) * kSlotStride], (unsigned long long)__double_as_longlong(d2));
     }
 }
 
-// Closes the step of every replica (one workgroup, a thread per replica): the
-// list-rebuild decision from the largest displacement k_integrate recorded
-// (check_shifts, fast.cpp:1741-1776), the step's stack and HB energy sums, the
-// HB table switch, the step counter and the LR refresh flag.
+// Closes the step of every replica (a wavefront per replica): the list-rebuild
+// decision from the largest displacement k_integrate recorded (check_shifts,
+// fast.cpp:1741-1776), the step's stack and HB energy sums, the HB table switch,
+// the step counter and the LR refresh flag.  The per-slice, per-slot and
+// per-group parts are read by the lanes at once and reduced across the
+// wavefront (their sums in the order of the parts, as one thread would add them).
 __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
-    {
-        for (int r = threadIdx.x; r < D.n_rep; r += blockDim.x) {
-            RepState &S = D.rs[r];
-            const RepParams &P = D.rp[r];
-            if (D.st_slices > 1) {   // stack energies of the step's slices
-                double s2 = 0, s3 = 0;
-                for (int q = 0; q < D.st_slices; ++q) {
-                    s2 += D.st_part[2 * (kMaxStSlices * r + q)];
-                    s3 += D.st_part[2 * (kMaxStSlices * r + q) + 1];
-                }
-                S.e[EI_E2_ST] = s2;
-                S.e[EI_E3_ST] = s3;
-            }
-            unsigned long long mb = 0;
-            for (int q = 0; q < kMaxSlots; ++q) {
-                mb = mb > S.maxd2[q] ? mb : S.maxd2[q];
-                S.maxd2[q] = 0;
-            }
-            const double m = sqrt(__longlong_as_double((long long)mb));
-            // a NaN or infinite coordinate (its bits order above every finite
-            // displacement) stops the run with an error instead of propagating
-            if (!(m < 1e30)) S.overflow |= 32;
-            S.progress += m + m;
-            S.need_rebuild = S.progress > P.skin;
-            if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
-            hb_reduce_parts(D, r);   // the resolution groups of this step
-            S.tab_fresh = 0;
-            if (S.tab_pending) {   // HB tables built beside this step's resolution
-                S.tab ^= 1;
-                S.n_act = S.nx_act;
-                S.n_asite = S.nx_asite;
-                S.n_ainc = S.nx_ainc;
-                S.tab_pending = 0;
-                S.tab_fresh = 1;
-            }
-            S.klok += 1;
-            S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
-        }
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (r >= D.n_rep) return;
+    RepState &S = D.rs[r];
+    const RepParams &P = D.rp[r];
+    const int ns = D.st_slices > 1 ? D.st_slices : 0;
+    double s2 = 0.0, s3 = 0.0, h2 = 0.0, h3 = 0.0;
+    unsigned long long mb = 0;
+    int it = 0;
+    if (lane < ns) {
+        s2 = D.st_part[2 * (kMaxStSlices * r + lane)];
+        s3 = D.st_part[2 * (kMaxStSlices * r + lane) + 1];
     }
+    if (lane < kMaxSlots) {
+        unsigned long long *w = D.maxd2 + ((size_t)r * kMaxSlots + lane) * kSlotStride;
+        mb = *w;
+        *w = 0;
+    }
+    if (lane < D.hb_groups) {
+        h2 = D.hb_e_part[2 * (kMaxHbGroups * r + lane)];
+        h3 = D.hb_e_part[2 * (kMaxHbGroups * r + lane) + 1];
+        it = D.hb_iter_part[kMaxHbGroups * r + lane];
+    }
+    // lane 0 adds the parts in order (a readlane each; the sums match a serial loop)
+    double t2 = 0.0, t3 = 0.0, u2 = 0.0, u3 = 0.0;
+    for (int q = 0; q < ns; ++q) { t2 += readlane_d(s2, q); t3 += readlane_d(s3, q); }
+    for (int q = 0; q < D.hb_groups; ++q) { u2 += readlane_d(h2, q); u3 += readlane_d(h3, q); }
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long x = (unsigned long long)__shfl_xor((long long)mb, o, 64);
+        mb = mb > x ? mb : x;
+        it = max(it, __shfl_xor(it, o, 64));
+    }
+    if (lane != 0) return;
+    if (ns) {
+        S.e[EI_E2_ST] = t2;
+        S.e[EI_E3_ST] = t3;
+    }
+    S.e[EI_E2_HB] = u2;
+    S.e[EI_E3_HB] = u3;
+    S.hb_iters = it;
+    const double m = sqrt(__longlong_as_double((long long)mb));
+    // a NaN or infinite coordinate (its bits order above every finite
+    // displacement) stops the run with an error instead of propagating
+    if (!(m < 1e30)) S.overflow |= 32;
+    S.progress += m + m;
+    S.need_rebuild = S.progress > P.skin;
+    if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+    S.tab_fresh = 0;
+    if (S.tab_pending) {   // HB tables built beside this step's resolution
+        S.tab ^= 1;
+        S.n_act = S.nx_act;
+        S.n_asite = S.nx_asite;
+        S.n_ainc = S.nx_ainc;
+        S.tab_pending = 0;
+        S.tab_fresh = 1;
+    }
+    S.klok += 1;
+    S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
 }
 
 // ------------------------------------------------------------ replica exchange
