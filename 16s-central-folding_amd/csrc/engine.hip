@@ -17,6 +17,7 @@
 #include "../../include/tis_gpu.h"
 #include "backward.h"
 #include "observe.h"
+#include "persist.h"
 
 namespace tis_host {
 void set_error(const std::string &msg);
@@ -86,6 +87,23 @@ double *scratch_on_device(int dev) {
 }
 }  // namespace
 
+// Slots of c_pdev held by live contexts (a launch's descriptions go to its
+// context's slot, so contexts on other streams never overwrite them).
+namespace {
+std::mutex g_pslot_mu;
+bool g_pslot_used[kPSlots] = {};
+int take_persist_slot() {
+    std::lock_guard<std::mutex> lock(g_pslot_mu);
+    for (int k = 0; k < kPSlots; ++k)
+        if (!g_pslot_used[k]) { g_pslot_used[k] = true; return k; }
+    return -1;
+}
+void give_persist_slot(int k) {
+    std::lock_guard<std::mutex> lock(g_pslot_mu);
+    g_pslot_used[k] = false;
+}
+}  // namespace
+
 struct tis_ctx {
     int n_rep = 0;
     tis_options opt{};
@@ -129,6 +147,14 @@ struct tis_ctx {
     // launches of tis_step: graph replays, steps replayed from graphs, steps
     // launched one by one
     long long n_graph_launch = 0, n_graph_step = 0, n_eager_step = 0;
+    // persistent step (k_persist): taken when set up at creation; launches and steps since
+    bool persist = false, persist_ready = false;
+    std::string persist_why = "not set up";
+    int n_cu = 0, p_G = 0, p_q = 0;
+    int *p_mem = nullptr;             // barrier counters, then status, then the XCC table
+    int p_slot = -1;                  // slot of c_pdev (persist.h)
+    size_t p_bar_ints = 0;
+    long long n_persist_launch = 0, n_persist_step = 0;
     // kernel timing: when profiling is on, every launch of tis_step is bracketed
     // by HIP events on the stream it runs on (no graph), with the device idle
     // before and after it, and the durations are accumulated per kernel name
@@ -157,6 +183,7 @@ struct tis_ctx {
     double *scratch_dev = nullptr;   // this device's `scratch` (tis_force_match_noalloc)
 
     ~tis_ctx() {
+        if (p_slot >= 0) give_persist_slot(p_slot);
         drop_graphs();
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         for (void *p : owned) (void)hipFree(p);
@@ -420,6 +447,33 @@ struct tis_ctx {
         HIP_TRY(hipMemcpy(D.rs + r, &s, sizeof s, hipMemcpyHostToDevice));
     }
     std::vector<RepState> rs_host;    // n_rep, sized at creation (check_flags allocates nothing)
+    int *p_status() const { return p_mem + p_bar_ints; }
+    int *p_xcc() const { return p_mem + p_bar_ints + kPCtr; }
+    // One cooperative launch of k_persist for n steps (n = 0: the placement
+    // check only), from the current state buffers; the host's parity follows.
+    void launch_persist(long long n) {
+        PArgs A{p_slot, p_mem, p_status(), p_xcc(), (int)n, p_G, p_q, hb_groups};
+        Dev Dv[2] = {D, D};
+        std::swap(Dv[1].pos, Dv[1].pos_nx);
+        std::swap(Dv[1].old, Dv[1].old_nx);
+        std::swap(Dv[1].vel, Dv[1].vel_nx);
+        std::swap(Dv[1].f_mb, Dv[1].f_mb_nx);
+        HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_pdev), Dv, sizeof Dv, 2 * sizeof(Dev) * p_slot,
+                                       hipMemcpyHostToDevice, s_main));
+        HIP_TRY(hipMemsetAsync(p_mem, 0, (p_bar_ints + kPCtr) * sizeof(int), s_main));
+        void *args[] = {&A};
+        const void *fn = D.precision == TIS_PREC_FP64 ? (const void *)k_persist<double> : (const void *)k_persist<float>;
+        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(n_cu), dim3(kPBlock), args, kHbLds, s_main));
+        set_parity(par ^ (int)(n & 1));
+        if (n > 0) { n_persist_launch += 1; n_persist_step += n; }
+    }
+    // Status bits of the last persistent launch (synchronises the stream).
+    int persist_status() {
+        int st = 0;
+        HIP_TRY(hipMemcpyAsync(&st, p_status(), sizeof st, hipMemcpyDeviceToHost, s_main));
+        HIP_TRY(hipStreamSynchronize(s_main));
+        return st;
+    }
     void check_flags() {
         std::vector<RepState> &s = rs_host;
         HIP_TRY(hipMemcpy(s.data(), D.rs, n_rep * sizeof(RepState), hipMemcpyDeviceToHost));
@@ -551,6 +605,7 @@ extern "C" void tis_default_options(tis_options *o) {
     o->list_waves = kListWaves;
     o->hb_groups = 4;
     o->hb_table_lag = 1;
+    o->persistent = 0;
 }
 
 static int fail(const Fail &f) {
@@ -575,6 +630,53 @@ static int guarded(F &&body) {
         tis_host::set_error("out of host memory");
         return TIS_ENOMEM;
     }
+}
+
+// Sets up the persistent step (persist.h) when the context and the device allow
+// it: Philox draws, HB tables built beside the resolution, multi-body forces
+// summed in LDS, at least one replica per XCD and kHbLds of LDS for one
+// workgroup per CU; the placement check then runs once (a launch of no steps).
+// Otherwise persist_why says why and tis_step keeps to launch_step.
+static void setup_persistent(tis_ctx *c) {
+    const Dev &D = c->D;
+    auto no = [&](const std::string &why) { c->persist_why = why; };
+    if (!c->opt.persistent) return no("switched off (tis_options.persistent = 0)");
+    if (D.rng != TIS_RNG_PHILOX) return no("reference stream draws");
+    if (!D.hb_lag) return no("HB tables not built beside the resolution");
+    if (!c->mb_lds) return no("multi-body forces not summed in LDS");
+    int coop = 0;
+    HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->dev_id));
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->dev_id));
+    if (!coop) return no("no cooperative launch");
+    if (c->n_cu % kXcds) return no("CU count not a multiple of 8");
+    if (c->n_rep < kXcds) return no("fewer replicas than XCDs");
+    c->p_q = (c->n_rep + kXcds - 1) / kXcds;
+    c->p_G = c->n_cu / kXcds / c->p_q;
+    if (c->p_G < c->hb_groups + 2) return no("too few CUs per replica for the HB groups and the other roles");
+    const size_t sites_lds = 4 * ((size_t)kAIncMax + kASiteMax + 1 + kMaxHbChunks + 1 + 32) + 2 * c->max_site_lds();
+    if (sites_lds > kHbLds) return no("HB sites above the LDS of the persistent workgroup");
+    if (c->mb_lds_bytes + 4096 + 8 * (kPBlock / 64) > kHbLds) return no("multi-body window above the LDS");
+    const void *fn = D.precision == TIS_PREC_FP64 ? (const void *)k_persist<double> : (const void *)k_persist<float>;
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHbLds));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kPBlock, kHbLds));
+    if (per_cu < 1) return no("the persistent workgroup does not fit a CU");
+    c->p_bar_ints = (3 * (size_t)c->n_rep + 1) * kPCtr;
+    c->p_mem = dalloc<int>(c->owned, c->p_bar_ints + kPCtr + c->n_cu);
+    c->p_slot = take_persist_slot();
+    if (c->p_slot < 0) return no("all constant-memory slots of the persistent step in use");
+    int st = 0;
+    try {
+        c->launch_persist(0);
+        st = c->persist_status();
+    } catch (const DeviceError &e) {
+        (void)hipGetLastError();
+        return no("placement check launch failed: " + e.msg);
+    }
+    if (st & PSTAT_PLACEMENT) return no("workgroups of one replica would not share an XCD");
+    if (st) return no("placement check failed");
+    c->persist_ready = c->persist = true;
+    c->persist_why = "";
 }
 
 extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *opt, tis_ctx **out) {
@@ -1006,6 +1108,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             c->put_rs(r, s);
         }
         c->check_flags();
+        setup_persistent(c);
         HIP_TRY(hipGetLastError());
         *out = c;
         return TIS_OK;
@@ -1139,6 +1242,27 @@ extern "C" int tis_prepare_graph(tis_ctx *c, int graph_steps) {
     });
 }
 
+extern "C" int tis_set_persistent(tis_ctx *c, int on) {
+    if (!c || (on != 0 && on != 1)) return fail(Fail{TIS_EINVAL, "tis_set_persistent: bad arguments"});
+    if (on && !c->persist_ready)
+        return fail(Fail{TIS_EINVAL, "tis_set_persistent: the persistent step is not available: " + c->persist_why});
+    c->persist = on != 0;
+    return TIS_OK;
+}
+
+extern "C" int tis_get_persistent_counts(tis_ctx *c, int *active, long long *launches, long long *steps,
+                                         char *why, int why_len) {
+    if (!c) return fail(Fail{TIS_EINVAL, "tis_get_persistent_counts: bad arguments"});
+    if (active) *active = c->persist ? 1 : 0;
+    if (launches) *launches = c->n_persist_launch;
+    if (steps) *steps = c->n_persist_step;
+    if (why && why_len > 0) {
+        std::strncpy(why, c->persist_why.c_str(), (size_t)why_len - 1);
+        why[why_len - 1] = 0;
+    }
+    return TIS_OK;
+}
+
 extern "C" int tis_get_launch_counts(tis_ctx *c, long long *graph_launches, long long *graph_steps,
                                      long long *eager_steps) {
     if (!c) return fail(Fail{TIS_EINVAL, "tis_get_launch_counts: bad arguments"});
@@ -1152,6 +1276,18 @@ extern "C" int tis_get_launch_counts(tis_ctx *c, long long *graph_launches, long
 // graphs of graph_steps and eager launches for the remainder.
 static void run_steps(tis_ctx *c, long long n_steps) {
     long long done = 0;
+    if (c->persist && !c->profiling && c->p_G >= c->hb_groups + 2) {
+        // at most kPersistChunk steps per launch (barrier counters stay small)
+        constexpr long long kPersistChunk = 20000;
+        while (done < n_steps) {
+            const long long m = std::min(kPersistChunk, n_steps - done);
+            c->launch_persist(m);
+            done += m;
+        }
+        if (c->persist_status() & PSTAT_TIMEOUT)
+            throw Fail{TIS_EDEVICE, "persistent step: a barrier wait gave up (workgroups of a replica out of step)"};
+        return;
+    }
     if (c->profiling) {
         // bounded batches so that the event pool stays small
         while (done < n_steps) {

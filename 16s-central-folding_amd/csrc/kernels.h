@@ -81,6 +81,43 @@ TIS_DEV uint32_t philox_key1(const Dev &D, const RepState &S) { return (uint32_t
 
 TIS_DEV double4 d4add(double4 a, double4 b) { return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, 0.0); }
 
+// ------------------------------------------------------------ load policies
+// Loads of the arrays a step writes (positions, forces, lists, HB tables, the
+// replica state).  LdPlain: an ordinary load; the step's separate kernels only
+// read what earlier kernels wrote.  LdCoh: the persistent step (k_persist),
+// whose workgroups hand positions, forces and tables to each other between
+// the phases of one launch.  A device-scope relaxed atomic load (sc1) misses
+// the CU's own L1, which may still hold a line from before the other
+// workgroup's store, and reads the L2 that all workgroups of a replica share
+// (they run on one XCD, and every store has reached that L2 before the
+// barrier that separates the phases).  Wide types load as 8-byte words.
+struct LdPlain {
+    template <class T>
+    TIS_DEV static T ld(const T *p) { return *p; }
+};
+struct LdCoh {
+    template <class T>
+    TIS_DEV static T ld(const T *p) {
+        return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    TIS_DEV static double4 ld(const double4 *p) {
+        const double *q = (const double *)p;
+        return make_double4(ld(q), ld(q + 1), ld(q + 2), ld(q + 3));
+    }
+    TIS_DEV static double2 ld(const double2 *p) {
+        const double *q = (const double *)p;
+        return make_double2(ld(q), ld(q + 1));
+    }
+    TIS_DEV static int2 ld(const int2 *p) {
+        const long long v = ld((const long long *)p);
+        return make_int2((int)(unsigned)v, (int)(v >> 32));
+    }
+    TIS_DEV static int4 ld(const int4 *p) {
+        const long long a = ld((const long long *)p), b = ld((const long long *)p + 1);
+        return make_int4((int)(unsigned)a, (int)(a >> 32), (int)(unsigned)b, (int)(b >> 32));
+    }
+};
+
 // The .w word of a cell-ordered position holds the particle index (low 32 bits)
 // and its type word (high 32 bits), so one 32 B load gives a neighbour's
 // position, identity and type.
@@ -114,13 +151,13 @@ TIS_DEV void atomic_add_force(LdsForce f, int i, V3<R> v) {
 }
 
 // Position of particle i as the force routines of precision R see it: the FP64
-// position, with its tangent from D.jv for derivatives.
-template <class R>
+// position (load policy L), with its tangent from D.jv for derivatives.
+template <class R, class L = LdPlain>
 struct PtLoad {
-    TIS_DEV static double4 at(const Dev &D, int i) { return D.pos[i]; }
+    TIS_DEV static double4 at(const Dev &D, int i) { return L::ld(D.pos + i); }
 };
-template <>
-struct PtLoad<Dual> {
+template <class L>
+struct PtLoad<Dual, L> {
     TIS_DEV static DualPt at(const Dev &D, int i) { return {D.pos[i], D.jv[i]}; }
 };
 
@@ -159,19 +196,18 @@ TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
 }
 
 // ------------------------------------------------------------ k_bin
-__global__ void __launch_bounds__(1024) k_bin(Dev D) {
-    const int r = blockIdx.x;
+// Counting sort of replica r's particles into cells by one workgroup (any
+// multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20.
+template <class L>
+TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch) {
     RepState &S = D.rs[r];
-    if (!S.need_rebuild) return;
     const RepParams &P = D.rp[r];
-    __shared__ int cnt[kMaxCells];
-    __shared__ int scratch[20];
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2], nc = M0 * M1 * M2;
     for (int c = threadIdx.x; c < nc; c += blockDim.x) cnt[c] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
         const int p = P.p0 + i;
-        const double4 x = D.pos[p];
+        const double4 x = L::ld(D.pos + p);
         const double xs[3] = {x.x, x.y, x.z};
         int ci[3];
 #pragma unroll
@@ -204,8 +240,8 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
     __syncthreads();
     for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
         const int p = P.p0 + i;
-        const int q = P.p0 + cnt[D.cell_of[p]] + D.cell_rank[p];
-        const double4 x = D.pos[p];
+        const int q = P.p0 + cnt[L::ld(D.cell_of + p)] + L::ld(D.cell_rank + p);
+        const double4 x = L::ld(D.pos + p);
         D.sorted[q] = p;
         D.crank[p] = q;
         D.cpos[q] = make_double4(x.x, x.y, x.z, cp_pack(p, D.ptype[p]));
@@ -214,6 +250,13 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
         S.max_sr = 0;
         S.max_lr = 0;
     }
+}
+__global__ void __launch_bounds__(1024) k_bin(Dev D) {
+    const int r = blockIdx.x;
+    if (!D.rs[r].need_rebuild) return;
+    __shared__ int cnt[kMaxCells];
+    __shared__ int scratch[20];
+    bin_replica<LdPlain>(D, r, cnt, scratch);
 }
 
 // ------------------------------------------------------------ neighbour cells
@@ -225,6 +268,7 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
 // contiguous ranges of particles, one per lane: lane l < *nseg gets its range's
 // first cell-order index and length (0 for an empty piece).  An axis with fewer
 // than 5 cells contributes each of its cells once.
+template <class L>
 TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *nseg, int *sstart, int *slen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
@@ -257,8 +301,8 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ns
         }
         if (zhi >= zlo) {
             const int base = P.cell0 + (n0 * M1 + n1) * M2;
-            *sstart = D.cell_start[base + zlo];
-            *slen = D.cell_start[base + zhi + 1] - *sstart;
+            *sstart = L::ld(D.cell_start + base + zlo);
+            *slen = L::ld(D.cell_start + base + zhi + 1) - *sstart;
         }
     }
 }
@@ -271,11 +315,11 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ns
 // whatever the cell occupancies; a lane finds its cell by a scalar walk over the
 // few cells that overlap its chunk.  Two chunks are in flight: the load of the
 // next chunk is issued before the current one is tested.
-template <class F>
+template <class L, class F>
 TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
     const int lane = threadIdx.x & 63;
     int ncell, cstart, clen;
-    neighbour_cells(D, P, cell, &ncell, &cstart, &clen);
+    neighbour_cells<L>(D, P, cell, &ncell, &cstart, &clen);
     const int incl = wave_incl_scan(clen);
     const int total = readlane_i(incl, 63);
     if (total == 0) return;
@@ -291,7 +335,7 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
             if (ex >= t0 + 64) break;
             if (t >= ex && t < in) addr = readlane_i(cstart, cc) + (t - ex);
         }
-        *x = addr >= 0 ? cp[addr] : make_double4(0, 0, 0, none);
+        *x = addr >= 0 ? L::ld(cp + addr) : make_double4(0, 0, 0, none);
         *q = P.p0 + addr;
     };
     double4 cur;
@@ -318,20 +362,19 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
 // the replicas whose displacement trigger fired; the others cost one flag read.
 // mode 0: record the largest per-particle counts only (sizes the slots), mode 1: build.
 constexpr int kListWaves = 4096;
-__global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
+// Lists of replica r's particles, wavefront w0 of nw taking particles w0, w0 + nw, ...
+template <class L>
+TIS_DEV void list_replica(const Dev &D, int mode, int r, int w0, int nw) {
     const int lane = threadIdx.x & 63;
-    const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int r = 0; r < D.n_rep; ++r) {
+    {
         RepState &S = D.rs[r];
-        if (!S.need_rebuild) continue;
         const RepParams &P = D.rp[r];
         const long long stride = (long long)P.cap_sr + P.cap_lr;
         for (int t = w0; t < P.n; t += nw) {
-            const double4 xi = D.cpos[P.p0 + t];
+            const double4 xi = L::ld(D.cpos + P.p0 + t);
             const int p = cp_index(xi.w);
             const bool chg_i = (cp_type(xi.w) >> 8) & 1;
-            const int cell = D.cell_of[p];
+            const int cell = L::ld(D.cell_of + p);
             // entries are cell-order indices q of the partners; bonded / 1-3
             // partners of p are stored as ~q, so the step path leaves out their WCA term
             int ex[kExcl];
@@ -340,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
             int *out_sr = D.pool + P.pool0 + (long long)(p - P.p0) * stride;
             int *out_lr = out_sr + P.cap_sr;
             int a = 0, b = 0;
-            for_neighbour_cells_wave(D, P, cell, [&](int j, const double4 &xj, int qj) {
+            for_neighbour_cells_wave<L>(D, P, cell, [&](int j, const double4 &xj, int qj) {
                 bool isr = false, ilr = false;
                 if (j >= 0 && j != p) {
                     const double r2 = min_image_r2(xi, xj, P.L, P.hL);
@@ -374,6 +417,12 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
         }
     }
 }
+__global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
+    const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int r = 0; r < D.n_rep; ++r)
+        if (D.rs[r].need_rebuild) list_replica<LdPlain>(D, mode, r, w0, nw);
+}
 
 // ------------------------------------------------------------ HB rebuild
 // One buffer of the HB rebuild tables (the arrays hold two, tab_* elements apart).
@@ -394,18 +443,17 @@ TIS_DEV HbTab hb_tab(const Dev &D, int b) {
 // after reading them.
 constexpr int kStatusBlock = 1024;
 constexpr int kMaxHbChunks = 64;   // candidates per replica <= kMaxHbChunks * kStatusBlock
-__global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
-    const int r = blockIdx.y;
-    if (!D.rs[r].need_rebuild) return;
+// One chunk of kStatusBlock candidates of replica r (a workgroup of
+// kStatusBlock threads).
+template <class L>
+TIS_DEV void hbstatus_chunk(const Dev &D, int lag, int r, int chunk, int *scratch) {
     const RepParams &P = D.rp[r];
-    if ((int)(blockIdx.x * kStatusBlock) >= P.nhb) return;
-    __shared__ int scratch[20];
-    const int k = blockIdx.x * kStatusBlock + threadIdx.x;
+    const int k = chunk * kStatusBlock + threadIdx.x;
     bool live = false;
     if (k < P.nhb) {
         const int g = P.hb0 + k;
         const int b1 = D.hb_b[6 * g + 1], b4 = D.hb_b[6 * g + 4];
-        live = min_image_r2(D.pos[b1], D.pos[b4], P.L, P.hL) <= P.rsr2;
+        live = min_image_r2(L::ld(D.pos + b1), L::ld(D.pos + b4), P.L, P.hL) <= P.rsr2;
         D.hb_status[g] = live ? 0 : -1;
         if (!lag) D.hb_out[g] = live ? 0 : -1;   // lagged: the resolution running beside owns hb_out
         if (live)
@@ -415,8 +463,15 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
     // chunk offsets
     int tot;
     const int rank = block_excl_sum(live, scratch, &tot);
-    if (live) D.act_tmp[P.hb0 + blockIdx.x * kStatusBlock + rank] = P.hb0 + k;   // the chunk's live ids, packed
-    if (threadIdx.x == 0) D.chunk_cnt[kMaxHbChunks * r + blockIdx.x] = tot;
+    if (live) D.act_tmp[P.hb0 + chunk * kStatusBlock + rank] = P.hb0 + k;   // the chunk's live ids, packed
+    if (threadIdx.x == 0) D.chunk_cnt[kMaxHbChunks * r + chunk] = tot;
+}
+__global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
+    const int r = blockIdx.y;
+    if (!D.rs[r].need_rebuild) return;
+    if ((int)(blockIdx.x * kStatusBlock) >= D.rp[r].nhb) return;
+    __shared__ int scratch[20];
+    hbstatus_chunk<LdPlain>(D, lag, r, blockIdx.x, scratch);
 }
 
 // k_hbsites: one workgroup per rebuilt replica.  Active sites (with at least one
@@ -425,22 +480,25 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
 // incidence), and every live candidate's incidences as active-site indices.
 // Dynamic LDS: 2 * nsite ints.  lag: write the buffer the resolution is not
 // reading (it runs beside this kernel) and leave the switch to k_commit.
-__global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
-    const int r = blockIdx.x;
+// LDS of one workgroup of hbsites_replica: 2 nsite ints (site map, fill
+// counts), the chunk offsets, the site segment offsets and the segments.
+struct HbSitesLds {
+    int *smem, *scratch, *chunk_off, *aoff_l, *seg_l;
+};
+template <class L>
+TIS_DEV void hbsites_replica(const Dev &D, int lag, int r, HbSitesLds W) {
     RepState &S = D.rs[r];
-    if (!S.need_rebuild) return;
     const RepParams &P = D.rp[r];
-    const HbTab T = hb_tab(D, lag ? S.tab ^ 1 : S.tab);
-    extern __shared__ int smem[];
-    int *site_map = smem, *fill = smem + P.nsite;
-    __shared__ int scratch[20];
-    __shared__ int chunk_off[kMaxHbChunks + 1];
-    __shared__ int aoff_l[kASiteMax + 1];   // site segment offsets
-    __shared__ int seg_l[kAIncMax];         // (static rank << 16 | live index) per site segment
+    const int tab = L::ld(&S.tab);
+    const HbTab T = hb_tab(D, lag ? tab ^ 1 : tab);
+    int *site_map = W.smem, *fill = W.smem + P.nsite;
+    int *scratch = W.scratch, *chunk_off = W.chunk_off;
+    int *aoff_l = W.aoff_l;   // site segment offsets
+    int *seg_l = W.seg_l;     // (static rank << 16 | live index) per site segment
     // live candidates in candidate order: chunk offsets, then chunk-local ranks
     const int nchunk = (P.nhb + kStatusBlock - 1) / kStatusBlock;
     if (threadIdx.x < 64) {
-        const int c = threadIdx.x < nchunk ? D.chunk_cnt[kMaxHbChunks * r + threadIdx.x] : 0;
+        const int c = threadIdx.x < nchunk ? L::ld(D.chunk_cnt + kMaxHbChunks * r + threadIdx.x) : 0;
         const int incl = wave_incl_scan(c);
         chunk_off[threadIdx.x] = incl - c;
         if (threadIdx.x == 63) chunk_off[kMaxHbChunks] = incl;
@@ -457,7 +515,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
                 if (chunk_off[mid] <= a) lo = mid;
                 else hi = mid;
             }
-            const int g = D.act_tmp[P.hb0 + lo * kStatusBlock + (a - chunk_off[lo])];
+            const int g = L::ld(D.act_tmp + P.hb0 + lo * kStatusBlock + (a - chunk_off[lo]));
             T.act_hb[P.hb0 + a] = g;
             const int *hb = D.hb_b + 6 * g;
             T.act_rec[2 * (P.hb0 + a)] = make_int4(g, hb[0], hb[1], hb[2]);
@@ -476,7 +534,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
     const int s_lo = min(P.nsite, (int)threadIdx.x * per), s_hi = min(P.nsite, s_lo + per);
     int my_as = 0, my_inc = 0;
     for (int s = s_lo; s < s_hi; ++s) {
-        const int c = D.site_cnt[P.site0 + s];
+        const int c = L::ld(D.site_cnt + P.site0 + s);
         my_as += c > 0;
         my_inc += c;
     }
@@ -484,7 +542,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
     int rank = block_excl_sum(my_as, scratch, &n_as);
     int off = block_excl_sum(my_inc, scratch, &n_inc);
     for (int s = s_lo; s < s_hi; ++s) {
-        const int cnt = D.site_cnt[P.site0 + s];
+        const int cnt = L::ld(D.site_cnt + P.site0 + s);
         D.site_cnt[P.site0 + s] = 0;   // cleared for the next build
         fill[s] = 0;
         if (cnt > 0) {
@@ -515,7 +573,7 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
         const int a = a0 + threadIdx.x;
         int c = 0, g = 0;
         if (a < n_act) {
-            g = T.act_hb[P.hb0 + a];
+            g = L::ld(T.act_hb + P.hb0 + a);
             c = D.hb_inc_off[g + 1] - D.hb_inc_off[g];
         }
         int tot;
@@ -559,6 +617,16 @@ __global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
             S.n_ainc = n_inc;
         }
     }
+}
+__global__ void __launch_bounds__(1024) k_hbsites(Dev D, int lag) {
+    const int r = blockIdx.x;
+    if (!D.rs[r].need_rebuild) return;
+    extern __shared__ int smem[];
+    __shared__ int scratch[20];
+    __shared__ int chunk_off[kMaxHbChunks + 1];
+    __shared__ int aoff_l[kASiteMax + 1];
+    __shared__ int seg_l[kAIncMax];
+    hbsites_replica<LdPlain>(D, lag, r, HbSitesLds{smem, scratch, chunk_off, aoff_l, seg_l});
 }
 
 TIS_DEV int ceil_index(double u, int n) {   // j = ceil(u n) clamped to 1..n (fast.cpp:2257)
@@ -707,14 +775,15 @@ struct HbDraws {
     uint32_t c0, c1, k0, k1;
     const int32_t *stream;
     long long cur, lim;
-    TIS_DEV HbDraws(const Dev &D, int r, const RepState &S) {
+    // klok, epoch and cursor: the replica's step counter, Philox epoch and stream cursor
+    TIS_DEV HbDraws(const Dev &D, int r, long long klok, unsigned epoch, long long cursor) {
         rng = D.rng;
-        c0 = (uint32_t)S.klok;
-        c1 = (uint32_t)(S.klok >> 32);
+        c0 = (uint32_t)klok;
+        c1 = (uint32_t)(klok >> 32);
         k0 = philox_key0(D, r);
-        k1 = philox_key1(D, S);
+        k1 = (uint32_t)(D.seed >> 32) + epoch;
         stream = rng ? D.stream + D.stream_base[r] : nullptr;
-        cur = S.stream_cursor;
+        cur = cursor;
         lim = rng ? D.stream_len[r] : 0;
     }
     // draws available (stream mode); draws at or past it read 0.5 and the caller
@@ -1028,18 +1097,17 @@ __global__ void __launch_bounds__(kBlock) k_hbsum(Dev D) {
 // gate the six bead forces of the candidate, to be added by k_hbres if it keeps
 // it.  Computed in precision R once per candidate, for every resolution group.
 // Grid (kActMax / kBlock, replicas); outputs at r * kActMax + live index.
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
-    const int r = blockIdx.y;
-    const RepState &S = D.rs[r];
+// Candidate a (live index) of replica r; n_act and tab: the replica's current
+// table counts and buffer.
+template <class R, class L>
+TIS_DEV void hbenergy_one(const Dev &D, int r, int a, int n_act, int tab) {
     const RepParams &P = D.rp[r];
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n_act = (S.overflow & 2) ? 0 : S.n_act;
     if (a >= n_act) return;
-    const HbTab T = hb_tab(D, S.tab);
-    const int4 ra = T.act_rec[2 * (P.hb0 + a)], rb = T.act_rec[2 * (P.hb0 + a) + 1];
+    const HbTab T = hb_tab(D, tab);
+    const int4 ra = L::ld(T.act_rec + 2 * (P.hb0 + a)), rb = L::ld(T.act_rec + 2 * (P.hb0 + a) + 1);
     const int g = ra.x;
-    const double4 pt[6] = {D.pos[ra.y], D.pos[ra.z], D.pos[ra.w], D.pos[rb.x], D.pos[rb.y], D.pos[rb.z]};
+    const double4 pt[6] = {L::ld(D.pos + ra.y), L::ld(D.pos + ra.z), L::ld(D.pos + ra.w),
+                           L::ld(D.pos + rb.x), L::ld(D.pos + rb.y), L::ld(D.pos + rb.z)};
     const double *geo = D.hb_g + 6 * g;
     const double hE = D.hb_E[g];
     const size_t o = (size_t)kActMax * r + a;
@@ -1064,6 +1132,13 @@ __global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
     }
     D.hb_en[o] = e;
     D.hb_gate[o] = f;
+}
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
+    const int r = blockIdx.y;
+    const RepState &S = D.rs[r];
+    const int n_act = (S.overflow & 2) ? 0 : S.n_act;
+    hbenergy_one<R, LdPlain>(D, r, blockIdx.x * blockDim.x + threadIdx.x, n_act, S.tab);
 }
 
 // Energies, gates and valence resolution of the live HB candidates; writes the
@@ -1094,12 +1169,12 @@ __global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
 // component: group 0).  Each
 // component draws from its own Philox stream, so the outcome does not depend on
 // the number of groups.  The reference-stream process runs in group 0.
-template <class R>
-__global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
-    const int r = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x;
+// Group grp of ngrp of replica r's resolution, a workgroup of kResBlock
+// threads with kHbLds bytes of LDS at lds.
+template <class R, class L>
+TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsigned char *lds) {
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
-    extern __shared__ unsigned char lds[];
     double *E = (double *)lds;
     double *uwin = E + kActMax;
     int *par = (int *)(uwin + kResWaves * kCompWin);
@@ -1117,21 +1192,21 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
     __shared__ int sh_err, sh_next;
-    const int n_act = S.n_act, n_as = S.n_asite, n_ainc = S.n_ainc;
-    const HbTab T = hb_tab(D, S.tab);
+    const int n_act = L::ld(&S.n_act), n_as = L::ld(&S.n_asite), n_ainc = L::ld(&S.n_ainc);
+    const HbTab T = hb_tab(D, L::ld(&S.tab));
     if (threadIdx.x == 0) {   // this group's parts (summed by k_hbsum or k_commit)
         D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
         D.hb_e_part[2 * (kMaxHbGroups * r + grp)] = 0.0;
         D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = 0.0;
     }
-    if (S.overflow & 2) return;
+    if (L::ld(&S.overflow) & 2) return;
     if (D.rng == 1 && grp > 0) return;
     if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     // ---- the rebuild tables with the active sites' valences, and the candidate
     // energies and gates of k_hbenergy: one level of loads, all issued before any
     // is used.
-    const int n_hinc = T.ahb_inc_off[P.hb0 + r + n_act];
+    const int n_hinc = L::ld(T.ahb_inc_off + P.hb0 + r + n_act);
     if (n_hinc > kAIncMax) {                      // inconsistent rebuild tables
         if (tid == 0) atomicOr(&S.overflow, 2);
         return;
@@ -1145,8 +1220,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
-        t_aoff[k] = i <= n_as ? T.asite_off[P.site0 + r + i] : 0;
-        t_val[k] = i < n_as ? T.asite_val[P.site0 + i] : 0;
+        t_aoff[k] = i <= n_as ? L::ld(T.asite_off + P.site0 + r + i) : 0;
+        t_val[k] = i < n_as ? L::ld(T.asite_val + P.site0 + i) : 0;
     }
     // energies and gates of the live candidates (k_hbenergy)
     const double *hb_en = D.hb_en + (size_t)kActMax * r;
@@ -1154,22 +1229,22 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
         const int i = tid + k * kResBlock;
-        t_hoff[k] = i <= n_act ? T.ahb_inc_off[P.hb0 + r + i] : 0;
-        t_e[k] = i < n_act ? hb_en[i] : 0.0;
-        t_f[k] = i < n_act ? hb_gate[i] : (unsigned char)FLAG_GATED;
+        t_hoff[k] = i <= n_act ? L::ld(T.ahb_inc_off + P.hb0 + r + i) : 0;
+        t_e[k] = i < n_act ? L::ld(hb_en + i) : 0.0;
+        t_f[k] = i < n_act ? L::ld(hb_gate + i) : (unsigned char)FLAG_GATED;
     }
 #pragma unroll
     for (int k = 0; k < KI; ++k) {
         const int i = tid + k * kResBlock;
-        t_alist[k] = i < n_ainc ? T.asite_list[P.inc0 + i] : 0;
-        t_hinc[k] = i < n_hinc ? T.ahb_inc[P.inc0 + i] : 0;
+        t_alist[k] = i < n_ainc ? L::ld(T.asite_list + P.inc0 + i) : 0;
+        t_hinc[k] = i < n_hinc ? L::ld(T.ahb_inc + P.inc0 + i) : 0;
     }
     // the thread's first candidate's record with the tables (later ones, rare
     // below 1024 live candidates, load in turn when their forces are added)
     int4 ra0 = make_int4(0, 0, 0, 0), rb0 = ra0;
     if (tid < n_act) {
-        ra0 = T.act_rec[2 * (P.hb0 + tid)];
-        rb0 = T.act_rec[2 * (P.hb0 + tid) + 1];
+        ra0 = L::ld(T.act_rec + 2 * (P.hb0 + tid));
+        rb0 = L::ld(T.act_rec + 2 * (P.hb0 + tid) + 1);
     }
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
@@ -1233,7 +1308,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         }
     }
     __syncthreads();
-    const HbDraws dr(D, r, S);
+    const HbDraws dr(D, r, L::ld(&S.klok), L::ld(&S.rng_epoch), L::ld(&S.stream_cursor));
     if (D.rng == 1) {
         // incidences as over-subscribed indices (-1: a site that is not)
         for (int q = tid; q < n_hinc; q += nt) hinc[q] = A1[hinc[q]];
@@ -1363,8 +1438,8 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         if (owner != grp) continue;
         int4 ra = ra0, rb = rb0;
         if (a != tid) {
-            ra = T.act_rec[2 * (P.hb0 + a)];
-            rb = T.act_rec[2 * (P.hb0 + a) + 1];
+            ra = L::ld(T.act_rec + 2 * (P.hb0 + a));
+            rb = L::ld(T.act_rec + 2 * (P.hb0 + a) + 1);
         }
         const int g = ra.x;
         D.hb_out[g] = (signed char)flag[a];
@@ -1374,7 +1449,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
             const R *fr = (const R *)D.hb_frc + 18 * ((size_t)kActMax * r + a);
             V3<R> fb[6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) fb[q] = V3<R>{fr[3 * q], fr[3 * q + 1], fr[3 * q + 2]};
+            for (int q = 0; q < 6; ++q) fb[q] = V3<R>{L::ld(fr + 3 * q), L::ld(fr + 3 * q + 1), L::ld(fr + 3 * q + 2)};
             const double e = E[a];
             if (D.hb_dode[g]) e3 += e; else e2 += e;
 #pragma unroll
@@ -1393,10 +1468,10 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
     // first resolution on a table switched by k_commit: candidates outside it
     // (inactive in the rebuild that made it) report -1; the listed ones were
     // written above
-    if (S.tab_fresh) {
+    if (L::ld(&S.tab_fresh)) {
         const int f0 = D.rng == 1 ? 0 : grp, fs = D.rng == 1 ? 1 : ngrp;
         for (int k = f0 * nt + tid; k < P.nhb; k += fs * nt)
-            if (D.hb_status[P.hb0 + k] < 0) D.hb_out[P.hb0 + k] = -1;
+            if (L::ld(D.hb_status + P.hb0 + k) < 0) D.hb_out[P.hb0 + k] = -1;
     }
     if (tid == 0) {
         D.hb_iter_part[kMaxHbGroups * r + grp] = sh_iters;
@@ -1411,7 +1486,7 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         __shared__ int sh_flags;
         if (threadIdx.x == 0) sh_flags = 0;
         __syncthreads();
-        const long long c0 = S.stream_cursor + sh_draws;
+        const long long c0 = L::ld(&S.stream_cursor) + sh_draws;
         const long long lim = D.stream_len[r];
         const int32_t *st = D.stream + D.stream_base[r];
         double *mw = D.maxwell + 3000 * (size_t)r;
@@ -1453,17 +1528,22 @@ __global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
         S.stream_cursor += sh_draws;   // evaluation consumes the HB draws only
     }
 }
+template <class R>
+__global__ void __launch_bounds__(kResBlock) k_hbres(Dev D, int mode) {
+    extern __shared__ unsigned char lds[];
+    hbres_group<R, LdPlain>(D, mode, blockIdx.y, blockIdx.x, gridDim.x, lds);
+}
 
 // Tertiary stack g (stacking_interactions, fast.cpp:1911-2141): E / (1 + Q6)
 // with Q6 the weighted squared deviations of one distance, two angles and three
 // dihedrals; returns the energy and adds the forces (see sextet_forces for vdotf).
-template <class R, class FT>
+template <class R, class L = LdPlain, class FT>
 TIS_DEV R tertiary_stack(const Dev &D, FT F, int g, double *vdotf) {
     using Pt = typename PtOf<R>::type;
     const int *b = D.st3_b + 6 * g;
     const double *geo = D.st3_g + 6 * g;
-    const Pt p0 = PtLoad<R>::at(D, b[0]), p1 = PtLoad<R>::at(D, b[1]), p2 = PtLoad<R>::at(D, b[2]);
-    const Pt p3 = PtLoad<R>::at(D, b[3]), p4 = PtLoad<R>::at(D, b[4]), p5 = PtLoad<R>::at(D, b[5]);
+    const Pt p0 = PtLoad<R, L>::at(D, b[0]), p1 = PtLoad<R, L>::at(D, b[1]), p2 = PtLoad<R, L>::at(D, b[2]);
+    const Pt p3 = PtLoad<R, L>::at(D, b[3]), p4 = PtLoad<R, L>::at(D, b[4]), p5 = PtLoad<R, L>::at(D, b[5]);
     V3<R> v14 = diff<R>(p1, p4);
     R dr = sqrt(dot(v14, v14)) - (R)geo[0];
     R c1, c2;
@@ -1490,14 +1570,14 @@ TIS_DEV R tertiary_stack(const Dev &D, FT F, int g, double *vdotf) {
 // Secondary stack g (stacking_interactions, fast.cpp:2060-2141): E / (1 + Q3)
 // over the B-B distance and two backbone dihedrals; returns the energy and adds
 // the forces (vdotf as in sextet_forces).
-template <class R, class FT>
+template <class R, class L = LdPlain, class FT>
 TIS_DEV R secondary_stack(const Dev &D, FT F, int g, double *vdotf) {
     using Pt = typename PtOf<R>::type;
     const int *b = D.st2_b + 8 * g;   // P_n S_n B_n P_n+1 S_n+1 B_n+1 P_n+2
     const double2 rE = D.st2_rE[g];
-    const Pt q0 = PtLoad<R>::at(D, b[0]), q1 = PtLoad<R>::at(D, b[1]), q2 = PtLoad<R>::at(D, b[2]);
-    const Pt q3 = PtLoad<R>::at(D, b[3]), q4 = PtLoad<R>::at(D, b[4]), q5 = PtLoad<R>::at(D, b[5]);
-    const Pt q6 = PtLoad<R>::at(D, b[6]);
+    const Pt q0 = PtLoad<R, L>::at(D, b[0]), q1 = PtLoad<R, L>::at(D, b[1]), q2 = PtLoad<R, L>::at(D, b[2]);
+    const Pt q3 = PtLoad<R, L>::at(D, b[3]), q4 = PtLoad<R, L>::at(D, b[4]), q5 = PtLoad<R, L>::at(D, b[5]);
+    const Pt q6 = PtLoad<R, L>::at(D, b[6]);
     V3<R> v25 = diff<R>(q2, q5);
     R rr = sqrt(dot(v25, v25));
     V3<R> g1[4], g2[4];
@@ -1532,13 +1612,13 @@ TIS_DEV R secondary_stack(const Dev &D, FT F, int g, double *vdotf) {
 // fast.cpp:2354-2520: bead_bond_force k (r - r0)^2, bead_valence_force
 // k (theta - theta0)^2): adds the forces, returns the energy in *eb or *ea and the
 // end beads (i, k) whose WCA the reference removes; false past the list's end.
-template <class R, class FT>
+template <class R, class L = LdPlain, class FT>
 TIS_DEV bool bonded_term(const Dev &D, FT F, int g, int nbond, int nang, int *i_out, int *k_out, double *eb,
                          double *ea) {
     if (g < nbond) {
         const int2 ij = D.bond_ij[g];
         const double2 kr = D.bond_kr[g];
-        V3<R> d = diff<R>(PtLoad<R>::at(D, ij.x), PtLoad<R>::at(D, ij.y));
+        V3<R> d = diff<R>(PtLoad<R, L>::at(D, ij.x), PtLoad<R, L>::at(D, ij.y));
         R rr;
         R t = bond_coef<R>(d, (R)kr.x, (R)kr.y, &rr);
         atomic_add_force(F, ij.x, d * (-t));
@@ -1552,8 +1632,8 @@ TIS_DEV bool bonded_term(const Dev &D, FT F, int g, int nbond, int nang, int *i_
         const int4 t3 = D.ang_ijk[g - nbond];
         const double2 kt = D.ang_kt[g - nbond];
         const int i = t3.x, j = t3.y, k = t3.z;
-        const auto pj = PtLoad<R>::at(D, j);
-        V3<R> ba = diff<R>(pj, PtLoad<R>::at(D, i)), bc = diff<R>(pj, PtLoad<R>::at(D, k));
+        const auto pj = PtLoad<R, L>::at(D, j);
+        V3<R> ba = diff<R>(pj, PtLoad<R, L>::at(D, i)), bc = diff<R>(pj, PtLoad<R, L>::at(D, k));
         R c;
         V3<R> g0, g2;
         angle_grad<R>(ba, bc, &c, &g0, &g2);
@@ -1585,6 +1665,8 @@ constexpr int kMaxStSlices = 64;  // stack workgroups per replica in a step (D.s
 // kEvalEnergy (as kEvalFull without writing pair forces), kEvalForce (the step's
 // forces with a fresh LR term, no energies).
 enum { kEvalStep = 0, kEvalFull = 1, kEvalEnergy = 2, kEvalForce = 3 };
+template <class R, class L>
+TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, unsigned mask, int eval);
 template <class R, int ROLE>
 __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned mask, int eval) {
     double4 *F = D.f_mb;
@@ -1683,11 +1765,20 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     // butterfly reduction inside the group
     const int lanes = D.pair_lanes;
     const int t = (blk * blockDim.x + threadIdx.x) / lanes;
-    const int sub = threadIdx.x % lanes;
-    const bool valid = t < D.n_tot;
+    pair_gather<R, LdPlain>(D, t, threadIdx.x % lanes, lanes, t < D.n_tot, mask, eval);
+}
+
+// Pair forces of the particle at cell-order index t (valid: t names a particle;
+// every lane of the wavefront takes part, as the lanes of a particle's group
+// exchange partial sums), lane sub of its group of `lanes`.
+template <class R, class L>
+TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, unsigned mask, int eval) {
+    const bool want_e = eval == kEvalFull || eval == kEvalEnergy;
+    const bool split = want_e;
+    const bool write_f = eval != kEvalEnergy;
     // own and neighbour positions from the cell-ordered copy (kept current by
-    // k_commit): neighbours in the same few cells share cache lines
-    const double4 xi = D.cpos[valid ? t : 0];
+    // the integrator): neighbours in the same few cells share cache lines
+    const double4 xi = L::ld(D.cpos + (valid ? t : 0));
     const int p = cp_index(xi.w);
     const int r = D.prep[p];
     const RepParams &P = D.rp[r];
@@ -1695,8 +1786,8 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     const int pti = cp_type(xi.w);
     const int ti = pti & 15;
     const bool chg_i = (pti >> 8) & 1;
-    const int *nb = D.pool + P.pool0 + D.nb_start[p];
-    const int nsr = valid ? D.nb_nsr[p] : 0, nlr = valid ? D.nb_nlr[p] : 0;
+    const int *nb = D.pool + P.pool0 + L::ld(D.nb_start + p);
+    const int nsr = valid ? L::ld(D.nb_nsr + p) : 0, nlr = valid ? L::ld(D.nb_nlr + p) : 0;
     auto group_sum = [lanes](double v) {
         for (int o = 1; o < lanes; o <<= 1) v += __shfl_xor(v, o, 64);
         return v;
@@ -1705,9 +1796,9 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
         R fx = 0, fy = 0, fz = 0;
         double ew = 0, ec = 0;
         for (int q = sub; q < nsr; q += lanes) {
-            const int e = nb[q];
+            const int e = L::ld(nb + q);
             const bool bonded = e < 0;
-            const double4 xj = D.cpos[bonded ? ~e : e];
+            const double4 xj = L::ld(D.cpos + (bonded ? ~e : e));
             double3 dd;
             const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
             const int ptj = cp_type(xj.w);
@@ -1740,14 +1831,14 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     }
     // the LR refresh is decided per replica; a group never spans two particles,
     // but a wavefront may span two replicas, so the test stays per lane
-    const bool lr = (mask & 16) && (eval != kEvalStep || S.lr_due);
+    const bool lr = (mask & 16) && (eval != kEvalStep || L::ld(&S.lr_due));
     if (__ballot(lr)) {
         R fx = 0, fy = 0, fz = 0;
         double ec = 0;
         const int n_lr = lr ? nlr : 0;
         const int *nb_lr = nb + P.cap_sr;
         for (int q = sub; q < n_lr; q += lanes) {
-            const double4 xj = D.cpos[nb_lr[q]];
+            const double4 xj = L::ld(D.cpos + L::ld(nb_lr + q));
             double3 dd;
             const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
             if (r2 > kCoulCut2) continue;
@@ -1782,13 +1873,11 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
 // stacks into st_energy, each slice's E2 / E3 stack sums into st_part (summed
 // by k_commit).
 constexpr int kMbBlock = 256;
-template <class R>
-__global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
-    const int sl = blockIdx.x, r = blockIdx.y, S = gridDim.x;
+// Slice sl of S of replica r, one workgroup (any multiple of 64 threads up to
+// 1024); LDS: lf (3 doubles per bead of the window), off (4096 bytes), red (16 doubles).
+template <class R, class L>
+TIS_DEV void multibody_slice(const Dev &D, int r, int sl, int S, double *lf, unsigned char *off, double *red) {
     const RepParams &P = D.rp[r];
-    extern __shared__ double lf[];
-    __shared__ unsigned char off[4096];
-    __shared__ double red[kMbBlock / 64];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int w_lo = D.mb_win[2 * (r * S + sl)], w_n = D.mb_win[2 * (r * S + sl) + 1] - w_lo;
     for (int k = tid; k < 3 * w_n; k += nt) lf[k] = 0.0;
@@ -1798,7 +1887,7 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
     for (int k = tid; k < P.nst3; k += nt) {
         const int g = P.st30 + k;
         const int *b = D.st3_b + 6 * g;
-        V3<R> v = diff<R>(D.pos[b[1]], D.pos[b[4]]);
+        V3<R> v = diff<R>(L::ld(D.pos + b[1]), L::ld(D.pos + b[4]));
         R dr = sqrt(dot(v, v)) - (R)D.st3_g[6 * g];
         if (dr < (R)10) {
             const int2 dd = D.st3_dis[g];
@@ -1812,13 +1901,13 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
     double e2 = 0.0, e3 = 0.0;
     for (int q = o0[0] + tid; q < o1[0]; q += nt) {
         const int g = D.mb_terms[q];
-        const double e = (double)tertiary_stack<R>(D, D.f_mb, g, nullptr);
+        const double e = (double)tertiary_stack<R, L>(D, D.f_mb, g, nullptr);
         D.st_energy[g] = e;
         e3 += e;
     }
     for (int q = o0[1] + tid; q < o1[1]; q += nt) {
         const int g = D.mb_terms[q];
-        if (!off[g - P.st20]) e2 += (double)secondary_stack<R>(D, F, g, nullptr);
+        if (!off[g - P.st20]) e2 += (double)secondary_stack<R, L>(D, F, g, nullptr);
     }
     const int nbond = D.rp[D.n_rep - 1].bond0 + D.rp[D.n_rep - 1].nbond;
     const int nang = D.rp[D.n_rep - 1].ang0 + D.rp[D.n_rep - 1].nang;
@@ -1827,12 +1916,12 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
         const int g = D.mb_terms[t < nbs ? o0[2] + t : o0[3] + t - nbs];
         int i, k;
         double eb, ea;
-        bonded_term<R>(D, F, g, nbond, nang, &i, &k, &eb, &ea);
+        bonded_term<R, L>(D, F, g, nbond, nang, &i, &k, &eb, &ea);
         if (!D.bsub[g]) continue;
         // a pair listed by a second bonded term: its WCA removed once more (the
         // step leaves bonded / 1-3 pairs out of the pair sum; k_force explains)
         double3 dd;
-        const double r2 = min_image_r2(D.pos[i], D.pos[k], P.L, P.hL, &dd);
+        const double r2 = min_image_r2(L::ld(D.pos + i), L::ld(D.pos + k), P.L, P.hL, &dd);
         const int ab = (D.ptype[i] & 15) * kNT + (D.ptype[k] & 15);
         const R f = -wca_factor<R>(r2, ab, P);
         V3<R> v = {(R)dd.x * f, (R)dd.y * f, (R)dd.z * f};
@@ -1856,13 +1945,21 @@ __global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
         D.st_part[2 * (kMaxStSlices * r + sl) + 1] = s3;
     }
 }
+template <class R>
+__global__ void __launch_bounds__(kMbBlock) k_multibody(Dev D) {
+    extern __shared__ double lf[];
+    __shared__ unsigned char off[4096];
+    __shared__ double red[kMbBlock / 64];
+    multibody_slice<R, LdPlain>(D, blockIdx.y, blockIdx.x, gridDim.x, lf, off, red);
+}
 
 // ------------------------------------------------------------ k_integrate
-TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double out[3]) {
+// klok, epoch: the replica's step counter and Philox epoch.
+TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, long long klok, unsigned epoch, int p, double out[3]) {
     const int local = p - P.p0;
     if (D.rng == 0) {
-        Philox4 o = philox4x32_10((uint32_t)S.klok, (uint32_t)(S.klok >> 32), (uint32_t)local, kTagNoise,
-                                  philox_key0(D, r), philox_key1(D, S));
+        Philox4 o = philox4x32_10((uint32_t)klok, (uint32_t)(klok >> 32), (uint32_t)local, kTagNoise,
+                                  philox_key0(D, r), (uint32_t)(D.seed >> 32) + epoch);
         const double a = sqrt(-2.0 * log(u01_open_closed(o.v[0]))), b = 2.0 * kPi * u01_closed_open(o.v[1]);
         const double c = sqrt(-2.0 * log(u01_open_closed(o.v[2]))), d = 2.0 * kPi * u01_closed_open(o.v[3]);
         out[0] = a * cos(b);
@@ -1883,42 +1980,49 @@ TIS_DEV void noise3(const Dev &D, int r, const RepParams &P, const RepState &S, 
 // the particle and the key, so k_noise computes it beside the step's force
 // kernels into D.nz; the reference stream's table is made by the HB resolution
 // of the same step, so that mode forms it in the integrator.
+template <class L>
 TIS_DEV double3 random_force(const Dev &D, int r, const RepParams &P, const RepState &S, int p, int t) {
     if (D.rng == 0) {
-        const double4 z = D.nz[p];
+        const double4 z = L::ld(D.nz + p);
         return make_double3(z.x, z.y, z.z);
     }
     double nz[3];
-    noise3(D, r, P, S, p, nz);
+    noise3(D, r, P, L::ld(&S.klok), L::ld(&S.rng_epoch), p, nz);
     const double sg = P.sigma[t] * D.noise_scale;
     return make_double3(nz[0] * sg, nz[1] * sg, nz[2] * sg);
 }
 
 // k_noise (Philox draws only): D.nz[p] = sigma_type * noise_scale * xi for the
 // step about to run, one thread per particle.
+// klok, epoch: replica r's step counter and Philox epoch.
+TIS_DEV void noise_one(const Dev &D, int r, int p, long long klok, unsigned epoch) {
+    const int t = D.ptype[p] & 15;
+    const RepParams &P = D.rp[r];
+    double nz[3];
+    noise3(D, r, P, klok, epoch, p, nz);
+    const double sg = P.sigma[t] * D.noise_scale;
+    D.nz[p] = make_double4(nz[0] * sg, nz[1] * sg, nz[2] * sg, 0.0);
+}
 __global__ void __launch_bounds__(kBlock) k_noise(Dev D) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.n_tot) return;
-    const int r = D.prep[p], t = D.ptype[p] & 15;
-    const RepParams &P = D.rp[r];
-    double nz[3];
-    noise3(D, r, P, D.rs[r], p, nz);
-    const double sg = P.sigma[t] * D.noise_scale;
-    D.nz[p] = make_double4(nz[0] * sg, nz[1] * sg, nz[2] * sg, 0.0);
+    const int r = D.prep[p];
+    noise_one(D, r, p, D.rs[r].klok, D.rs[r].rng_epoch);
 }
 
 // leap_frog_move_atom (fast.cpp:570-642) for particle p; returns the new position
 // and velocity (position before recentring).  Only RNA beads carry multi-body
 // forces (ions' f_mb entries stay zero and are not read).
+template <class L>
 TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S, int p, double xn[3],
                       double vn[3], double xs[3]) {
     const int t = D.ptype[p] & 15;
     const bool bead = p - P.p0 < P.nbeads;
-    const double4 f0 = D.f_sr[p], f1 = D.f_lr[p];
-    const double4 f2 = bead ? D.f_mb[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double3 nz = random_force(D, r, P, S, p, t);
+    const double4 f0 = L::ld(D.f_sr + p), f1 = L::ld(D.f_lr + p);
+    const double4 f2 = bead ? L::ld(D.f_mb + p) : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double3 nz = random_force<L>(D, r, P, S, p, t);
     const double f[3] = {f0.x + f1.x + f2.x + nz.x, f0.y + f1.y + f2.y + nz.y, f0.z + f1.z + f2.z + nz.z};
-    const double4 x = D.pos[p], xo = D.old[p], v = D.vel[p];
+    const double4 x = L::ld(D.pos + p), xo = L::ld(D.old + p), v = L::ld(D.vel + p);
     const double xv[3] = {x.x, x.y, x.z}, xov[3] = {xo.x, xo.y, xo.z}, vv[3] = {v.x, v.y, v.z};
     for (int c = 0; c < 3; ++c) vn[c] = P.K1[t] * vv[c] + P.K2[t] * f[c];
     double vsq = P.mass[t] * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]) / P.vcap;
@@ -1937,17 +2041,19 @@ TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S
 // reach another's read, the recentring bead's included); the cell-ordered copy
 // the pair gather reads is refreshed (its .w, identity and type, stays) and the
 // multi-body force buffer of the next step cleared.
-__global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Particle p (valid: p names a particle; every lane of the wavefront takes part
+// in the displacement maximum, which goes to word `slot` of its replica).
+template <class L>
+TIS_DEV void integrate_one(const Dev &D, int p, bool valid, int slot) {
     double d2 = 0.0;   // squared displacement of this thread's particle
     int rp = -1;       // its replica
-    if (p < D.n_tot) {
+    if (valid) {
         const int r = D.prep[p];
         const RepParams &P = D.rp[r];
         RepState &S = D.rs[r];
         double xn[3], vn[3], xs[3], kx[3], kv[3], ks[3];
-        leapfrog(D, r, P, S, p, xn, vn, xs);
-        leapfrog(D, r, P, S, P.kh, kx, kv, ks);   // the recentring bead (adjust_box, fast.cpp:2632-2656)
+        leapfrog<L>(D, r, P, S, p, xn, vn, xs);
+        leapfrog<L>(D, r, P, S, P.kh, kx, kv, ks);   // the recentring bead (adjust_box, fast.cpp:2632-2656)
         double on[3];
         for (int c = 0; c < 3; ++c) {
             const double s = P.hL[c] - kx[c];
@@ -1967,7 +2073,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         D.old_nx[p] = make_double4(on[0], on[1], on[2], 0.0);
         D.vel_nx[p] = make_double4(vn[0], vn[1], vn[2], 0.0);
         if (p - P.p0 < P.nbeads) D.f_mb_nx[p] = make_double4(0.0, 0.0, 0.0, 0.0);
-        double4 *c = D.cpos + D.crank[p];
+        double4 *c = D.cpos + L::ld(D.crank + p);
         c->x = xn[0];
         c->y = xn[1];
         c->z = xn[2];
@@ -1978,7 +2084,6 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
     // replica, each on a cache line of its own (by wavefront index, so no one
     // word or line takes every wavefront's atomic)
     const int r0 = __builtin_amdgcn_readfirstlane(rp);
-    const int slot = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kMaxSlots;
     if (__ballot(rp == r0) == ~0ull) {
         double m = d2;
         for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -1988,6 +2093,10 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
         atomicMax(&D.maxd2[((size_t)rp * kMaxSlots + slot) * kSlotStride], (unsigned long long)__double_as_longlong(d2));
     }
 }
+__global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    integrate_one<LdPlain>(D, p, p < D.n_tot, (int)(p >> 6) % kMaxSlots);
+}
 
 // Closes the step of every replica (a wavefront per replica): the list-rebuild
 // decision from the largest displacement k_integrate recorded (check_shifts,
@@ -1995,9 +2104,10 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
 // the step counter and the LR refresh flag.  The per-slice, per-slot and
 // per-group parts are read by the lanes at once and reduced across the
 // wavefront (their sums in the order of the parts, as one thread would add them).
-__global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
-    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (r >= D.n_rep) return;
+// Replica r, one whole wavefront.
+template <class L>
+TIS_DEV void commit_replica(const Dev &D, int r) {
+    const int lane = threadIdx.x & 63;
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
     const int ns = D.st_slices > 1 ? D.st_slices : 0;
@@ -2005,18 +2115,18 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
     unsigned long long mb = 0;
     int it = 0;
     if (lane < ns) {
-        s2 = D.st_part[2 * (kMaxStSlices * r + lane)];
-        s3 = D.st_part[2 * (kMaxStSlices * r + lane) + 1];
+        s2 = L::ld(D.st_part + 2 * (kMaxStSlices * r + lane));
+        s3 = L::ld(D.st_part + 2 * (kMaxStSlices * r + lane) + 1);
     }
     if (lane < kMaxSlots) {
         unsigned long long *w = D.maxd2 + ((size_t)r * kMaxSlots + lane) * kSlotStride;
-        mb = *w;
+        mb = L::ld(w);
         *w = 0;
     }
     if (lane < D.hb_groups) {
-        h2 = D.hb_e_part[2 * (kMaxHbGroups * r + lane)];
-        h3 = D.hb_e_part[2 * (kMaxHbGroups * r + lane) + 1];
-        it = D.hb_iter_part[kMaxHbGroups * r + lane];
+        h2 = L::ld(D.hb_e_part + 2 * (kMaxHbGroups * r + lane));
+        h3 = L::ld(D.hb_e_part + 2 * (kMaxHbGroups * r + lane) + 1);
+        it = L::ld(D.hb_iter_part + kMaxHbGroups * r + lane);
     }
     // lane 0 adds the parts in order (a readlane each; the sums match a serial loop)
     double t2 = 0.0, t3 = 0.0, u2 = 0.0, u3 = 0.0;
@@ -2038,21 +2148,28 @@ __global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
     const double m = sqrt(__longlong_as_double((long long)mb));
     // a NaN or infinite coordinate (its bits order above every finite
     // displacement) stops the run with an error instead of propagating
-    if (!(m < 1e30)) S.overflow |= 32;
-    S.progress += m + m;
-    S.need_rebuild = S.progress > P.skin;
-    if (S.need_rebuild) { S.progress = 0; S.n_rebuilds += 1; }
+    if (!(m < 1e30)) atomicOr(&S.overflow, 32);
+    double prog = L::ld(&S.progress) + (m + m);
+    const int need = prog > P.skin;
+    if (need) { prog = 0; S.n_rebuilds = L::ld(&S.n_rebuilds) + 1; }
+    S.progress = prog;
+    S.need_rebuild = need;
     S.tab_fresh = 0;
-    if (S.tab_pending) {   // HB tables built beside this step's resolution
-        S.tab ^= 1;
-        S.n_act = S.nx_act;
-        S.n_asite = S.nx_asite;
-        S.n_ainc = S.nx_ainc;
+    if (L::ld(&S.tab_pending)) {   // HB tables built beside this step's resolution
+        S.tab = L::ld(&S.tab) ^ 1;
+        S.n_act = L::ld(&S.nx_act);
+        S.n_asite = L::ld(&S.nx_asite);
+        S.n_ainc = L::ld(&S.nx_ainc);
         S.tab_pending = 0;
         S.tab_fresh = 1;
     }
-    S.klok += 1;
-    S.lr_due = (S.klok % P.lr_every) == 1 || S.progress == 0;
+    const long long klok = L::ld(&S.klok) + 1;
+    S.klok = klok;
+    S.lr_due = (klok % P.lr_every) == 1 || prog == 0;
+}
+__global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (r < D.n_rep) commit_replica<LdPlain>(D, r);
 }
 
 // ------------------------------------------------------------ replica exchange

@@ -63,7 +63,7 @@ class Options(C.Structure):
                 ("r_lr", C.c_double), ("lr_every", C.c_int), ("use_graph", C.c_int),
                 ("graph_steps", C.c_int), ("exchange_every", C.c_longlong), ("pair_lanes", C.c_int),
                 ("pair_block", C.c_int), ("list_waves", C.c_int), ("hb_groups", C.c_int),
-                ("hb_table_lag", C.c_int)]
+                ("hb_table_lag", C.c_int), ("persistent", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -167,6 +167,9 @@ def load_library(path=LIB_PATH):
         lib.tis_get_launch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                               C.POINTER(C.c_longlong)]
         lib.tis_step.argtypes = [C.c_void_p, C.c_longlong]
+        lib.tis_set_persistent.argtypes = [C.c_void_p, C.c_int]
+        lib.tis_get_persistent_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                                  C.POINTER(C.c_longlong), C.c_char_p, C.c_int]
         lib.tis_get_energies.argtypes = [C.c_void_p, C.c_int, C.POINTER(Energies)]
         lib.tis_get_stack_energies.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
         lib.tis_eval_forces.argtypes = [C.c_void_p, C.c_int, C.c_uint, _dp, C.POINTER(Energies)]
@@ -383,6 +386,16 @@ class Engine:
         v = [C.c_longlong(0) for _ in range(3)]
         check(self.lib, self.lib.tis_get_launch_counts(self.ctx, *[C.byref(x) for x in v]), "tis_get_launch_counts")
         return {"graph_launches": v[0].value, "graph_steps": v[1].value, "eager_steps": v[2].value}
+
+    def set_persistent(self, on):
+        check(self.lib, self.lib.tis_set_persistent(self.ctx, 1 if on else 0), "tis_set_persistent")
+
+    def persistent_counts(self):
+        act, nl, ns = C.c_int(0), C.c_longlong(0), C.c_longlong(0)
+        why = C.create_string_buffer(256)
+        check(self.lib, self.lib.tis_get_persistent_counts(self.ctx, C.byref(act), C.byref(nl), C.byref(ns), why, 256),
+              "tis_get_persistent_counts")
+        return {"active": bool(act.value), "launches": nl.value, "steps": ns.value, "why": why.value.decode()}
 
     def set_rng_epoch(self, epoch, rep=-1):
         check(self.lib, self.lib.tis_set_rng_epoch(self.ctx, rep, epoch), "tis_set_rng_epoch")
