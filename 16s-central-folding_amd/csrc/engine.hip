@@ -247,7 +247,7 @@ struct tis_ctx {
         timed("k_list", sl, [&] { k_list<<<list_grid(), kBlock, 0, sl>>>(D, 1); });
     }
     // lag: inside a step, beside the resolution, into the table buffer it is not
-    // reading (switched by k_commit)
+    // reading (switched by the step's close)
     void launch_rebuild_hb(hipStream_t sh, int lag) {
         timed("k_hbstatus", sh, [&] { k_hbstatus<<<dim3(max_hb_chunks(), n_rep), kStatusBlock, 0, sh>>>(D, lag); });
         timed("k_hbsites", sh, [&] { k_hbsites<<<n_rep, 1024, 2 * max_site_lds(), sh>>>(D, lag); });
@@ -277,7 +277,7 @@ struct tis_ctx {
 
     // HB terms on stream st: energies and gates, the valence resolution, forces.
     // The groups' energy sums and serial depths are reduced by k_hbsum, or in a
-    // step (mode bit 0) by k_commit.
+    // step (mode bit 0) by the step's close in k_integrate.
     void launch_hb(hipStream_t st, int mode) {
         launch_hb_energy(st);
         launch_hb_resolution(st, mode);
@@ -405,9 +405,8 @@ struct tis_ctx {
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
+        if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));   // the close reads the new tables' counts
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
-        if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));
-        timed("k_commit", s_main, [&] { k_commit<<<grid(64 * n_rep), kBlock, 0, s_main>>>(D); });
         set_parity(par ^ 1);   // the new state is in the other buffers
     }
 
@@ -759,6 +758,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.f_sr = dalloc<double4>(own, N);
         D.nz = dalloc<double4>(own, N);
         D.maxd2 = dalloc<unsigned long long>(own, (size_t)n_rep * kMaxSlots * kSlotStride);
+        D.close_cnt = dalloc<int>(own, (size_t)n_rep * kCloseStride);
         D.f_lr = dalloc<double4>(own, N);
         D.ptype = dalloc<int>(own, N);
         D.prep = dalloc<int>(own, N);

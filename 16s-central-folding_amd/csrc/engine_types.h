@@ -16,6 +16,7 @@ constexpr int kBlock = 256;
 constexpr int kExcl = 8;          // bonded and 1-3 partners per particle
 constexpr int kMaxSlots = 32;     // words of a replica's largest-displacement maximum
 constexpr int kSlotStride = 16;   // one 128-byte line per word (unsigned long long units)
+constexpr int kCloseStride = 32;  // one 128-byte line per replica's close counter (int units)
 
 // Static parameters of one replica (read-only on the device).
 struct RepParams {
@@ -54,7 +55,7 @@ struct RepState {
     int hb_iters;
     // HB rebuild tables: two buffers; `tab` is the one the resolution reads.  A
     // rebuild inside a step writes the other one (counts in nx_*) and sets
-    // tab_pending; k_commit then switches (tab_fresh for one step after).
+    // tab_pending; the step's close then switches (tab_fresh for one step after).
     int tab, tab_pending, tab_fresh;
     int nx_act, nx_asite, nx_ainc;
     int max_sr, max_lr;           // largest per-particle SR / LR counts of the last build
@@ -91,6 +92,7 @@ struct Dev {
     double4 *pos_nx, *old_nx, *vel_nx;
     double4 *f_sr, *f_lr, *f_mb, *f_mb_nx;   // f_mb: RNA beads only (ions' entries stay zero)
     double4 *nz;                  // random force of the step (Philox draws; k_noise)
+    int *close_cnt;               // k_integrate: workgroups done per replica (kCloseStride apart)
     unsigned long long *maxd2;    // bits of max |x - x_old|^2 of the current step: kMaxSlots words
                                   // per replica, word q of replica r at (r kMaxSlots + q) kSlotStride
     double4 *jv, *jf;             // force matching: position tangent in, force tangent out
@@ -98,7 +100,7 @@ struct Dev {
     int *prep;                    // replica of each particle
     // cells and lists
     int *cell_of, *cell_rank, *cell_start, *sorted;
-    double4 *cpos;                // positions in cell order (current: k_commit updates them),
+    double4 *cpos;                // positions in cell order (current: k_integrate updates them),
                                   // .w = particle index (low word) and type word (high word)
     int *crank;                   // cell-order index of each particle (last rebuild)
     int *excl;                    // kExcl bonded / 1-3 partners per particle (global, -1 none)

@@ -9,7 +9,7 @@
 //                 SR_/LR_interactions_lists :1783-1907)
 //   k_integrate   noise, leap-frog with speed cap, recentring, ion wrap, displacement
 //                 reduction (full_forces :2590-2625, move_rigid_units :2660-2732)
-//   k_commit      new state into place, rebuild decision (check_shifts :1741-1776)
+//   (k_integrate  closes each replica's step: rebuild decision, check_shifts :1741-1776)
 //   k_bin         LDS counting sort of particles into cells (separate_in_cells :1461-1542)
 //   k_list        full neighbour lists, SR and LR, from the 27 neighbouring cells
 //                 (populate_lists :1546-1700)
@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(kStatusBlock) k_hbstatus(Dev D, int lag) {
 // the static site list HB_ATOM (restored from the precomputed rank of every
 // incidence), and every live candidate's incidences as active-site indices.
 // Dynamic LDS: 2 * nsite ints.  lag: write the buffer the resolution is not
-// reading (it runs beside this kernel) and leave the switch to k_commit.
+// reading (it runs beside this kernel) and leave the switch to the step's close.
 // LDS of one workgroup of hbsites_replica: 2 nsite ints (site map, fill
 // counts), the chunk offsets, the site segment offsets and the segments.
 struct HbSitesLds {
@@ -1071,7 +1071,7 @@ TIS_DEV void uf_union(volatile int *par, int a, int b) {
 }
 
 // Sums of the resolution groups of each replica (one thread per replica): the
-// HB energies and the serial depth.  Steps do the same inside k_commit.
+// HB energies and the serial depth.  Steps do the same in their close.
 TIS_DEV void hb_reduce_parts(const Dev &D, int r) {
     RepState &S = D.rs[r];
     double e2 = 0.0, e3 = 0.0;
@@ -1194,7 +1194,7 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
     __shared__ int sh_err, sh_next;
     const int n_act = L::ld(&S.n_act), n_as = L::ld(&S.n_asite), n_ainc = L::ld(&S.n_ainc);
     const HbTab T = hb_tab(D, L::ld(&S.tab));
-    if (threadIdx.x == 0) {   // this group's parts (summed by k_hbsum or k_commit)
+    if (threadIdx.x == 0) {   // this group's parts (summed by k_hbsum or the step's close)
         D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
         D.hb_e_part[2 * (kMaxHbGroups * r + grp)] = 0.0;
         D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = 0.0;
@@ -1465,7 +1465,7 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
             D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = s3;
         }
     }
-    // first resolution on a table switched by k_commit: candidates outside it
+    // first resolution on a table switched by the step's close: candidates outside it
     // (inactive in the rebuild that made it) report -1; the listed ones were
     // written above
     if (L::ld(&S.tab_fresh)) {
@@ -1672,7 +1672,6 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     double4 *F = D.f_mb;
     const bool want_e = eval == kEvalFull || eval == kEvalEnergy;
     const bool split = want_e;
-    const bool write_f = eval != kEvalEnergy;
     // stack blocks: in a step each replica's stacks are shared out over
     // D.st_slices workgroups (one stack per thread), whose energy sums k_integrate
     // adds up; evaluations use one workgroup per replica
@@ -1871,7 +1870,7 @@ TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, un
 // join distant beads and add theirs to f_mb directly.  Dynamic LDS: 3 doubles
 // per bead of the largest window.  Energies as k_force in a step: tertiary
 // stacks into st_energy, each slice's E2 / E3 stack sums into st_part (summed
-// by k_commit).
+// by the step's close).
 constexpr int kMbBlock = 256;
 // Slice sl of S of replica r, one workgroup (any multiple of 64 threads up to
 // 1024); LDS: lf (3 doubles per bead of the window), off (4096 bytes), red (16 doubles).
@@ -2093,10 +2092,6 @@ TIS_DEV void integrate_one(const Dev &D, int p, bool valid, int slot) {
         atomicMax(&D.maxd2[((size_t)rp * kMaxSlots + slot) * kSlotStride], (unsigned long long)__double_as_longlong(d2));
     }
 }
-__global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    integrate_one<LdPlain>(D, p, p < D.n_tot, (int)(p >> 6) % kMaxSlots);
-}
 
 // Closes the step of every replica (a wavefront per replica): the list-rebuild
 // decision from the largest displacement k_integrate recorded (check_shifts,
@@ -2167,10 +2162,41 @@ TIS_DEV void commit_replica(const Dev &D, int r) {
     S.klok = klok;
     S.lr_due = (klok % P.lr_every) == 1 || prog == 0;
 }
-__global__ void __launch_bounds__(kBlock) k_commit(Dev D) {
-    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (r < D.n_rep) commit_replica<LdPlain>(D, r);
+
+// k_integrate: the integration of every particle, then the close of each
+// replica's step (commit_replica, check_shifts) by the workgroup that finishes
+// the replica's particles last.  Each workgroup, once its waves' displacement
+// maxima are in place (the atomics completed: s_waitcnt), counts itself at each
+// replica it touched (a returning atomic on D.close_cnt); the one that brings a
+// replica's count to its number of workgroups reads that replica's maxima with
+// device-scope loads (coherent with the atomics on the same words) and closes
+// the step, so no kernel of its own follows.  Every other input of the close
+// comes from kernels the launch waited for.
+__global__ void __launch_bounds__(kBlock) k_integrate(Dev D) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    integrate_one<LdPlain>(D, p, p < D.n_tot, (int)(p >> 6) % kMaxSlots);
+    __shared__ int last[kBlock], n_last;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int pa = blockIdx.x * blockDim.x, pb = min(pa + (int)blockDim.x, D.n_tot) - 1;
+        int n = 0;
+        for (int r = D.prep[pa]; r <= D.prep[pb]; ++r) {
+            const RepParams &P = D.rp[r];
+            const int b0 = P.p0 / (int)blockDim.x, b1 = (P.p0 + P.n - 1) / (int)blockDim.x;
+            if (atomicAdd(D.close_cnt + (size_t)r * kCloseStride, 1) == b1 - b0) last[n++] = r;
+        }
+        n_last = n;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    for (int q = 0; q < n_last; ++q) {
+        const int r = last[q];
+        if (threadIdx.x == 0) D.close_cnt[(size_t)r * kCloseStride] = 0;
+        commit_replica<LdCoh>(D, r);
+    }
 }
+
 
 // ------------------------------------------------------------ replica exchange
 // Helpers of the on-device temperature exchange between neighbouring replicas
