@@ -73,8 +73,10 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
                    gate reuses), and the resolution's 27 B (energy 8, gate 1, id 4,
                    outcome 2, incidence offsets 4, two incidence entries 8); 12 B
                    per site; 144 B per kept candidate (its 6 bead forces)
-      k_integrate  148 B per particle (x, x_old, v, 3 forces read, 3 states written)
-      k_commit     128 B per particle
+      k_integrate  312 B per particle (x, x_old, v read and written 192, pair forces
+                   and random force read 96, cell-ordered copy written 24) + 32 B per
+                   bead (multi-body force); the close of each replica's step is a few
+                   hundred bytes per replica
     """
     tot = 0.0
     for r, h in enumerate(handles):
@@ -89,9 +91,7 @@ def algorithmic_bytes(eng, handles, kernel, lr_every=10):
         elif kernel == "k_hbres":
             tot += 147 * nact + 12 * d["n_hb_sites"] + 144 * int(np.sum(eng.hb_outcome(r) == 1))
         elif kernel == "k_integrate":
-            tot += 148 * n
-        elif kernel == "k_commit":
-            tot += 128 * n
+            tot += 312 * n + 32 * d["n_beads"]
         else:
             return None
     return tot
