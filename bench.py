@@ -264,11 +264,11 @@ def cpu_baseline(systems, steps_each):
     return total_steps / total_t
 
 
-def measured_valu():
-    """Per-kernel VALU issue fraction on the chain workload from the committed
-    counter summary (SQ_INSTS_VALU per dispatch over the dispatch's duration
-    times 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per cycle), or None."""
-    path = os.path.join(REPO, "profiles", "pmc_chain1500.json")
+def measured_valu(summary="pmc_chain1500.json"):
+    """Per-kernel VALU issue fraction from a committed counter summary
+    (SQ_INSTS_VALU per dispatch over the dispatch's duration times 256 CUs x
+    2.4 GHz, one wave64 VALU instruction per CU per cycle), or None."""
+    path = os.path.join(REPO, "profiles", summary)
     try:
         with open(path) as f:
             return {k: v for k, v in json.load(f)["kernels"].items() if "valu_frac" in v}
@@ -645,6 +645,7 @@ def main():
             cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
                    "sample": f"FP64 CPU oracle (reference cell lists, glibc-stream noise), "
                              f"{args.cpu_steps} steps of each of the 16 replicas, single thread"}
+    valu = measured_valu("pmc_traffic_batched16_16SCD.json")
     out = {
         "metric": METRIC,
         "value": value,
@@ -676,6 +677,7 @@ def main():
         "roofline_step": {"bound": "hbm", "bytes_per_step": b_step, "achieved": b_step * (value / n_rep) / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": b_step * (value / n_rep) / 1e9 / HBM_PEAK_GBS},
+        "valu_issue_frac": None if valu is None else {k: v["valu_frac"] for k, v in valu.items()},
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": [e["n_hb_active"] for e in energies],
         "cpu_baseline": cpu,
