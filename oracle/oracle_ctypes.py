@@ -35,6 +35,7 @@ def load(path=LIB_PATH):
     lib.oracle_get_state.argtypes = [vp, dp, dp, dp]
     lib.oracle_set_stream.argtypes = [vp, C.POINTER(C.c_int32), C.c_longlong]
     lib.oracle_set_stream_glibc.argtypes = [vp, C.c_uint, C.c_longlong]
+    lib.oracle_stream_glibc_more.argtypes = [vp, C.c_longlong]
     lib.oracle_stream_cursor.argtypes = [vp]
     lib.oracle_stream_cursor.restype = C.c_longlong
     lib.oracle_eval.argtypes = [vp, C.c_uint, dp, dp]
@@ -98,6 +99,10 @@ class Oracle:
 
     def set_stream_glibc(self, seed, n):
         assert self.lib.oracle_set_stream_glibc(self.ctx, seed, n) == 0
+
+    def stream_glibc_more(self, n):
+        """Keeps the unconsumed values and appends the next n of the glibc sequence."""
+        assert self.lib.oracle_stream_glibc_more(self.ctx, n) == 0
 
     def cursor(self):
         return self.lib.oracle_stream_cursor(self.ctx)

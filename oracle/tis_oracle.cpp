@@ -709,6 +709,15 @@ int oracle_set_stream(oracle_ctx *o, const int32_t *raw, long long n) {
 
 long long oracle_stream_cursor(const oracle_ctx *o) { return o ? o->cursor : -1; }
 
+int oracle_stream_glibc_more(oracle_ctx *o, long long n) {
+    if (!o || n < 0) return TIS_EINVAL;
+    o->stream.erase(o->stream.begin(), o->stream.begin() + std::min<long long>(o->cursor, (long long)o->stream.size()));
+    for (long long q = 0; q < n; ++q) o->stream.push_back(std::rand());
+    o->cursor = 0;
+    o->stream_overflow = false;
+    return TIS_OK;
+}
+
 int oracle_set_stream_glibc(oracle_ctx *o, unsigned seed, long long n) {
     if (!o || n < 0) return TIS_EINVAL;
     o->stream.resize(n);

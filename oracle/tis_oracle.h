@@ -51,6 +51,11 @@ int oracle_set_stream(oracle_ctx *o, const int32_t *raw, long long n);
 long long oracle_stream_cursor(const oracle_ctx *o);
 /* Fills the stream with n values of the C library's rand() after srand(seed). */
 int oracle_set_stream_glibc(oracle_ctx *o, unsigned seed, long long n);
+/* Long runs: drops the consumed part of the stream and appends the next n
+ * values of the same glibc sequence (the process's rand() state continues from
+ * the last oracle_set_stream_glibc / oracle_stream_glibc_more; one context per
+ * process at a time). */
+int oracle_stream_glibc_more(oracle_ctx *o, long long n);
 /* Forces of the selected terms (TIS_TERM_* mask) at the current state, written to
  * f_out (3N), energies to e_out[OE_N].  The HB term consumes the stream. */
 int oracle_eval(oracle_ctx *o, unsigned mask, double *f_out, double *e_out);
