@@ -16,6 +16,8 @@ the state.  The two ensembles (64 and 16 trajectories) must agree in mean: Welch
 independent; with about 15 degrees of freedom on the oracle side the chance of a
 false alarm over all 13 is about 0.5 %).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -88,3 +90,68 @@ def test_ensemble_at_fixed_time_matches_the_reference_process(lib, cfg1):
               f"+- {c.std(ddof=1):10.6f}   z {z:+.2f}", flush=True)
         assert abs(z) < 4.5, k
     print("max |z|", worst)
+
+
+# ---------------------------------------------------------------- long window
+# tests/golden/stats_5mM_1e5.npz: 32 trajectories of the reference's process (the
+# FP64 oracle with glibc streams, sequential HB resolution and the reference's
+# noise-slot sharing) from the same start state, 10^5 steps each, sampled every
+# 10^4 steps (tests/golden/make_stats_fixture.py).
+STATS = os.path.join(os.path.dirname(__file__), "golden", "stats_5mM_1e5.npz")
+N_LONG = 64
+COLUMNS = ("rg", "bound_mg", "e_hb", "e_stack", "n_kept", "p_mg_1", "p_mg_2", "p_mg_3", "p_mg_4", "p_mg_5",
+           "st3_formed")
+
+
+def _rows(rg, bound_mg, e_hb, e_stack, n_kept, p_mg, st3, st3_E):
+    """Observables of one state: the scalars, Mg contacts summed over five
+    blocks of consecutive phosphates, and the tertiary stacks below half their
+    stacking energy (Q < 1: formed)."""
+    blocks = np.array_split(np.arange(p_mg.shape[-1]), 5)
+    return [rg, bound_mg, e_hb, e_stack, n_kept] + [float(p_mg[b].sum()) for b in blocks] + \
+        [float(np.count_nonzero(st3 < 0.5 * st3_E))]
+
+
+@pytest.mark.skipif(not os.path.exists(STATS), reason="statistics fixture not generated")
+def test_long_window_observables_match_the_reference_process(lib, cfg1):
+    """Rg, bound Mg, Mg contacts along the chain, HB and stacking energies, kept
+    HB candidates and formed tertiary stacks at 10 fixed times up to 10^5 steps
+    (244 ps): 64 device trajectories (mixed precision, Philox noise per particle,
+    component-parallel HB resolution) against the 32 reference-process
+    trajectories of the fixture.  Every (time, observable) pair must agree in
+    mean, Welch |z| < 5 (110 comparisons, each between independent
+    trajectories: about 0.2 % chance of a false alarm in all with ~31 degrees
+    of freedom on the reference side).  The 10^9-step equilibrium means the
+    reference reports (analysis/Analysis_storedvalues_ave.txt) lie far beyond a
+    test and stay unpinned."""
+    ref = np.load(STATS)
+    times = ref["times"]
+    d = cfg1[0]
+    st3_E = np.asarray(d["st3_E"])
+    p_beads = np.nonzero(np.asarray(d["cls"])[: d["n_beads"]] == OB.CLS_P)[0]
+    eng = T.Engine(lib, [T.SystemHandle(d)] * N_LONG, precision=T.PREC_MIXED, rng=T.RNG_PHILOX, seed=2024)
+    done, worst = 0, 0.0
+    for k, t in enumerate(times):
+        eng.step(int(t) - done)
+        done = int(t)
+        eng.observe(0)
+        eng.observe_now()
+        g = []
+        for r in range(N_LONG):
+            o = eng.observables(r)
+            g.append(_rows(o["rg"], o["bound_mg"], o["e2_hb"] + o["e3_hb"], o["e2_stack"] + o["e3_stack"],
+                           o["hb_kept"].sum(), o["per_bead"][p_beads, 0], o["st3_mean"], st3_E))
+        c = [_rows(ref["rg"][j, k], ref["bound_mg"][j, k], ref["e_hb"][j, k], ref["e_stack"][j, k],
+                   ref["n_kept"][j, k], ref["p_mg"][j, k].astype(float), ref["st3"][j, k], st3_E)
+             for j in range(ref["rg"].shape[0])]
+        g, c = np.asarray(g, dtype=float), np.asarray(c, dtype=float)
+        line = []
+        for q, name in enumerate(COLUMNS):
+            se = np.sqrt(g[:, q].var(ddof=1) / len(g) + c[:, q].var(ddof=1) / len(c))
+            z = 0.0 if se == 0 else (g[:, q].mean() - c[:, q].mean()) / se
+            worst = max(worst, abs(z))
+            line.append(f"{name} {g[:, q].mean():.3f}/{c[:, q].mean():.3f} ({z:+.1f})")
+            assert abs(z) < 5.0, (int(t), name, g[:, q].mean(), c[:, q].mean(), z)
+        print(f"step {int(t)}: " + ", ".join(line), flush=True)
+    print("max |z|", worst)
+    eng.close()
