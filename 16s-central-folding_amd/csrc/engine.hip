@@ -181,6 +181,7 @@ struct tis_ctx {
     double *fm_t3 = nullptr, *fm_out3 = nullptr, *fm_small = nullptr;   // packed target / outputs, parameter staging
     double *fm_loss = nullptr, *fm_ion = nullptr, *fm_ghb = nullptr, *fm_gst3 = nullptr, *fm_gst2 = nullptr;
     double *scratch_dev = nullptr;   // this device's `scratch` (tis_force_match_noalloc)
+    unsigned long long *hb_trace_buf = nullptr;   // tis_trace_hb ring (allocated when first switched on)
 
     ~tis_ctx() {
         if (p_slot >= 0) give_persist_slot(p_slot);
@@ -1213,8 +1214,6 @@ extern "C" int tis_set_rng_epoch(tis_ctx *c, int rep, unsigned epoch) {
     });
 }
 
-// Captures and instantiates the graph of opt.graph_steps steps (launching
-// nothing) unless it exists.
 // Captures and instantiates the graph of opt.graph_steps steps starting from the
 // current parity of the state buffers (launching nothing) unless it exists.
 // Capture issues no work, so the parity is restored afterwards.
@@ -1261,6 +1260,30 @@ extern "C" int tis_get_persistent_counts(tis_ctx *c, int *active, long long *lau
         why[why_len - 1] = 0;
     }
     return TIS_OK;
+}
+
+extern "C" int tis_trace_hb(tis_ctx *c, int on) {
+    if (!c || (on != 0 && on != 1)) return fail(Fail{TIS_EINVAL, "tis_trace_hb: bad arguments"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        const size_t words = (size_t)kHbTraceSteps * c->n_rep * kMaxHbGroups * kHbTraceWords;
+        if (on && !c->hb_trace_buf) c->hb_trace_buf = dalloc<unsigned long long>(c->owned, words);
+        if (on) HIP_TRY(hipMemset(c->hb_trace_buf, 0, words * sizeof(unsigned long long)));
+        c->D.hb_trace = on ? c->hb_trace_buf : nullptr;
+        c->drop_graphs();   // captured graphs hold the old D
+        return TIS_OK;
+    });
+}
+
+extern "C" int tis_get_hb_trace(tis_ctx *c, unsigned long long *out, long long n) {
+    const long long words = c ? (long long)kHbTraceSteps * c->n_rep * kMaxHbGroups * kHbTraceWords : 0;
+    if (!c || !out || n < words) return fail(Fail{TIS_EINVAL, "tis_get_hb_trace: bad arguments"});
+    if (!c->hb_trace_buf) return fail(Fail{TIS_EINVAL, "tis_get_hb_trace: the trace was never switched on"});
+    return guarded([&]() -> int {
+        HIP_TRY(hipStreamSynchronize(c->s_main));
+        HIP_TRY(hipMemcpy(out, c->hb_trace_buf, words * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        return TIS_OK;
+    });
 }
 
 extern "C" int tis_get_launch_counts(tis_ctx *c, long long *graph_launches, long long *graph_steps,

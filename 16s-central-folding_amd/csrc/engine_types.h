@@ -70,6 +70,14 @@ constexpr int kActMax = 3072;     // live candidates
 constexpr int kASiteMax = 3072;   // sites with live candidates
 constexpr int kAIncMax = 8192;    // incidences of live candidates
 
+// Phase trace of the HB resolution: a ring of kHbTraceSteps steps (slot klok %
+// kHbTraceSteps); per (slot, replica, group) kHbTraceWords words: the real-time
+// clock (100 MHz) at the group's start and at the end of each phase (tables in
+// LDS, excess ranked, components joined, roots ranked, resolution, outcome and
+// forces, end), the serial depth, then n_over | n_comp << 16 | n_live << 32.
+constexpr int kHbTraceSteps = 64;
+constexpr int kHbTraceWords = 10;
+
 struct Dev {
     int n_rep;
     int n_tot;
@@ -136,6 +144,7 @@ struct Dev {
     int *hb_iter_part;            // per replica and resolution group: serial depth of its wavefronts
     double *hb_e_part;            // per replica and resolution group: its E2, E3 HB sums
     signed char *hb_out;          // outcome of the last HB evaluation: -1 not live-listed, 0 gated, 1 kept, 2 off
+    unsigned long long *hb_trace; // phase clock of the resolution (tis_trace_hb), nullptr when off
     // per replica and live index (r * kActMax + a), written by k_hbenergy: energy,
     // gate flag and the six bead forces (18 values of the force precision)
     double *hb_en;

@@ -647,13 +647,15 @@ static_assert(kResWaves * kCompWin >= kSeqWin, "window regions are shared by the
 static_assert(kASiteMax <= 64 * 64, "over-subscribed sites must fit a 64 x 64 bit mask");
 // LDS layout of one replica's working set (bytes, in this order):
 //   E[kActMax] double | uwin[kResWaves * kCompWin] double |
-//   par, excess, apack [kASiteMax] int | A1, croot, cidx [kASiteMax] short |
-//   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag[kActMax] u8
+//   par, excess, apack [kASiteMax] int | croot [kASiteMax] short | A1 [2 kASiteMax] short |
+//   aoff[kASiteMax + 4] short | hoff[kActMax + 4] short | alist, hinc [kAIncMax] short | flag, dcls [kActMax] u8
 constexpr size_t kHbLds = kActMax * 8 + kResWaves * kCompWin * 8 + 3 * kASiteMax * 4 + 3 * kASiteMax * 2 +
-                          (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + kActMax;
+                          (kASiteMax + 4) * 2 + (kActMax + 4) * 2 + 2 * kAIncMax * 2 + 2 * kActMax;
 static_assert(kHbLds <= 160 * 1024 - 1024, "HB working set must fit the LDS of one CU");
 
-enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2 };
+// FLAG_TAKEN (LDS only): switched off and taken back (k_hbres)
+enum { FLAG_GATED = 0, FLAG_LIVE = 1, FLAG_OFF = 2, FLAG_TAKEN = 3 };
+constexpr unsigned char kGateDode = 0x10;   // k_hbenergy's gate byte: tertiary class (E3_HB)
 
 // HB geometry of candidate positions p[0..5] with its constants geo (r0, theta1,
 // theta2, psi0, psi1, psi2): Q and the re-wrapped dihedral targets.
@@ -845,7 +847,8 @@ TIS_DEV void wave_lds_sync() {
 // (i, c), c < i, triangular index t = i(i-1)/2 + c) and reads their bits, a
 // shorter one evaluates the tests on its path.  Draws come from an LDS window of `win` values that the wavefront
 // refills.  Sets *ovf on a site with more than 64 live candidates or an
-// inconsistent table.
+// inconsistent table.  The switched-off candidates are recorded in offl (the
+// first cap of them; *noff_out counts all).
 // Pairs (i, c), c < i, of the all-at-once walk evaluated by this lane in passes
 // 0..3: triangular index t = 64 p + lane = i (i - 1) / 2 + c.
 TIS_DEV void walk_pairs(int pi[4], int pc[4]) {
@@ -926,13 +929,13 @@ TIS_DEV int resolve_site_pass(int pk, const short *alist, const unsigned char *f
 TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, const int *apack,
                           const short *alist, const short *hoff, const short *hinc, unsigned char *flag,
                           const double *E, double T, double *ubuf, int win, const HbDraws &dr, int comp,
-                          int *iters_out, long long *base_out, bool *ovf) {
+                          short *offl, int cap, int *iters_out, int *noff_out, long long *base_out, bool *ovf) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     int pi[4], pc[4];
     walk_pairs(pi, pc);
     long long base = 0, win_lo = 0;
-    int iters = 0;
+    int iters = 0, noff = 0;
     const double invT = 1.0 / T;
     dr.fill(ubuf, 0, win, lane, 64, comp);
     wave_lds_sync();
@@ -963,8 +966,13 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
         const int off = resolve_site_pass(apack[ci], alist, flag, E, u, T, invT, pi, pc, &nK);
         if (off < 0) { *ovf = true; break; }
         base += 2 * nK;
-        // ---- switch it off; sites whose excess reaches 0 leave the mask
-        if (lane == 0) flag[off] = FLAG_OFF;
+        // ---- switch it off (recorded while the record has room); sites whose
+        // excess reaches 0 leave the mask
+        if (lane == 0) {
+            flag[off] = FLAG_OFF;
+            if (noff < cap) offl[noff] = (short)off;
+        }
+        ++noff;
         const int h0 = hoff[off], hn = hoff[off + 1] - h0;
         if (hn > 64) { *ovf = true; break; }
         int b = -1;
@@ -987,6 +995,7 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
     }
     if (base > dr.avail()) *ovf = true;
     *iters_out = iters;
+    *noff_out = noff;
     *base_out = base;
 }
 
@@ -1000,13 +1009,14 @@ TIS_DEV void resolve_wave(unsigned long long w, int *excess, int c0, int nA0, co
 // lower lane excesses only (once per incidence, multiplicity kept).
 TIS_DEV void resolve_small(int nm, int mci, int ex, int pk, const short *alist, const short *hoff,
                            const short *hinc, unsigned char *flag, const double *E, double T, double *ubuf,
-                           int win, const HbDraws &dr, int comp, int *iters_out, long long *base_out, bool *ovf) {
+                           int win, const HbDraws &dr, int comp, short *offl, int cap, int *iters_out,
+                           int *noff_out, long long *base_out, bool *ovf) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     int pi[4], pc[4];
     walk_pairs(pi, pc);
     long long base = 0, win_lo = 0;
-    int iters = 0;
+    int iters = 0, noff = 0;
     const double invT = 1.0 / T;
     int exv = lane < nm ? ex : 0;
     dr.fill(ubuf, 0, win, lane, 64, comp);
@@ -1031,8 +1041,13 @@ TIS_DEV void resolve_small(int nm, int mci, int ex, int pk, const short *alist, 
         const int off = resolve_site_pass(readlane_i(pk, L), alist, flag, E, u, T, invT, pi, pc, &nK);
         if (off < 0) { *ovf = true; break; }
         base += 2 * nK;
-        // ---- switch it off and lower the excess of its sites
-        if (lane == 0) flag[off] = FLAG_OFF;
+        // ---- switch it off (recorded while the record has room) and lower the
+        // excess of its sites
+        if (lane == 0) {
+            flag[off] = FLAG_OFF;
+            if (noff < cap) offl[noff] = (short)off;
+        }
+        ++noff;
         const int h0 = hoff[off], hn = hoff[off + 1] - h0;
         if (hn > 64) { *ovf = true; break; }
         const int ci2 = lane < hn ? hinc[h0 + lane] : -1;   // over-subscribed index, or -1
@@ -1044,6 +1059,7 @@ TIS_DEV void resolve_small(int nm, int mci, int ex, int pk, const short *alist, 
     }
     if (base > dr.avail()) *ovf = true;
     *iters_out = iters;
+    *noff_out = noff;
     *base_out = base;
 }
 
@@ -1094,9 +1110,12 @@ __global__ void __launch_bounds__(kBlock) k_hbsum(Dev D) {
 // fast.cpp:2180-2237, forces :2333-2345), one thread per candidate, before the
 // valence resolution decides which are kept: E = E_hb exp(-Q) when |r - r0| <= 2 A
 // for the (k11, k21) pair (gate flag FLAG_LIVE), else 0 (FLAG_GATED); inside the
-// gate the six bead forces of the candidate, to be added by k_hbres if it keeps
-// it.  Computed in precision R once per candidate, for every resolution group.
-// Grid (kActMax / kBlock, replicas); outputs at r * kActMax + live index.
+// gate the six bead forces, which k_hbres adds to the step's multi-body forces
+// (and takes back from the candidates it switches off).  The outcome (hb_out,
+// act_flag) is the gate until the resolution changes it.  Computed in precision R
+// once per candidate, for every resolution group.  Grid (kActMax / kBlock,
+// replicas); outputs at r * kActMax + live index, the gate byte with kGateDode
+// set for the tertiary class (E3_HB).
 // Candidate a (live index) of replica r; n_act and tab: the replica's current
 // table counts and buffer.
 template <class R, class L>
@@ -1110,6 +1129,7 @@ TIS_DEV void hbenergy_one(const Dev &D, int r, int a, int n_act, int tab) {
                            L::ld(D.pos + rb.x), L::ld(D.pos + rb.y), L::ld(D.pos + rb.z)};
     const double *geo = D.hb_g + 6 * g;
     const double hE = D.hb_E[g];
+    const unsigned char cls = D.hb_dode[g] ? kGateDode : 0;
     const size_t o = (size_t)kActMax * r + a;
     const V3<R> v = diff<R>(pt[1], pt[4]);
     const R rr = sqrt(dot(v, v));
@@ -1131,8 +1151,38 @@ TIS_DEV void hbenergy_one(const Dev &D, int r, int a, int n_act, int tab) {
         }
     }
     D.hb_en[o] = e;
-    D.hb_gate[o] = f;
+    D.hb_gate[o] = f | cls;
+    D.hb_out[g] = (signed char)f;
+    D.act_flag[P.hb0 + a] = f;
 }
+
+// The six bead forces k_hbenergy kept for live candidate a (a < 0: none), added
+// to the step's multi-body forces, or, for a candidate the resolution switched
+// off (off), subtracted again, with its outcome set to FLAG_OFF.
+template <class R, class L>
+struct HbForces {
+    const Dev *D;
+    const int4 *rec;             // T.act_rec + 2 hb0: records by live index
+    const R *frc;                // the replica's kept forces, 18 per live index
+    unsigned char *act_flag;     // D.act_flag + hb0
+    TIS_DEV void operator()(int a, bool off) const {
+        if (a < 0) return;
+        const int4 ra = L::ld(rec + 2 * a), rb = L::ld(rec + 2 * a + 1);
+        const R *fr = frc + 18 * (size_t)a;
+        const R sg = off ? (R)-1 : (R)1;
+        V3<R> fb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            fb[q] = V3<R>{sg * L::ld(fr + 3 * q), sg * L::ld(fr + 3 * q + 1), sg * L::ld(fr + 3 * q + 2)};
+        if (off) {
+            D->hb_out[ra.x] = (signed char)FLAG_OFF;
+            act_flag[a] = FLAG_OFF;
+        }
+        const int bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) atomic_add_force(D->f_mb, bd[q], fb[q]);
+    }
+};
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
     const int r = blockIdx.y;
@@ -1141,9 +1191,11 @@ __global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
     hbenergy_one<R, LdPlain>(D, r, blockIdx.x * blockDim.x + threadIdx.x, n_act, S.tab);
 }
 
-// Energies, gates and valence resolution of the live HB candidates; writes the
-// outcome of every live candidate.  mode bit 0: a step (advance the stream
-// cursor past the HB draws and build the noise table).
+// Valence resolution of the live HB candidates (hydrogen_bonds_interactions,
+// fast.cpp:2239-2315) after k_hbenergy's gates, energies and forces; switches
+// off candidates until no site exceeds its valence, takes back what k_hbenergy
+// added for them and sums the kept candidates' energies.  mode bit 0: a step
+// (advance the stream cursor past the HB draws and build the noise table).
 //
 // Reference stream: the exact sequential process on wavefront 0.  Philox draws:
 // the over-subscribed sites split into the connected components of the graph
@@ -1156,19 +1208,14 @@ __global__ void __launch_bounds__(kBlock) k_hbenergy(Dev D) {
 // distribution of switched-off candidates as the single sequential loop, with
 // the serial depth of the largest wavefront's share instead of the total.
 //
-// First, the energy and distance gate of every live candidate
-// (hydrogen_bonds, fast.cpp:2180-2237): E = E_hb exp(-Q) when |r - r0| <= 2 A for
-// the (k11, k21) pair, gated otherwise; computed in precision R into LDS by every
-// group.  Last, the forces of the kept candidates and the groups' energy sums.
-//
 // Grid (groups, replicas).  With Philox draws the components of a replica are
 // shared out over the groups (component k to group k mod groups, each group's
 // wavefronts taking its components in turn); every group builds the tables and
-// components itself and writes back the outcome (and the forces of the kept
-// ones) of the candidates whose component it ran (candidates outside every
-// component: group 0).  Each
-// component draws from its own Philox stream, so the outcome does not depend on
-// the number of groups.  The reference-stream process runs in group 0.
+// components itself, and the wavefront that ran a component takes back the
+// candidates it switched off as soon as it is done.  Each component draws from
+// its own Philox stream, so the outcome does not depend on the number of groups.
+// The reference-stream process runs in group 0.  Energy sums: group 0 adds
+// every live candidate's energy, each group subtracts those it switched off.
 // Group grp of ngrp of replica r's resolution, a workgroup of kResBlock
 // threads with kHbLds bytes of LDS at lds.
 template <class R, class L>
@@ -1180,20 +1227,32 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
     int *par = (int *)(uwin + kResWaves * kCompWin);
     int *excess = par + kASiteMax;
     int *apack = excess + kASiteMax;
-    short *A1 = (short *)(apack + kASiteMax);
-    short *croot = A1 + kASiteMax;
-    short *cidx = croot + kASiteMax;
-    short *aoff = cidx + kASiteMax;
+    int *cnt = apack;   // live incidences per active site, until the ranking writes apack
+    short *croot = (short *)(apack + kASiteMax);
+    short *A1 = croot + kASiteMax;
+    short *offl = A1;   // 2 kASiteMax entries: each wavefront's switched-off candidates, once A1 is read
+    short *aoff = A1 + 2 * kASiteMax;
     short *hoff = aoff + kASiteMax + 4;
     short *alist = hoff + kActMax + 4;
     short *hinc = alist + kAIncMax;
     unsigned char *flag = (unsigned char *)(hinc + kAIncMax);
+    unsigned char *dcls = flag + kActMax;   // kGateDode of each live candidate
     __shared__ int scratch[20];
     __shared__ int sh_iters;
     __shared__ long long sh_draws;
-    __shared__ int sh_err, sh_next;
+    __shared__ int sh_err, sh_next, sh_fnext;
     const int n_act = L::ld(&S.n_act), n_as = L::ld(&S.n_asite), n_ainc = L::ld(&S.n_ainc);
     const HbTab T = hb_tab(D, L::ld(&S.tab));
+    // phase clock (tis_trace_hb): thread 0 stamps the end of each phase
+    unsigned long long *trc = nullptr;
+    if (D.hb_trace && threadIdx.x == 0) {
+        const long long slot = L::ld(&S.klok) % kHbTraceSteps;
+        trc = D.hb_trace + (((size_t)slot * D.n_rep + r) * kMaxHbGroups + grp) * kHbTraceWords;
+        trc[0] = __builtin_amdgcn_s_memrealtime();
+    }
+    auto stamp = [&](int k) {
+        if (trc) trc[k] = __builtin_amdgcn_s_memrealtime();
+    };
     if (threadIdx.x == 0) {   // this group's parts (summed by k_hbsum or the step's close)
         D.hb_iter_part[kMaxHbGroups * r + grp] = 0;
         D.hb_e_part[2 * (kMaxHbGroups * r + grp)] = 0.0;
@@ -1201,13 +1260,13 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
     }
     if (L::ld(&S.overflow) & 2) return;
     if (D.rng == 1 && grp > 0) return;
-    if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; }
+    if (threadIdx.x == 0) { sh_err = 0; sh_draws = 0; sh_iters = 0; sh_next = kResWaves; sh_fnext = 0; }
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
     // ---- the rebuild tables with the active sites' valences, and the candidate
     // energies and gates of k_hbenergy: one level of loads, all issued before any
-    // is used.
-    const int n_hinc = L::ld(T.ahb_inc_off + P.hb0 + r + n_act);
-    if (n_hinc > kAIncMax) {                      // inconsistent rebuild tables
+    // is used.  The live incidences counted from the sites (n_ainc) are those
+    // counted from the candidates.
+    if (n_ainc > kAIncMax) {                      // inconsistent rebuild tables
         if (tid == 0) atomicOr(&S.overflow, 2);
         return;
     }
@@ -1237,20 +1296,13 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
     for (int k = 0; k < KI; ++k) {
         const int i = tid + k * kResBlock;
         t_alist[k] = i < n_ainc ? L::ld(T.asite_list + P.inc0 + i) : 0;
-        t_hinc[k] = i < n_hinc ? L::ld(T.ahb_inc + P.inc0 + i) : 0;
-    }
-    // the thread's first candidate's record with the tables (later ones, rare
-    // below 1024 live candidates, load in turn when their forces are added)
-    int4 ra0 = make_int4(0, 0, 0, 0), rb0 = ra0;
-    if (tid < n_act) {
-        ra0 = L::ld(T.act_rec + 2 * (P.hb0 + tid));
-        rb0 = L::ld(T.act_rec + 2 * (P.hb0 + tid) + 1);
+        t_hinc[k] = i < n_ainc ? L::ld(T.ahb_inc + P.inc0 + i) : 0;
     }
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
         const int i = tid + k * kResBlock;
         if (i <= n_as) aoff[i] = (short)t_aoff[k];
-        if (i < n_as) excess[i] = t_val[k];       // the valence, until the excess replaces it
+        if (i < n_as) { excess[i] = t_val[k]; cnt[i] = 0; }   // the valence, until the excess replaces it
     }
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
@@ -1258,14 +1310,42 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
         if (i <= n_act) hoff[i] = (short)t_hoff[k];
         if (i < n_act) {
             E[i] = t_e[k];
-            flag[i] = t_f[k];
+            flag[i] = t_f[k] & 0x0f;
+            dcls[i] = t_f[k] & kGateDode;
         }
     }
 #pragma unroll
     for (int k = 0; k < KI; ++k) {
         const int i = tid + k * kResBlock;
-        if (i < n_ainc) alist[i] = (short)t_alist[k];
-        if (i < n_hinc) hinc[i] = (short)t_hinc[k];
+        if (i < n_ainc) {
+            alist[i] = (short)t_alist[k];
+            hinc[i] = (short)t_hinc[k];
+        }
+    }
+    __syncthreads();
+    stamp(1);
+    // ---- live incidences per active site: each live candidate's thread adds
+    // one at every site it lists (with multiplicity, as the sites' lists hold it)
+    int t_hend[KC];
+    bool t_live[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int a = tid + k * kResBlock;
+        t_hend[k] = a < n_act ? hoff[a + 1] : 0;
+        t_live[k] = a < n_act && flag[a] == FLAG_LIVE;
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        if (t_live[k]) {
+            const int q0 = t_hoff[k], nq = t_hend[k] - q0;
+            int st[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) st[j] = j < nq ? hinc[q0 + j] : 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j < nq) atomicAdd(&cnt[st[j]], 1);
+            for (int q = q0 + 4; q < q0 + nq; ++q) atomicAdd(&cnt[hinc[q]], 1);
+        }
     }
     __syncthreads();
     // ---- excess = live incidences - valence per active site; the
@@ -1283,14 +1363,12 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
             const int s = s_lo + k;
             exs[k] = 0;
             if (k < per && s < n_as) {
-                int c = 0;
-                for (int q = aoff[s]; q < aoff[s + 1]; ++q) c += flag[alist[q]] == FLAG_LIVE;
-                exs[k] = c - excess[s];
+                exs[k] = cnt[s] - excess[s];
                 mine += exs[k] > 0;
             }
         }
-        // excess re-indexed by over-subscribed index (rank <= s); every valence
-        // was read before the scan's barriers
+        // excess re-indexed by over-subscribed index (rank <= s); every count and
+        // valence was read before the scan's barriers
         int rank = block_excl_sum(mine, scratch, &nA0);
 #pragma unroll
         for (int k = 0; k < KS; ++k) {
@@ -1308,49 +1386,118 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
         }
     }
     __syncthreads();
+    stamp(2);
+    int ncomp = 0;
     const HbDraws dr(D, r, L::ld(&S.klok), L::ld(&S.rng_epoch), L::ld(&S.stream_cursor));
+    const HbForces<R, L> frc{&D, T.act_rec + 2 * P.hb0, (const R *)D.hb_frc + 18 * (size_t)kActMax * r,
+                             D.act_flag + P.hb0};
+    // The forces of every live candidate, added by the wavefronts that have no
+    // (more) components to run, while the others resolve theirs: this group's
+    // share of the live indices, in chunks of 64 handed out by sh_fnext
+    auto add_live = [&]() {
+        const bool all = D.rng == 1;
+        const int lo = all ? 0 : (int)((long long)n_act * grp / ngrp);
+        const int hi = all ? n_act : (int)((long long)n_act * (grp + 1) / ngrp);
+        for (;;) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(&sh_fnext, 1);
+            c = __shfl(c, 0, 64);
+            const int a = lo + 64 * c + lane;
+            if (lo + 64 * c >= hi) break;
+            // the gate never changes here (the resolution moves live to off)
+            frc(a < hi && flag[a] != FLAG_GATED ? a : -1, false);
+        }
+    };
+    // A wavefront takes back the candidates it switched off: the n it recorded
+    // (up to cap), then, if the record filled up, the rest, found by their outcome
+    // and their component (root; -1: every over-subscribed site)
+    auto take_wave = [&](const short *rec, int n, int cap, int root) {
+        const int nr = min(n, cap);
+        for (int i0 = 0; i0 < nr; i0 += 64) {
+            const int a = i0 + lane < nr ? rec[i0 + lane] : -1;
+            frc(a, true);
+            if (a >= 0) flag[a] = FLAG_TAKEN;
+        }
+        if (n > cap) {
+            for (int a0 = 0; a0 < n_act; a0 += 64) {
+                const int a = a0 + lane;
+                bool mine = a < n_act && flag[a] == FLAG_OFF;
+                if (mine && root >= 0) {
+                    int ci = -1;
+                    for (int q = hoff[a]; q < hoff[a + 1] && ci < 0; ++q) ci = hinc[q];
+                    mine = ci >= 0 && par[ci] == root;
+                }
+                frc(mine ? a : -1, true);
+                if (mine) flag[a] = FLAG_TAKEN;
+            }
+        }
+        wave_lds_sync();
+    };
+    // ---- each live candidate's thread turns its incidences into over-subscribed
+    // indices (-1: a site that is not) and, with Philox draws, joins the
+    // over-subscribed sites it touches (union-find over A0 indices; roots set with
+    // the excess).  Only live candidates are ever switched off, so only their
+    // incidences are read later.
+    {
+        volatile int *vpar = par;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            if (t_live[k]) {
+                const int q0 = t_hoff[k], nq = t_hend[k] - q0;
+                int st[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) st[j] = j < nq ? hinc[q0 + j] : 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) st[j] = j < nq ? A1[st[j]] : -1;
+                for (int q = q0 + 4; q < q0 + nq; ++q) hinc[q] = A1[hinc[q]];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < nq) hinc[q0 + j] = (short)st[j];
+                if (D.rng == 0) {
+                    int first = -1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (j >= nq || st[j] < 0) continue;
+                        if (first < 0) first = st[j];
+                        else uf_union(vpar, first, st[j]);
+                    }
+                    for (int q = q0 + 4; q < q0 + nq; ++q) {
+                        const int ci = hinc[q];
+                        if (ci < 0) continue;
+                        if (first < 0) first = ci;
+                        else uf_union(vpar, first, ci);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    stamp(3);
     if (D.rng == 1) {
-        // incidences as over-subscribed indices (-1: a site that is not)
-        for (int q = tid; q < n_hinc; q += nt) hinc[q] = A1[hinc[q]];
-        __syncthreads();
+        stamp(4);
         // reference stream: the exact sequential process on wavefront 0, over all
         // over-subscribed sites
         if (wv == 0) {
             const int c = 64 * lane;
             const unsigned long long w0 = c + 64 <= nA0 ? ~0ull : (c < nA0 ? (1ull << (nA0 - c)) - 1ull : 0ull);
-            int iters = 0;
+            int iters = 0, noff = 0;
             long long base = 0;
             bool ovf = false;
-            resolve_wave(w0, excess, 0, nA0, apack, alist, hoff, hinc, flag, E, P.T, uwin, kSeqWin, dr, 0, &iters,
-                         &base, &ovf);
+            resolve_wave(w0, excess, 0, nA0, apack, alist, hoff, hinc, flag, E, P.T, uwin, kSeqWin, dr, 0, offl,
+                         2 * kASiteMax, &iters, &noff, &base, &ovf);
+            take_wave(offl, noff, 2 * kASiteMax, -1);
             if (lane == 0) {
                 sh_iters = iters;
                 sh_draws = base;
                 if (ovf) sh_err = 1;
             }
         }
+        add_live();
     } else {
-        // ---- components of the over-subscribed sites (union-find over A0 indices;
-        // roots set with the excess).  Each candidate's thread turns its own
-        // incidences into over-subscribed indices (-1: a site that is not) and,
-        // if the candidate is live, joins the sites it touches.
         volatile int *vpar = par;
-        for (int a = tid; a < n_act; a += nt) {
-            const bool live = flag[a] == FLAG_LIVE;
-            int first = -1;
-            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = A1[hinc[q]];
-                hinc[q] = (short)ci;
-                if (!live || ci < 0) continue;
-                if (first < 0) first = ci;
-                else uf_union(vpar, first, ci);
-            }
-        }
-        __syncthreads();
         // roots (smallest index of each component) in order, each thread over a
         // contiguous range of indices; every index points at its root afterwards
         // (finds only shorten paths, so they may run beside each other)
-        int ncomp;
         {
             const int per = (nA0 + nt - 1) / nt;
             const int c_lo = min(nA0, tid * per);
@@ -1371,19 +1518,24 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
                 const int ci = c_lo + k;
                 if (k < per && ci < nA0 && rts[k] == ci) {
                     croot[rank] = (short)ci;
-                    cidx[ci] = (short)rank;
                     ++rank;
                 }
             }
         }
         __syncthreads();
+        stamp(4);
         // ---- each wavefront takes the next component in root order when it is done
         double *ub = uwin + wv * kCompWin;
+        constexpr int kOffCap = 2 * kASiteMax / kResWaves;
+        short *offw = offl + wv * kOffCap;
         int wave_iters = 0;
         bool ovf = false;
         for (int j = wv;;) {
             const int k = grp + ngrp * j;
-            if (k >= ncomp) break;
+            if (k >= ncomp) {
+                add_live();
+                break;
+            }
             const int root = croot[k], cfirst = root >> 6, c0 = cfirst << 6;
             // member mask (word w from the ballot over indices c0 + 64 w .. + 63)
             // and, up to 64 members, the members in index order in lanes 0, 1, ...
@@ -1401,16 +1553,17 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
                     ++nm;
                 }
             }
-            int it = 0;
+            int it = 0, noff = 0;
             long long base = 0;
             if (nm <= 64) {
                 const int exm = lane < nm ? excess[mci] : 0, pkm = lane < nm ? apack[mci] : 0;
-                resolve_small(nm, mci, exm, pkm, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it, &base,
-                              &ovf);
+                resolve_small(nm, mci, exm, pkm, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, offw,
+                              kOffCap, &it, &noff, &base, &ovf);
             } else {
-                resolve_wave(m, excess, c0, nA0, apack, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, &it,
-                             &base, &ovf);
+                resolve_wave(m, excess, c0, nA0, apack, alist, hoff, hinc, flag, E, P.T, ub, kCompWin, dr, k, offw,
+                             kOffCap, &it, &noff, &base, &ovf);
             }
+            take_wave(offw, noff, kOffCap, root);
             wave_iters += it;
             if (ovf) break;
             int nj = 0;
@@ -1423,40 +1576,24 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
         }
     }
     __syncthreads();
-    // ---- outcome and, for the kept candidates, energies and the forces
-    // k_hbenergy computed (hydrogen_bonds_interactions, fast.cpp:2333-2345), by
-    // the group that ran the candidate's component
-    double e2 = 0.0, e3 = 0.0;
-    for (int a = tid; a < n_act; a += nt) {
-        int owner = 0;
-        if (ngrp > 1 && D.rng == 0 && flag[a] != FLAG_GATED) {
-            for (int q = hoff[a]; q < hoff[a + 1]; ++q) {
-                const int ci = hinc[q];
-                if (ci >= 0) { owner = cidx[par[ci]] % ngrp; break; }
-            }
-        }
-        if (owner != grp) continue;
-        int4 ra = ra0, rb = rb0;
-        if (a != tid) {
-            ra = L::ld(T.act_rec + 2 * (P.hb0 + a));
-            rb = L::ld(T.act_rec + 2 * (P.hb0 + a) + 1);
-        }
-        const int g = ra.x;
-        D.hb_out[g] = (signed char)flag[a];
-        D.act_flag[P.hb0 + a] = flag[a];
-        if (flag[a] == FLAG_LIVE) {
-            const int bd[6] = {ra.y, ra.z, ra.w, rb.x, rb.y, rb.z};
-            const R *fr = (const R *)D.hb_frc + 18 * ((size_t)kActMax * r + a);
-            V3<R> fb[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) fb[q] = V3<R>{L::ld(fr + 3 * q), L::ld(fr + 3 * q + 1), L::ld(fr + 3 * q + 2)};
-            const double e = E[a];
-            if (D.hb_dode[g]) e3 += e; else e2 += e;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) atomic_add_force(D.f_mb, bd[q], fb[q]);
-        }
+    stamp(5);
+    if (trc) {
+        trc[8] = (unsigned long long)sh_iters;
+        trc[9] = (unsigned long long)nA0 | (unsigned long long)ncomp << 16 | (unsigned long long)n_act << 32;
     }
+    // ---- energy sums (E2_HB, E3_HB): group 0 adds every live candidate's
+    // energy, every group subtracts those it switched off (their forces were
+    // taken back by the wavefront that ran their component)
     {
+        double e2 = 0.0, e3 = 0.0;
+        for (int a = tid; a < n_act; a += nt) {
+            const unsigned char f = flag[a];
+            if (f == FLAG_GATED) continue;
+            const double e = E[a];
+            double v = grp == 0 ? e : 0.0;
+            if (f >= FLAG_OFF) v -= e;
+            if (dcls[a]) e3 += v; else e2 += v;
+        }
         __shared__ double red[kResWaves];   // one per wavefront of the block
         const double s2 = block_sum_double(e2, red);
         const double s3 = block_sum_double(e3, red);
@@ -1465,9 +1602,10 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
             D.hb_e_part[2 * (kMaxHbGroups * r + grp) + 1] = s3;
         }
     }
+    stamp(6);
     // first resolution on a table switched by the step's close: candidates outside it
-    // (inactive in the rebuild that made it) report -1; the listed ones were
-    // written above
+    // (inactive in the rebuild that made it) report -1; k_hbenergy wrote the
+    // listed ones
     if (L::ld(&S.tab_fresh)) {
         const int f0 = D.rng == 1 ? 0 : grp, fs = D.rng == 1 ? 1 : ngrp;
         for (int k = f0 * nt + tid; k < P.nhb; k += fs * nt)
@@ -1478,6 +1616,7 @@ TIS_DEV void hbres_group(const Dev &D, int mode, int r, int grp, int ngrp, unsig
         if (D.rng == 1) S.hb_draws = sh_draws;
         if (sh_err) atomicOr(&S.overflow, D.rng ? 4 : 16);
     }
+    stamp(7);
     // ---- reference-stream noise table: 1500 Box-Muller pairs after the HB draws
     // (full_forces, fast.cpp:2598-2611).  r1 == 0 is redrawn in the reference; when
     // no r1 slot of this step's block holds a 0 (all but ~1e-6 of steps) the pairs

@@ -164,6 +164,8 @@ def load_library(path=LIB_PATH):
         lib.tis_set_step_counter.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
         lib.tis_set_rng_epoch.argtypes = [C.c_void_p, C.c_int, C.c_uint]
         lib.tis_prepare_graph.argtypes = [C.c_void_p, C.c_int]
+        lib.tis_trace_hb.argtypes = [C.c_void_p, C.c_int]
+        lib.tis_get_hb_trace.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_longlong]
         lib.tis_get_launch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                               C.POINTER(C.c_longlong)]
         lib.tis_step.argtypes = [C.c_void_p, C.c_longlong]
@@ -386,6 +388,20 @@ class Engine:
         v = [C.c_longlong(0) for _ in range(3)]
         check(self.lib, self.lib.tis_get_launch_counts(self.ctx, *[C.byref(x) for x in v]), "tis_get_launch_counts")
         return {"graph_launches": v[0].value, "graph_steps": v[1].value, "eager_steps": v[2].value}
+
+    # layout of tis_get_hb_trace: (step slot, replica, group, word)
+    HB_TRACE_STEPS, HB_TRACE_GROUPS, HB_TRACE_WORDS = 64, 16, 10
+
+    def trace_hb(self, on):
+        check(self.lib, self.lib.tis_trace_hb(self.ctx, 1 if on else 0), "tis_trace_hb")
+
+    def hb_trace(self):
+        """The resolution's phase ring as a uint64 array (64, n_rep, 16, 10)."""
+        shape = (self.HB_TRACE_STEPS, len(self.handles), self.HB_TRACE_GROUPS, self.HB_TRACE_WORDS)
+        out = np.zeros(shape, dtype=np.uint64)
+        check(self.lib, self.lib.tis_get_hb_trace(self.ctx, out.ctypes.data_as(C.POINTER(C.c_ulonglong)), out.size),
+              "tis_get_hb_trace")
+        return out
 
     def set_persistent(self, on):
         check(self.lib, self.lib.tis_set_persistent(self.ctx, 1 if on else 0), "tis_set_persistent")

@@ -294,6 +294,7 @@ def run_chain(args, lib):
         _, avg_ms = kt[dominant]
         bytes_launch = algorithmic_bytes(eng, [h], dominant)
         achieved = None if bytes_launch is None else bytes_launch / (avg_ms * 1e-3) / 1e9
+    hb_trace = hb_trace_summary(eng, args.hb_trace) if args.hb_trace > 0 else None
     e = eng.energies()
     b_step = step_bytes(eng, [h])
     eng.close()
@@ -336,6 +337,7 @@ def run_chain(args, lib):
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": e["n_hb_active"],
         "rebuilds": e["n_rebuilds"],
+        "hb_trace": hb_trace,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -540,6 +542,55 @@ def timed_run(lib, handles, launch, args, **engine_kw):
     return eng, dt, mode
 
 
+HB_PHASES = ("tables", "excess", "components", "roots", "resolution", "outcome_forces", "end")
+
+
+def hb_trace_summary(eng, rounds):
+    """Phase clock of the HB valence resolution (tis_trace_hb) over `rounds` x 64
+    graph-replayed steps: per phase of a (replica, group) workgroup the mean, p50,
+    p90 and max duration (us), the same for the group's whole span, the spread of
+    the groups' start times within a step, the dispatch span (first start to last
+    end of the step's resolution), the phases of the group that ends last, and the
+    sizes (over-subscribed sites, components, live candidates, serial depth)."""
+    eng.trace_hb(True)
+    eng.prepare_graph(64)
+    tr = []
+    for _ in range(rounds):
+        eng.step(64)
+        tr.append(eng.hb_trace().astype(np.int64))
+    eng.trace_hb(False)
+    tr = np.concatenate(tr)[:, :, :eng.opt.hb_groups, :]          # (steps, replicas, groups, words)
+    st = tr[..., :8]
+    us = 0.01                                                        # 100 MHz clock
+    ok = (st[..., 0] > 0) & (st[..., 7] >= st[..., 0])
+
+    def stats(v):
+        v = np.asarray(v, dtype=np.float64)
+        return {"mean": float(v.mean()), "p50": float(np.percentile(v, 50)),
+                "p90": float(np.percentile(v, 90)), "max": float(v.max())}
+
+    d = np.diff(st, axis=-1) * us
+    out = {"steps": int(tr.shape[0]), "workgroups_per_step": int(tr.shape[1] * tr.shape[2])}
+    out["phase_us"] = {name: stats(d[..., k][ok]) for k, name in enumerate(HB_PHASES)}
+    out["group_span_us"] = stats(((st[..., 7] - st[..., 0]) * us)[ok])
+    t0 = np.where(ok, st[..., 0], np.iinfo(np.int64).max).reshape(tr.shape[0], -1).min(axis=1)
+    t1 = np.where(ok, st[..., 7], 0).reshape(tr.shape[0], -1).max(axis=1)
+    out["dispatch_span_us"] = stats((t1 - t0) * us)
+    out["start_spread_us"] = stats(((st[..., 0] - t0[:, None, None]) * us)[ok])
+    flat = st.reshape(tr.shape[0], -1, 8)
+    last = np.argmax(np.where(ok.reshape(tr.shape[0], -1), flat[..., 7], 0), axis=1)
+    crit = flat[np.arange(tr.shape[0]), last]
+    out["last_group_phase_us"] = {name: float((np.diff(crit, axis=-1)[:, k] * us).mean())
+                                  for k, name in enumerate(HB_PHASES)}
+    out["last_group_start_us"] = float(((crit[:, 0] - t0) * us).mean())
+    sz = tr[..., 9]
+    out["serial_depth"] = stats(tr[..., 8][ok])
+    out["over_subscribed_sites"] = stats((sz & 0xffff)[ok])
+    out["components"] = stats(((sz >> 16) & 0xffff)[ok])
+    out["live_candidates"] = stats((sz >> 32)[ok])
+    return out
+
+
 def tune(lib, handles, args):
     """Launch shapes for the timed run: tis_autotune's choice on a scratch copy of
     the batch (its trials are outside the timed region), or the defaults."""
@@ -578,6 +629,9 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=400, help="steps per autotuner trial")
     ap.add_argument("--launch", type=json.loads, default=None, metavar="JSON",
                     help="launch shapes to use instead of tis_autotune's (e.g. for profiler runs)")
+    ap.add_argument("--hb-trace", type=int, default=0, metavar="ROUNDS",
+                    help="after the timed run, ROUNDS x 64 steps with the HB resolution's phase clock on "
+                         "(hb_trace in the line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -619,6 +673,7 @@ def main():
         n_launch, avg_ms = kt[dominant]
         bytes_launch = algorithmic_bytes(eng, handles, dominant)
         achieved = None if bytes_launch is None else bytes_launch / (avg_ms * 1e-3) / 1e9
+    hb_trace = hb_trace_summary(eng, args.hb_trace) if args.hb_trace > 0 else None
     energies = [eng.energies(r) for r in range(n_rep)]
     b_step = step_bytes(eng, handles)
     eng.close()
@@ -680,6 +735,7 @@ def main():
         "valu_issue_frac": None if valu is None else {k: v["valu_frac"] for k, v in valu.items()},
         "kernel_avg_ms": {k: v[1] for k, v in kt.items()},
         "hb_active": [e["n_hb_active"] for e in energies],
+        "hb_trace": hb_trace,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

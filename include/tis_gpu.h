@@ -302,6 +302,19 @@ int tis_get_persistent_counts(tis_ctx *ctx, int *active, long long *launches, lo
 int tis_get_launch_counts(tis_ctx *ctx, long long *graph_launches, long long *graph_steps,
                           long long *eager_steps);
 
+/* Phase trace of the HB valence resolution (profiling; no counterpart in the
+ * reference, whose only timing code is commented out, fast.cpp:6520-6541).
+ * on = 1: every later resolution writes, per (step slot klok % 64, replica,
+ * group), 10 words: the device real-time clock (100 MHz ticks) at the group's
+ * start and at the end of its phases (tables in LDS, excess ranked, components
+ * joined, roots ranked, resolution, outcome and forces, end), the serial depth
+ * (passes of its busiest wavefront), then n_over | n_comp << 16 | n_live << 32
+ * (over-subscribed sites, components, live candidates).  Results are unchanged.
+ * tis_get_hb_trace copies the ring, 64 * n_rep * 16 * 10 words (groups beyond
+ * hb_groups stay 0), into out (n words available). */
+int tis_trace_hb(tis_ctx *ctx, int on);
+int tis_get_hb_trace(tis_ctx *ctx, unsigned long long *out, long long n);
+
 /* Replica exchange (temperature ladder).  No counterpart in the reference,
  * whose replicas are separate MPI jobs; this is the on-device swap of the
  * batched configuration.  Neighbouring replicas r and r + 1 exchange when they
