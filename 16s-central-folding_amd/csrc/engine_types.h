@@ -114,6 +114,9 @@ struct Dev {
     int *excl;                    // kExcl bonded / 1-3 partners per particle (global, -1 none)
     unsigned char *bsub;          // per bonded term: subtract its pair's WCA in the step path too
     int *nb_start, *nb_nsr, *nb_nlr;
+    int4 *nb_meta;                // by cell-order index: pool offset of the particle's slot / 16 (slots
+                                  // are multiples of 16 entries), SR and LR counts, replica (the pair
+                                  // gather loads it beside cpos)
     int *pool;                    // neighbour entries: cell-order indices (~q for bonded / 1-3)
     // topology
     int2 *bond_ij;

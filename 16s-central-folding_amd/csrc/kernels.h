@@ -412,6 +412,8 @@ TIS_DEV void list_replica(const Dev &D, int mode, int r, int w0, int nw) {
                     D.nb_start[p] = (int)((p - P.p0) * stride);
                     D.nb_nsr[p] = min(a, P.cap_sr);
                     D.nb_nlr[p] = min(b, P.cap_lr);
+                    D.nb_meta[P.p0 + t] = make_int4((int)((P.pool0 + (p - P.p0) * stride) >> 4), min(a, P.cap_sr),
+                                                    min(b, P.cap_lr), r);
                 }
             }
         }
@@ -420,8 +422,16 @@ TIS_DEV void list_replica(const Dev &D, int mode, int r, int w0, int nw) {
 __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
     const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int r = 0; r < D.n_rep; ++r)
-        if (D.rs[r].need_rebuild) list_replica<LdPlain>(D, mode, r, w0, nw);
+    // the rebuild flags of 64 replicas at a time, one per lane, loaded at once
+    const int lane = threadIdx.x & 63;
+    for (int r0 = 0; r0 < D.n_rep; r0 += 64) {
+        unsigned long long due = __ballot(r0 + lane < D.n_rep && D.rs[r0 + lane].need_rebuild);
+        while (due) {
+            const int r = r0 + __builtin_ctzll(due);
+            due &= due - 1ull;
+            list_replica<LdPlain>(D, mode, r, w0, nw);
+        }
+    }
 }
 
 // ------------------------------------------------------------ HB rebuild
@@ -1906,6 +1916,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Dev D, int nb_bonded, unsigned
     pair_gather<R, LdPlain>(D, t, threadIdx.x % lanes, lanes, t < D.n_tot, mask, eval);
 }
 
+constexpr int kGatherBatch = 2;   // list entries per lane whose loads are issued together
 // Pair forces of the particle at cell-order index t (valid: t names a particle;
 // every lane of the wavefront takes part, as the lanes of a particle's group
 // exchange partial sums), lane sub of its group of `lanes`.
@@ -1917,15 +1928,16 @@ TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, un
     // own and neighbour positions from the cell-ordered copy (kept current by
     // the integrator): neighbours in the same few cells share cache lines
     const double4 xi = L::ld(D.cpos + (valid ? t : 0));
+    const int4 mt = L::ld(D.nb_meta + (valid ? t : 0));   // slot, counts and replica, beside the position
     const int p = cp_index(xi.w);
-    const int r = D.prep[p];
+    const int r = mt.w;
     const RepParams &P = D.rp[r];
     RepState &S = D.rs[r];
     const int pti = cp_type(xi.w);
     const int ti = pti & 15;
     const bool chg_i = (pti >> 8) & 1;
-    const int *nb = D.pool + P.pool0 + L::ld(D.nb_start + p);
-    const int nsr = valid ? L::ld(D.nb_nsr + p) : 0, nlr = valid ? L::ld(D.nb_nlr + p) : 0;
+    const int *nb = D.pool + ((size_t)mt.x << 4);
+    const int nsr = valid ? mt.y : 0, nlr = valid ? mt.z : 0;
     auto group_sum = [lanes](double v) {
         for (int o = 1; o < lanes; o <<= 1) v += __shfl_xor(v, o, 64);
         return v;
@@ -1933,27 +1945,45 @@ TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, un
     if (mask & 8) {
         R fx = 0, fy = 0, fz = 0;
         double ew = 0, ec = 0;
-        for (int q = sub; q < nsr; q += lanes) {
-            const int e = L::ld(nb + q);
-            const bool bonded = e < 0;
-            const double4 xj = L::ld(D.cpos + (bonded ? ~e : e));
-            double3 dd;
-            const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
-            const int ptj = cp_type(xj.w);
-            const int ab = ti * kNT + (ptj & 15);
-            // a bonded / 1-3 pair carries no net WCA force (the reference adds it
-            // here and removes it in the bonded terms); only evaluations, which
-            // report the reference's split by term, compute it
-            R f = (bonded && !split) ? (R)0 : wca_factor<R>(r2, ab, P);
-            const bool both = chg_i && ((ptj >> 8) & 1) && r2 <= kCoulCut2;
-            if (both) {
-                const R rr = sqrt((R)r2);
-                f += (R)P.qlb[ab] / (rr * (R)r2);
+        // kGatherBatch entries of this lane at a time: their indices, then their
+        // positions, are loaded together (two load levels per batch, not per entry)
+        for (int q0 = sub; q0 < nsr; q0 += kGatherBatch * lanes) {
+            int eb[kGatherBatch];
+            double4 xb[kGatherBatch];
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                const int q = q0 + k * lanes;
+                eb[k] = q < nsr ? L::ld(nb + q) : 0;
             }
-            fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
-            if (want_e) {
-                ew += wca_energy(r2, ab, P);
-                if (both) ec += P.qlb[ab] / sqrt(r2);
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                const int q = q0 + k * lanes;
+                xb[k] = q < nsr ? L::ld(D.cpos + (eb[k] < 0 ? ~eb[k] : eb[k])) : xi;
+            }
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                if (q0 + k * lanes >= nsr) break;
+                const int e = eb[k];
+                const bool bonded = e < 0;
+                const double4 xj = xb[k];
+                double3 dd;
+                const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
+                const int ptj = cp_type(xj.w);
+                const int ab = ti * kNT + (ptj & 15);
+                // a bonded / 1-3 pair carries no net WCA force (the reference adds it
+                // here and removes it in the bonded terms); only evaluations, which
+                // report the reference's split by term, compute it
+                R f = (bonded && !split) ? (R)0 : wca_factor<R>(r2, ab, P);
+                const bool both = chg_i && ((ptj >> 8) & 1) && r2 <= kCoulCut2;
+                if (both) {
+                    const R rr = sqrt((R)r2);
+                    f += (R)P.qlb[ab] / (rr * (R)r2);
+                }
+                fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
+                if (want_e) {
+                    ew += wca_energy(r2, ab, P);
+                    if (both) ec += P.qlb[ab] / sqrt(r2);
+                }
             }
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
@@ -1975,16 +2005,32 @@ TIS_DEV void pair_gather(const Dev &D, int t, int sub, int lanes, bool valid, un
         double ec = 0;
         const int n_lr = lr ? nlr : 0;
         const int *nb_lr = nb + P.cap_sr;
-        for (int q = sub; q < n_lr; q += lanes) {
-            const double4 xj = L::ld(D.cpos + L::ld(nb_lr + q));
-            double3 dd;
-            const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
-            if (r2 > kCoulCut2) continue;
-            const int ab = ti * kNT + (cp_type(xj.w) & 15);
-            const R rr = sqrt((R)r2);
-            const R f = (R)P.qlb[ab] / (rr * (R)r2);
-            fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
-            if (want_e) ec += P.qlb[ab] / sqrt(r2);
+        for (int q0 = sub; q0 < n_lr; q0 += kGatherBatch * lanes) {
+            int eb[kGatherBatch];
+            double4 xb[kGatherBatch];
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                const int q = q0 + k * lanes;
+                eb[k] = q < n_lr ? L::ld(nb_lr + q) : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                const int q = q0 + k * lanes;
+                xb[k] = q < n_lr ? L::ld(D.cpos + eb[k]) : xi;
+            }
+#pragma unroll
+            for (int k = 0; k < kGatherBatch; ++k) {
+                if (q0 + k * lanes >= n_lr) break;
+                const double4 xj = xb[k];
+                double3 dd;
+                const double r2 = min_image_r2(xi, xj, P.L, P.hL, &dd);
+                if (r2 > kCoulCut2) continue;
+                const int ab = ti * kNT + (cp_type(xj.w) & 15);
+                const R rr = sqrt((R)r2);
+                const R f = (R)P.qlb[ab] / (rr * (R)r2);
+                fx += (R)dd.x * f; fy += (R)dd.y * f; fz += (R)dd.z * f;
+                if (want_e) ec += P.qlb[ab] / sqrt(r2);
+            }
         }
         const double gx = group_sum((double)fx), gy = group_sum((double)fy), gz = group_sum((double)fz);
         if (lr && valid && sub == 0 && write_f) D.f_lr[p] = make_double4(gx, gy, gz, 0.0);

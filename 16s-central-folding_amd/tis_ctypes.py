@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtis_mi355x.so")
+# the engine library; TIS_LIB names another build of it
+LIB_PATH = os.environ.get("TIS_LIB", os.path.join(HERE, "libtis_mi355x.so"))
 DATA_16SCD = os.path.join(HERE, "data", "16SCD")
 
 NTYPES = 9
