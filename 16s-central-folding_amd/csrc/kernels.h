@@ -197,30 +197,50 @@ TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
 
 // ------------------------------------------------------------ k_bin
 // Counting sort of replica r's particles into cells by one workgroup (any
-// multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20.
+// multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20, ord
+// ord_cap 16-bit entries (the cell order, when the replica has at most ord_cap
+// particles).
+constexpr int kBinBatch = 4;
+constexpr int kBinOrd = 49152;   // LDS cell-order entries of k_bin
 template <class L>
-TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch) {
+TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned short *ord, int ord_cap) {
     RepState &S = D.rs[r];
     const RepParams &P = D.rp[r];
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2], nc = M0 * M1 * M2;
+    // the replica's box and cell sizes in registers: the stores below would
+    // otherwise make the compiler load them again for every particle
+    const double bL[3] = {P.L[0], P.L[1], P.L[2]}, blc[3] = {P.lc[0], P.lc[1], P.lc[2]};
+    const int bM[3] = {M0, M1, M2};
+    const int p0 = P.p0, n = P.n, cell0 = P.cell0;
     for (int c = threadIdx.x; c < nc; c += blockDim.x) cnt[c] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
-        const int p = P.p0 + i;
-        const double4 x = L::ld(D.pos + p);
-        const double xs[3] = {x.x, x.y, x.z};
-        int ci[3];
+    // kBinBatch particles per thread at a time, their loads issued together
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBinBatch * blockDim.x) {
+        double4 xb[kBinBatch];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            double t = xs[a];
-            if (t < 0) t += P.L[a];
-            else if (t > P.L[a]) t -= P.L[a];
-            int c = (int)ceil(t / P.lc[a]) - 1;
-            ci[a] = c < 0 ? 0 : (c >= P.M[a] ? P.M[a] - 1 : c);
+        for (int k = 0; k < kBinBatch; ++k) {
+            const int i = i0 + k * blockDim.x;
+            xb[k] = i < n ? L::ld(D.pos + p0 + i) : make_double4(0, 0, 0, 0);
         }
-        const int cell = (ci[0] * M1 + ci[1]) * M2 + ci[2];
-        D.cell_of[p] = cell;
-        D.cell_rank[p] = atomicAdd(&cnt[cell], 1);
+#pragma unroll
+        for (int k = 0; k < kBinBatch; ++k) {
+            const int i = i0 + k * blockDim.x;
+            if (i >= n) break;
+            const int p = p0 + i;
+            const double xs[3] = {xb[k].x, xb[k].y, xb[k].z};
+            int ci[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                double t = xs[a];
+                if (t < 0) t += bL[a];
+                else if (t > bL[a]) t -= bL[a];
+                int c = (int)ceil(t / blc[a]) - 1;
+                ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
+            }
+            const int cell = (ci[0] * M1 + ci[1]) * M2 + ci[2];
+            D.cell_of[p] = cell;
+            D.cell_rank[p] = atomicAdd(&cnt[cell], 1);
+        }
     }
     __syncthreads();
     // exclusive scan of the cell counts: contiguous chunk per thread, then block scan
@@ -233,18 +253,79 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch) {
     for (int c = c0; c < c0 + per && c < nc; ++c) {
         int v = cnt[c];
         cnt[c] = run;
-        D.cell_start[P.cell0 + c] = run;
+        D.cell_start[cell0 + c] = run;
         run += v;
     }
-    if (threadIdx.x == 0) D.cell_start[P.cell0 + nc] = total;
+    if (threadIdx.x == 0) D.cell_start[cell0 + nc] = total;
     __syncthreads();
-    for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
-        const int p = P.p0 + i;
-        const int q = P.p0 + cnt[L::ld(D.cell_of + p)] + L::ld(D.cell_rank + p);
-        const double4 x = L::ld(D.pos + p);
-        D.sorted[q] = p;
-        D.crank[p] = q;
-        D.cpos[q] = make_double4(x.x, x.y, x.z, cp_pack(p, D.ptype[p]));
+    if (n <= ord_cap) {
+        // the cell order in LDS first (ord[q - p0] = p - p0), then the cell-ordered
+        // arrays written in cell order: every global store of the sort lands on
+        // consecutive addresses (scattered partial-line stores leave the kernel's
+        // end-of-kernel cache write-back far slower)
+        for (int i0 = threadIdx.x; i0 < n; i0 += kBinBatch * blockDim.x) {
+            int cb[kBinBatch], rb[kBinBatch];
+#pragma unroll
+            for (int k = 0; k < kBinBatch; ++k) {
+                const int i = i0 + k * blockDim.x, p = p0 + (i < n ? i : 0);
+                cb[k] = L::ld(D.cell_of + p);
+                rb[k] = L::ld(D.cell_rank + p);
+            }
+#pragma unroll
+            for (int k = 0; k < kBinBatch; ++k) {
+                const int i = i0 + k * blockDim.x;
+                if (i >= n) break;
+                const int ql = cnt[cb[k]] + rb[k];
+                ord[ql] = (unsigned short)i;
+                D.crank[p0 + i] = p0 + ql;
+            }
+        }
+        __syncthreads();
+        for (int q0 = threadIdx.x; q0 < n; q0 += kBinBatch * blockDim.x) {
+            int pb[kBinBatch], tb[kBinBatch];
+            double4 xb[kBinBatch];
+#pragma unroll
+            for (int k = 0; k < kBinBatch; ++k) {
+                const int ql = q0 + k * blockDim.x;
+                pb[k] = p0 + (ql < n ? (int)ord[ql] : 0);
+                tb[k] = D.ptype[pb[k]];
+                xb[k] = L::ld(D.pos + pb[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kBinBatch; ++k) {
+                const int ql = q0 + k * blockDim.x;
+                if (ql >= n) break;
+                D.sorted[p0 + ql] = pb[k];
+                D.cpos[p0 + ql] = make_double4(xb[k].x, xb[k].y, xb[k].z, cp_pack(pb[k], tb[k]));
+            }
+        }
+        if (threadIdx.x == 0) {
+            S.max_sr = 0;
+            S.max_lr = 0;
+        }
+        return;
+    }
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBinBatch * blockDim.x) {
+        int cb[kBinBatch], rb[kBinBatch], tb[kBinBatch];
+        double4 xb[kBinBatch];
+#pragma unroll
+        for (int k = 0; k < kBinBatch; ++k) {
+            const int i = i0 + k * blockDim.x, p = p0 + (i < n ? i : 0);
+            cb[k] = L::ld(D.cell_of + p);
+            rb[k] = L::ld(D.cell_rank + p);
+            tb[k] = D.ptype[p];
+            xb[k] = L::ld(D.pos + p);
+        }
+#pragma unroll
+        for (int k = 0; k < kBinBatch; ++k) {
+            const int i = i0 + k * blockDim.x;
+            if (i >= n) break;
+            const int p = p0 + i;
+            const int q = p0 + cnt[cb[k]] + rb[k];
+            D.sorted[q] = p;
+            D.crank[p] = q;
+            D.cpos[q] = make_double4(xb[k].x, xb[k].y, xb[k].z, cp_pack(p, tb[k]));
+        }
     }
     if (threadIdx.x == 0) {
         S.max_sr = 0;
@@ -256,7 +337,8 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
     if (!D.rs[r].need_rebuild) return;
     __shared__ int cnt[kMaxCells];
     __shared__ int scratch[20];
-    bin_replica<LdPlain>(D, r, cnt, scratch);
+    __shared__ unsigned short ord[kBinOrd];
+    bin_replica<LdPlain>(D, r, cnt, scratch, ord, kBinOrd);
 }
 
 // ------------------------------------------------------------ neighbour cells
@@ -313,8 +395,12 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ns
 // as one sequence (cell c covers positions [excl_c, incl_c) of it, prefix sums
 // over the lanes that hold the cells), so every chunk of 64 lanes is full
 // whatever the cell occupancies; a lane finds its cell by a scalar walk over the
-// few cells that overlap its chunk.  Two chunks are in flight: the load of the
-// next chunk is issued before the current one is tested.
+// few cells that overlap its chunk.  Four chunks are in flight: the load of a
+// chunk is issued three chunks before it is tested (the loop is unrolled four
+// times so that each chunk keeps its registers until it is tested).  Every
+// lane loads in every chunk (a lane without a candidate reads the replica's
+// first entry and is marked by q = -1), so that the loads form a fixed sequence
+// the compiler can count in flight instead of waiting for all of them.
 template <class L, class F>
 TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
     const int lane = threadIdx.x & 63;
@@ -323,34 +409,43 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
     const int incl = wave_incl_scan(clen);
     const int total = readlane_i(incl, 63);
     if (total == 0) return;
-    const double4 *cp = D.cpos + P.p0;
+    const int p0 = P.p0;
+    const double4 *cp = D.cpos + p0;
     int c = 0;   // first cell whose range ends after the chunk start (uniform)
-    const double none = cp_pack(-1, 0);
     auto load = [&](int t0, double4 *x, int *q) {
-        while (readlane_i(incl, c) <= t0) ++c;
-        const int t = t0 + lane;
         int addr = -1;
-        for (int cc = c; cc < ncell; ++cc) {
-            const int in = readlane_i(incl, cc), ex = in - readlane_i(clen, cc);
-            if (ex >= t0 + 64) break;
-            if (t >= ex && t < in) addr = readlane_i(cstart, cc) + (t - ex);
+        if (t0 < total) {
+            while (readlane_i(incl, c) <= t0) ++c;
+            const int t = t0 + lane;
+            for (int cc = c; cc < ncell; ++cc) {
+                const int in = readlane_i(incl, cc), ex = in - readlane_i(clen, cc);
+                if (ex >= t0 + 64) break;
+                if (t >= ex && t < in) addr = readlane_i(cstart, cc) + (t - ex);
+            }
         }
-        *x = addr >= 0 ? L::ld(cp + addr) : make_double4(0, 0, 0, none);
-        *q = P.p0 + addr;
+        *x = L::ld(cp + (addr >= 0 ? addr : 0));
+        *q = addr >= 0 ? p0 + addr : -1;
     };
-    double4 cur;
-    int qcur;
-    load(0, &cur, &qcur);
-    for (int t0 = 0;;) {
-        t0 += 64;
-        const bool more = t0 < total;
-        double4 nxt = make_double4(0, 0, 0, none);
-        int qnxt = -1;
-        if (more) load(t0, &nxt, &qnxt);
-        visit(cp_index(cur.w), cur, qcur);
-        if (!more) break;
-        cur = nxt;
-        qcur = qnxt;
+    auto run = [&](const double4 &x, int q) { visit(q >= 0 ? cp_index(x.w) : -1, x, q); };
+    double4 x0, x1, x2, x3;
+    int q0, q1, q2, q3;
+    load(0, &x0, &q0);
+    load(64, &x1, &q1);
+    load(128, &x2, &q2);
+    load(192, &x3, &q3);
+    for (int t0 = 0;; t0 += 256) {
+        run(x0, q0);
+        if (t0 + 64 >= total) break;
+        load(t0 + 256, &x0, &q0);
+        run(x1, q1);
+        if (t0 + 128 >= total) break;
+        load(t0 + 320, &x1, &q1);
+        run(x2, q2);
+        if (t0 + 192 >= total) break;
+        load(t0 + 384, &x2, &q2);
+        run(x3, q3);
+        if (t0 + 256 >= total) break;
+        load(t0 + 448, &x3, &q3);
     }
 }
 
@@ -360,18 +455,27 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
 // then cap_lr LR entries), so there is no counting pass and no shared cursor.
 // The grid is fixed (kListWaves wavefronts) and strides over the particles of
 // the replicas whose displacement trigger fired; the others cost one flag read.
-// mode 0: record the largest per-particle counts only (sizes the slots), mode 1: build.
+// MODE 0: record the largest per-particle counts only (sizes the slots), MODE 1:
+// build.  The list stores are made by every lane in every chunk, a lane without
+// an entry writing to a scratch word of D.list_sink (no branch around them, so
+// that the chunk loads stay in flight across them).
 constexpr int kListWaves = 4096;
 // Lists of replica r's particles, wavefront w0 of nw taking particles w0, w0 + nw, ...
-template <class L>
-TIS_DEV void list_replica(const Dev &D, int mode, int r, int w0, int nw) {
+template <class L, int MODE>
+TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
     const int lane = threadIdx.x & 63;
     {
         RepState &S = D.rs[r];
         const RepParams &P = D.rp[r];
-        const long long stride = (long long)P.cap_sr + P.cap_lr;
-        for (int t = w0; t < P.n; t += nw) {
-            const double4 xi = L::ld(D.cpos + P.p0 + t);
+        // the replica's box, radii and slot sizes in registers: the list stores
+        // below would otherwise make the compiler load them again for every chunk
+        const double bL[3] = {P.L[0], P.L[1], P.L[2]}, bhL[3] = {P.hL[0], P.hL[1], P.hL[2]};
+        const double rsr2 = P.rsr2, rlr2 = P.rlr2;
+        const int cap_sr = P.cap_sr, cap_lr = P.cap_lr, p0 = P.p0, n = P.n;
+        const long long pool0 = P.pool0;
+        const long long stride = (long long)cap_sr + cap_lr;
+        for (int t = w0; t < n; t += nw) {
+            const double4 xi = L::ld(D.cpos + p0 + t);
             const int p = cp_index(xi.w);
             const bool chg_i = (cp_type(xi.w) >> 8) & 1;
             const int cell = L::ld(D.cell_of + p);
@@ -380,40 +484,41 @@ TIS_DEV void list_replica(const Dev &D, int mode, int r, int w0, int nw) {
             int ex[kExcl];
 #pragma unroll
             for (int e = 0; e < kExcl; ++e) ex[e] = D.excl[kExcl * p + e];
-            int *out_sr = D.pool + P.pool0 + (long long)(p - P.p0) * stride;
-            int *out_lr = out_sr + P.cap_sr;
+            int *out_sr = D.pool + pool0 + (long long)(p - p0) * stride;
+            int *out_lr = out_sr + cap_sr;
+            int *sink = D.list_sink;
             int a = 0, b = 0;
             for_neighbour_cells_wave<L>(D, P, cell, [&](int j, const double4 &xj, int qj) {
-                bool isr = false, ilr = false;
-                if (j >= 0 && j != p) {
-                    const double r2 = min_image_r2(xi, xj, P.L, P.hL);
-                    isr = r2 <= P.rsr2;
-                    ilr = !isr && chg_i && r2 <= P.rlr2 && ((cp_type(xj.w) >> 8) & 1);
-                }
+                const double r2 = min_image_r2(xi, xj, bL, bhL);
+                const bool cand = j >= 0 && j != p;
+                const bool isr = cand && r2 <= rsr2;
+                const bool ilr = cand && !isr && chg_i && r2 <= rlr2 && ((cp_type(xj.w) >> 8) & 1);
                 const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
-                if (mode == 1) {
+                if (MODE == 1) {
                     const unsigned long long below = (1ull << lane) - 1ull;
                     bool is_ex = false;
 #pragma unroll
                     for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
                     const int ka = a + __popcll(ms & below), kb = b + __popcll(ml & below);
-                    if (isr && ka < P.cap_sr) out_sr[ka] = is_ex ? ~qj : qj;
-                    if (ilr && kb < P.cap_lr) out_lr[kb] = qj;
+                    int *ds = isr && ka < cap_sr ? out_sr + ka : sink + lane;
+                    int *dl = ilr && kb < cap_lr ? out_lr + kb : sink + 64 + lane;
+                    *ds = is_ex ? ~qj : qj;
+                    *dl = qj;
                 }
                 a += __popcll(ms);
                 b += __popcll(ml);
             });
             if (lane == 0) {
-                if (mode == 0) {
+                if (MODE == 0) {
                     atomicMax(&S.max_sr, a);
                     atomicMax(&S.max_lr, b);
                 } else {
-                    if (a > P.cap_sr || b > P.cap_lr) atomicOr(&S.overflow, 1);
-                    D.nb_start[p] = (int)((p - P.p0) * stride);
-                    D.nb_nsr[p] = min(a, P.cap_sr);
-                    D.nb_nlr[p] = min(b, P.cap_lr);
-                    D.nb_meta[P.p0 + t] = make_int4((int)((P.pool0 + (p - P.p0) * stride) >> 4), min(a, P.cap_sr),
-                                                    min(b, P.cap_lr), r);
+                    if (a > cap_sr || b > cap_lr) atomicOr(&S.overflow, 1);
+                    D.nb_start[p] = (int)((p - p0) * stride);
+                    D.nb_nsr[p] = min(a, cap_sr);
+                    D.nb_nlr[p] = min(b, cap_lr);
+                    D.nb_meta[p0 + t] = make_int4((int)((pool0 + (p - p0) * stride) >> 4), min(a, cap_sr),
+                                                  min(b, cap_lr), r);
                 }
             }
         }
@@ -429,7 +534,8 @@ __global__ void __launch_bounds__(kBlock) k_list(Dev D, int mode) {
         while (due) {
             const int r = r0 + __builtin_ctzll(due);
             due &= due - 1ull;
-            list_replica<LdPlain>(D, mode, r, w0, nw);
+            if (mode == 1) list_replica<LdPlain, 1>(D, r, w0, nw);
+            else list_replica<LdPlain, 0>(D, r, w0, nw);
         }
     }
 }
@@ -2223,7 +2329,7 @@ TIS_DEV void leapfrog(const Dev &D, int r, const RepParams &P, const RepState &S
 // The new state goes to the other buffer set (pos_nx, old_nx, vel_nx; every
 // read of this step's state is from the current set, so no particle's write can
 // reach another's read, the recentring bead's included); the cell-ordered copy
-// the pair gather reads is refreshed (its .w, identity and type, stays) and the
+// the pair gather reads is refreshed (whole entries, identity and type word too) and the
 // multi-body force buffer of the next step cleared.
 // Particle p (valid: p names a particle; every lane of the wavefront takes part
 // in the displacement maximum, which goes to word `slot` of its replica).
@@ -2257,10 +2363,8 @@ TIS_DEV void integrate_one(const Dev &D, int p, bool valid, int slot) {
         D.old_nx[p] = make_double4(on[0], on[1], on[2], 0.0);
         D.vel_nx[p] = make_double4(vn[0], vn[1], vn[2], 0.0);
         if (p - P.p0 < P.nbeads) D.f_mb_nx[p] = make_double4(0.0, 0.0, 0.0, 0.0);
-        double4 *c = D.cpos + L::ld(D.crank + p);
-        c->x = xn[0];
-        c->y = xn[1];
-        c->z = xn[2];
+        // the whole 32-byte entry (identity and type word again), not a partial one
+        D.cpos[L::ld(D.crank + p)] = make_double4(xn[0], xn[1], xn[2], cp_pack(p, D.ptype[p]));
     }
     // largest displacement of the step per replica (d2 >= 0, so its bits order
     // like the values): a wavefront maximum when the wavefront lies in one
