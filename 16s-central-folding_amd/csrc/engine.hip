@@ -727,6 +727,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
                 P.lc[a] = s.box[a] / P.M[a];
             }
             if ((long)P.M[0] * P.M[1] * P.M[2] > kMaxCells) throw Fail{TIS_EINVAL, "box too large for the cell grid"};
+            P.sr_h = (P.lc[0] >= o.r_sr && P.lc[1] >= o.r_sr && P.lc[2] >= o.r_sr) ? 1 : 2;
             P.T = s.T; P.lB = s.lB; P.h = s.h; P.vcap = 300.0 * s.T;
             P.rsr2 = o.r_sr * o.r_sr; P.rlr2 = o.r_lr * o.r_lr; P.skin = o.skin;
             P.lr_every = o.lr_every;

@@ -26,6 +26,8 @@ struct RepParams {
     double rsr2, rlr2, skin;
     int M[3];
     double lc[3];
+    int sr_h;                     // cells searched either side for a short-range-only list (1 when
+                                  // every cell is at least r_sr wide, else 2)
     int cell0;                    // offset into the cell-start array (ncell + 1 entries)
     long long pool0, pool_cap;    // neighbour-pool region: n slots of cap_sr + cap_lr entries
     int cap_sr, cap_lr;           // SR and LR entries per particle slot

@@ -210,6 +210,7 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
     // the replica's box and cell sizes in registers: the stores below would
     // otherwise make the compiler load them again for every particle
     const double bL[3] = {P.L[0], P.L[1], P.L[2]}, blc[3] = {P.lc[0], P.lc[1], P.lc[2]};
+    const double binv[3] = {1.0 / blc[0], 1.0 / blc[1], 1.0 / blc[2]};
     const int bM[3] = {M0, M1, M2};
     const int p0 = P.p0, n = P.n, cell0 = P.cell0;
     for (int c = threadIdx.x; c < nc; c += blockDim.x) cnt[c] = 0;
@@ -234,7 +235,12 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
                 double t = xs[a];
                 if (t < 0) t += bL[a];
                 else if (t > bL[a]) t -= bL[a];
-                int c = (int)ceil(t / blc[a]) - 1;
+                // ceil(t / lc) - 1 (separate_in_cells, fast.cpp:1510-1530): the
+                // quotient from the reciprocal, and from the division itself when it
+                // lies within 1e-9 of an integer (where the two may round apart)
+                double qt = t * binv[a];
+                if (fabs(qt - rint(qt)) < 1e-9 * (1.0 + fabs(qt))) qt = t / blc[a];
+                int c = (int)ceil(qt) - 1;
                 ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
             }
             const int cell = (ci[0] * M1 + ci[1]) * M2 + ci[2];
@@ -344,17 +350,21 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
 // ------------------------------------------------------------ neighbour cells
 // Cells are at least half the long-range list radius wide, so the particles
 // within r_lr of a particle in cell (c0, c1, c2) lie in cells c +- 2 along each
-// axis.  Cells are numbered with z fastest, so the five cells c2 - 2 .. c2 + 2 of
-// one (x, y) column are contiguous in cell order (two pieces where the range
-// wraps the box): the neighbourhood is at most 5 x 5 columns x 2 pieces = 50
-// contiguous ranges of particles, one per lane: lane l < *nseg gets its range's
-// first cell-order index and length (0 for an empty piece).  An axis with fewer
-// than 5 cells contributes each of its cells once.
+// axis (h = 2), and, when every cell is at least r_sr wide, those within r_sr in
+// cells c +- 1 (h = 1: particles that get no long-range list).  Cells are
+// numbered with z fastest, so the 2h + 1 cells c2 - h .. c2 + h of one (x, y)
+// column are contiguous in cell order (two pieces where the range wraps the box):
+// the neighbourhood is at most 5 x 5 columns x 2 pieces = 50 contiguous ranges of
+// particles, one per lane: lane l < *nseg gets its range's first cell-order index
+// and length (0 for an empty piece).  An axis with fewer than 2h + 1 cells
+// contributes each of its cells once.
 template <class L>
-TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *nseg, int *sstart, int *slen) {
+TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, int *nseg, int *sstart,
+                             int *slen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
-    const int wx = M0 < 5 ? M0 : 5, wy = M1 < 5 ? M1 : 5;
+    const int w = 2 * h + 1;
+    const int wx = M0 < w ? M0 : w, wy = M1 < w ? M1 : w;
     *nseg = 2 * wx * wy;
     const int lane = threadIdx.x & 63;
     *sstart = 0;
@@ -363,11 +373,11 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ns
     if (col < wx * wy) {
         const int ix = col / wy, iy = col % wy;
         int n0 = ix, n1 = iy;
-        if (M0 >= 5) { n0 = c0 - 2 + ix; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0; }
-        if (M1 >= 5) { n1 = c1 - 2 + iy; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0; }
+        if (M0 >= w) { n0 = c0 - h + ix; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0; }
+        if (M1 >= w) { n1 = c1 - h + iy; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0; }
         int zlo = 0, zhi = M2 - 1;           // this piece's z range (zhi < zlo: empty)
-        if (M2 >= 5) {
-            const int lo = c2 - 2, hi = c2 + 2;
+        if (M2 >= w) {
+            const int lo = c2 - h, hi = c2 + h;
             if (lo < 0) {
                 zlo = piece ? lo + M2 : 0;
                 zhi = piece ? M2 - 1 : hi;
@@ -402,10 +412,10 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int *ns
 // first entry and is marked by q = -1), so that the loads form a fixed sequence
 // the compiler can count in flight instead of waiting for all of them.
 template <class L, class F>
-TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, F &&visit) {
+TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, int h, F &&visit) {
     const int lane = threadIdx.x & 63;
     int ncell, cstart, clen;
-    neighbour_cells<L>(D, P, cell, &ncell, &cstart, &clen);
+    neighbour_cells<L>(D, P, cell, h, &ncell, &cstart, &clen);
     const int incl = wave_incl_scan(clen);
     const int total = readlane_i(incl, 63);
     if (total == 0) return;
@@ -471,7 +481,7 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
         // below would otherwise make the compiler load them again for every chunk
         const double bL[3] = {P.L[0], P.L[1], P.L[2]}, bhL[3] = {P.hL[0], P.hL[1], P.hL[2]};
         const double rsr2 = P.rsr2, rlr2 = P.rlr2;
-        const int cap_sr = P.cap_sr, cap_lr = P.cap_lr, p0 = P.p0, n = P.n;
+        const int cap_sr = P.cap_sr, cap_lr = P.cap_lr, p0 = P.p0, n = P.n, sr_h = P.sr_h;
         const long long pool0 = P.pool0;
         const long long stride = (long long)cap_sr + cap_lr;
         for (int t = w0; t < n; t += nw) {
@@ -488,7 +498,9 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
             int *out_lr = out_sr + cap_sr;
             int *sink = D.list_sink;
             int a = 0, b = 0;
-            for_neighbour_cells_wave<L>(D, P, cell, [&](int j, const double4 &xj, int qj) {
+            // an uncharged particle gets no long-range list: its short-range
+            // partners lie in the cells next to its own
+            for_neighbour_cells_wave<L>(D, P, cell, chg_i ? 2 : sr_h, [&](int j, const double4 &xj, int qj) {
                 const double r2 = min_image_r2(xi, xj, bL, bhL);
                 const bool cand = j >= 0 && j != p;
                 const bool isr = cand && r2 <= rsr2;
@@ -604,7 +616,12 @@ struct HbSitesLds {
 template <class L>
 TIS_DEV void hbsites_replica(const Dev &D, int lag, int r, HbSitesLds W) {
     RepState &S = D.rs[r];
-    const RepParams &P = D.rp[r];
+    const RepParams &Pg = D.rp[r];
+    // the replica's table offsets in registers: the stores below would otherwise
+    // make the compiler load them again in every loop
+    struct {
+        int hb0, nhb, site0, nsite, inc0;
+    } const P = {Pg.hb0, Pg.nhb, Pg.site0, Pg.nsite, Pg.inc0};
     const int tab = L::ld(&S.tab);
     const HbTab T = hb_tab(D, lag ? tab ^ 1 : tab);
     int *site_map = W.smem, *fill = W.smem + P.nsite;
