@@ -120,6 +120,7 @@ struct tis_ctx {
     bool mb_lds = false;
     int mb_slices = 1;
     size_t mb_lds_bytes = 0;
+    int max_n = 1;                    // particles of the largest replica
     int dev_id = 0;
     std::vector<long long> stream_cap;
     long long stream_total = 0;
@@ -243,8 +244,15 @@ struct tis_ctx {
     // List rebuild of the replicas whose displacement trigger fired (the others
     // exit at once): cell sort and neighbour slots on `sl`, live HB candidates and
     // the resolution's site tables on `sh` (they depend on positions only).
+    // The cell sort of the replicas whose trigger fired, over many workgroups.
+    void launch_bin(hipStream_t sl) {
+        const dim3 g((max_n + 1023) / 1024, n_rep);
+        timed("k_bin_count", sl, [&] { k_bin_count<<<g, 1024, 0, sl>>>(D); });
+        timed("k_bin_scan", sl, [&] { k_bin_scan<<<n_rep, 1024, 0, sl>>>(D); });
+        timed("k_bin_place", sl, [&] { k_bin_place<<<g, 1024, 0, sl>>>(D); });
+    }
     void launch_rebuild_lists(hipStream_t sl) {
-        timed("k_bin", sl, [&] { k_bin<<<n_rep, 1024, 0, sl>>>(D); });
+        launch_bin(sl);
         timed("k_list", sl, [&] { k_list<<<list_grid(), kBlock, 0, sl>>>(D, 1); });
     }
     // lag: inside a step, beside the resolution, into the table buffer it is not
@@ -405,8 +413,9 @@ struct tis_ctx {
             else k_force<float, 1><<<pair_grid(), pair_block, 0, s_main>>>(D, nb_bonded, 31u, kEvalStep);
         });
         HIP_TRY(hipStreamWaitEvent(s_main, ev_join, 0));
-        HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
-        if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));   // the close reads the new tables' counts
+        // the close reads the new tables' counts (recorded after the multi-body terms on their stream)
+        if (D.hb_lag) HIP_TRY(hipStreamWaitEvent(s_main, ev_join_rb, 0));
+        else HIP_TRY(hipStreamWaitEvent(s_main, ev_join_mb, 0));
         timed("k_integrate", s_main, [&] { k_integrate<<<grid(D.n_tot), kBlock, 0, s_main>>>(D); });
         set_parity(par ^ 1);   // the new state is in the other buffers
     }
@@ -718,6 +727,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             RepParams &P = c->rp[r];
             std::memset(&P, 0, sizeof P);
             P.p0 = p0; P.n = s.n_particles; P.nbeads = s.n_beads; P.kh = p0 + s.recentre;
+            c->max_n = std::max(c->max_n, s.n_particles);
             for (int a = 0; a < 3; ++a) {
                 P.L[a] = s.box[a];
                 P.hL[a] = 0.5 * s.box[a];
@@ -767,6 +777,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.cell_of = dalloc<int>(own, N);
         D.cell_rank = dalloc<int>(own, N);
         D.cell_start = dalloc<int>(own, cell0);
+        D.cell_cnt = dalloc<int>(own, cell0);
         D.sorted = dalloc<int>(own, N);
         D.cpos = dalloc<double4>(own, N);
         D.crank = dalloc<int>(own, N);
@@ -1085,7 +1096,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         }
         upload(D.rs, rs);
         HIP_TRY(hipMemcpy(D.rp, c->rp.data(), n_rep * sizeof(RepParams), hipMemcpyHostToDevice));
-        k_bin<<<n_rep, 1024, 0, c->s_main>>>(D);
+        c->launch_bin(c->s_main);
         k_list<<<c->list_grid(), kBlock, 0, c->s_main>>>(D, 0);
         HIP_TRY(hipStreamSynchronize(c->s_main));
         // fixed per-particle slots: twice the largest count at set-up plus margin

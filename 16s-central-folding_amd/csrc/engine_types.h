@@ -110,6 +110,7 @@ struct Dev {
     int *prep;                    // replica of each particle
     // cells and lists
     int *cell_of, *cell_rank, *cell_start, *sorted;
+    int *cell_cnt;                // particles per cell of a rebuild in progress (k_bin_count; cleared by k_bin_scan)
     double4 *cpos;                // positions in cell order (current: k_integrate updates them),
                                   // .w = particle index (low word) and type word (high word)
     int *crank;                   // cell-order index of each particle (last rebuild)

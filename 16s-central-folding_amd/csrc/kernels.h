@@ -200,7 +200,7 @@ TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
 // multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20, ord
 // ord_cap 16-bit entries (the cell order, when the replica has at most ord_cap
 // particles).
-constexpr int kBinBatch = 4;
+constexpr int kBinBatch = 8;
 constexpr int kBinOrd = 49152;   // LDS cell-order entries of k_bin
 template <class L>
 TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned short *ord, int ord_cap) {
@@ -345,6 +345,112 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
     __shared__ int scratch[20];
     __shared__ unsigned short ord[kBinOrd];
     bin_replica<LdPlain>(D, r, cnt, scratch, ord, kBinOrd);
+}
+
+// The same sort in three launches over many workgroups (a replica's particles
+// need not wait on one CU).  k_bin_count, grid (ceil(n_max / 1024), replicas):
+// each workgroup ranks its 1024 particles within their cells in LDS, then
+// reserves each touched cell's share of the replica's count with one global
+// atomic per cell (returning the share's base): rank = base + rank in the share.
+// k_bin_scan, one workgroup per replica: the cell starts (an exclusive scan of
+// the counts, which it clears for the next rebuild).  k_bin_place, grid as
+// k_bin_count: cell-order index = cell start + rank, and the cell-ordered
+// arrays.  Only replicas whose trigger fired do anything.
+TIS_DEV int cell_index(const double x[3], const double bL[3], const double blc[3], const double binv[3],
+                       const int bM[3]) {
+    int ci[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t = x[a];
+        if (t < 0) t += bL[a];
+        else if (t > bL[a]) t -= bL[a];
+        // ceil(t / lc) - 1 (separate_in_cells, fast.cpp:1510-1530), as in bin_replica
+        double qt = t * binv[a];
+        if (fabs(qt - rint(qt)) < 1e-9 * (1.0 + fabs(qt))) qt = t / blc[a];
+        const int c = (int)ceil(qt) - 1;
+        ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
+    }
+    return (ci[0] * bM[1] + ci[1]) * bM[2] + ci[2];
+}
+__global__ void __launch_bounds__(1024) k_bin_count(Dev D) {
+    const int r = blockIdx.y;
+    if (!D.rs[r].need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    const int n = P.n, p0 = P.p0, cell0 = P.cell0;
+    if ((int)(blockIdx.x * 1024) >= n) return;
+    const int bM[3] = {P.M[0], P.M[1], P.M[2]}, nc = bM[0] * bM[1] * bM[2];
+    const double bL[3] = {P.L[0], P.L[1], P.L[2]}, blc[3] = {P.lc[0], P.lc[1], P.lc[2]};
+    const double binv[3] = {1.0 / blc[0], 1.0 / blc[1], 1.0 / blc[2]};
+    __shared__ int lcnt[kMaxCells];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    double4 x = make_double4(0, 0, 0, 0);
+    if (i < n) x = D.pos[p0 + i];
+    for (int c = threadIdx.x; c < nc; c += 1024) lcnt[c] = 0;
+    __syncthreads();
+    int cell = 0, lrank = 0;
+    if (i < n) {
+        const double xs[3] = {x.x, x.y, x.z};
+        cell = cell_index(xs, bL, blc, binv, bM);
+        lrank = atomicAdd(&lcnt[cell], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += 1024) {
+        const int k = lcnt[c];
+        if (k > 0) lcnt[c] = atomicAdd(&D.cell_cnt[cell0 + c], k);
+    }
+    __syncthreads();
+    if (i < n) {
+        D.cell_of[p0 + i] = cell;
+        D.cell_rank[p0 + i] = lcnt[cell] + lrank;
+    }
+}
+__global__ void __launch_bounds__(1024) k_bin_scan(Dev D) {
+    const int r = blockIdx.x;
+    RepState &S = D.rs[r];
+    if (!S.need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    const int nc = P.M[0] * P.M[1] * P.M[2], cell0 = P.cell0;
+    __shared__ int scratch[20];
+    const int per = (nc + 1023) / 1024, c0 = threadIdx.x * per;
+    constexpr int kPer = (kMaxCells + 1023) / 1024;
+    int v[kPer], local = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int c = c0 + k;
+        v[k] = k < per && c < nc ? D.cell_cnt[cell0 + c] : 0;
+        local += v[k];
+    }
+    int total;
+    int run = block_excl_sum(local, scratch, &total);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int c = c0 + k;
+        if (k < per && c < nc) {
+            D.cell_start[cell0 + c] = run;
+            D.cell_cnt[cell0 + c] = 0;   // cleared for the next rebuild
+            run += v[k];
+        }
+    }
+    if (threadIdx.x == 0) {
+        D.cell_start[cell0 + nc] = total;
+        S.max_sr = 0;
+        S.max_lr = 0;
+    }
+}
+__global__ void __launch_bounds__(1024) k_bin_place(Dev D) {
+    const int r = blockIdx.y;
+    if (!D.rs[r].need_rebuild) return;
+    const RepParams &P = D.rp[r];
+    const int n = P.n, p0 = P.p0, cell0 = P.cell0;
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    if (i >= n) return;
+    const int p = p0 + i;
+    const int cell = D.cell_of[p], rank = D.cell_rank[p], pt = D.ptype[p];
+    const double4 x = D.pos[p];
+    const int q = p0 + D.cell_start[cell0 + cell] + rank;
+    D.sorted[q] = p;
+    D.crank[p] = q;
+    D.cpos[q] = make_double4(x.x, x.y, x.z, cp_pack(p, pt));
 }
 
 // ------------------------------------------------------------ neighbour cells
