@@ -785,7 +785,6 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
         D.nb_nsr = dalloc<int>(own, N);
         D.nb_nlr = dalloc<int>(own, N);
         D.nb_meta = dalloc<int4>(own, N);
-        D.list_sink = dalloc<int>(own, 128);
         // ---- topology, converted to global indices
         std::vector<int> ptype(N), prep(N);
         std::vector<double4> pos(N), old(N), vel(N);

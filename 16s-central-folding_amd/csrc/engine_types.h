@@ -121,7 +121,6 @@ struct Dev {
                                   // are multiples of 16 entries), SR and LR counts, replica (the pair
                                   // gather loads it beside cpos)
     int *pool;                    // neighbour entries: cell-order indices (~q for bonded / 1-3)
-    int *list_sink;               // 128 scratch words: list stores of lanes without an entry
     // topology
     int2 *bond_ij;
     double2 *bond_kr;

@@ -572,9 +572,8 @@ TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell
 // The grid is fixed (kListWaves wavefronts) and strides over the particles of
 // the replicas whose displacement trigger fired; the others cost one flag read.
 // MODE 0: record the largest per-particle counts only (sizes the slots), MODE 1:
-// build.  The list stores are made by every lane in every chunk, a lane without
-// an entry writing to a scratch word of D.list_sink (no branch around them, so
-// that the chunk loads stay in flight across them).
+// build.  A chunk's stores sit behind a wave-uniform test of its ballot (most
+// chunks hold no short-range partner, and uncharged particles no long-range one).
 constexpr int kListWaves = 4096;
 // Lists of replica r's particles, wavefront w0 of nw taking particles w0, w0 + nw, ...
 template <class L, int MODE>
@@ -602,7 +601,6 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
             for (int e = 0; e < kExcl; ++e) ex[e] = D.excl[kExcl * p + e];
             int *out_sr = D.pool + pool0 + (long long)(p - p0) * stride;
             int *out_lr = out_sr + cap_sr;
-            int *sink = D.list_sink;
             int a = 0, b = 0;
             // an uncharged particle gets no long-range list: its short-range
             // partners lie in the cells next to its own
@@ -614,14 +612,17 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
                 const unsigned long long ms = __ballot(isr), ml = __ballot(ilr);
                 if (MODE == 1) {
                     const unsigned long long below = (1ull << lane) - 1ull;
-                    bool is_ex = false;
+                    if (ms) {   // most chunks hold no short-range partner
+                        bool is_ex = false;
 #pragma unroll
-                    for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
-                    const int ka = a + __popcll(ms & below), kb = b + __popcll(ml & below);
-                    int *ds = isr && ka < cap_sr ? out_sr + ka : sink + lane;
-                    int *dl = ilr && kb < cap_lr ? out_lr + kb : sink + 64 + lane;
-                    *ds = is_ex ? ~qj : qj;
-                    *dl = qj;
+                        for (int e = 0; e < kExcl; ++e) is_ex |= ex[e] == j;
+                        const int ka = a + __popcll(ms & below);
+                        if (isr && ka < cap_sr) out_sr[ka] = is_ex ? ~qj : qj;
+                    }
+                    if (ml) {
+                        const int kb = b + __popcll(ml & below);
+                        if (ilr && kb < cap_lr) out_lr[kb] = qj;
+                    }
                 }
                 a += __popcll(ms);
                 b += __popcll(ml);
