@@ -61,6 +61,8 @@ struct RepState {
     int tab, tab_pending, tab_fresh;
     int nx_act, nx_asite, nx_ainc;
     int max_sr, max_lr;           // largest per-particle SR / LR counts of the last build
+    int crop_off;                 // set (and kept) once a cell sort met a coordinate outside the box
+                                  // after one wrap: list searches then take whole cells
     unsigned rng_epoch;           // Philox key word: a restart of klok on the same context moves to a
                                   // new epoch, so no draw of an earlier run repeats
     double e[12];                 // energies of the last force evaluation

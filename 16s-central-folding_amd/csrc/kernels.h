@@ -196,6 +196,28 @@ TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
 }
 
 // ------------------------------------------------------------ k_bin
+// The cell of position x (separate_in_cells, fast.cpp:1510-1530): each
+// coordinate wrapped once into the box, cell ceil(t / lc) - 1 clamped to the
+// grid, the quotient from the reciprocal, and from the division itself when it
+// lies within 1e-9 of an integer (where the two may round apart).  *outside is
+// set when a wrapped coordinate still lies outside [0, L] (its particle is then
+// not inside its cell, which the cropped list search relies on).
+TIS_DEV int cell_index(const double x[3], const double bL[3], const double blc[3], const double binv[3],
+                       const int bM[3], bool *outside) {
+    int ci[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t = x[a];
+        if (t < 0) t += bL[a];
+        else if (t > bL[a]) t -= bL[a];
+        *outside |= !(t >= 0.0 && t <= bL[a]);
+        double qt = t * binv[a];
+        if (fabs(qt - rint(qt)) < 1e-9 * (1.0 + fabs(qt))) qt = t / blc[a];
+        const int c = (int)ceil(qt) - 1;
+        ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
+    }
+    return (ci[0] * bM[1] + ci[1]) * bM[2] + ci[2];
+}
 // Counting sort of replica r's particles into cells by one workgroup (any
 // multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20, ord
 // ord_cap 16-bit entries (the cell order, when the replica has at most ord_cap
@@ -229,21 +251,9 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
             if (i >= n) break;
             const int p = p0 + i;
             const double xs[3] = {xb[k].x, xb[k].y, xb[k].z};
-            int ci[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                double t = xs[a];
-                if (t < 0) t += bL[a];
-                else if (t > bL[a]) t -= bL[a];
-                // ceil(t / lc) - 1 (separate_in_cells, fast.cpp:1510-1530): the
-                // quotient from the reciprocal, and from the division itself when it
-                // lies within 1e-9 of an integer (where the two may round apart)
-                double qt = t * binv[a];
-                if (fabs(qt - rint(qt)) < 1e-9 * (1.0 + fabs(qt))) qt = t / blc[a];
-                int c = (int)ceil(qt) - 1;
-                ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
-            }
-            const int cell = (ci[0] * M1 + ci[1]) * M2 + ci[2];
+            bool outside = false;
+            const int cell = cell_index(xs, bL, blc, binv, bM, &outside);
+            if (outside) atomicOr(&S.crop_off, 1);
             D.cell_of[p] = cell;
             D.cell_rank[p] = atomicAdd(&cnt[cell], 1);
         }
@@ -356,22 +366,6 @@ __global__ void __launch_bounds__(1024) k_bin(Dev D) {
 // the counts, which it clears for the next rebuild).  k_bin_place, grid as
 // k_bin_count: cell-order index = cell start + rank, and the cell-ordered
 // arrays.  Only replicas whose trigger fired do anything.
-TIS_DEV int cell_index(const double x[3], const double bL[3], const double blc[3], const double binv[3],
-                       const int bM[3]) {
-    int ci[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double t = x[a];
-        if (t < 0) t += bL[a];
-        else if (t > bL[a]) t -= bL[a];
-        // ceil(t / lc) - 1 (separate_in_cells, fast.cpp:1510-1530), as in bin_replica
-        double qt = t * binv[a];
-        if (fabs(qt - rint(qt)) < 1e-9 * (1.0 + fabs(qt))) qt = t / blc[a];
-        const int c = (int)ceil(qt) - 1;
-        ci[a] = c < 0 ? 0 : (c >= bM[a] ? bM[a] - 1 : c);
-    }
-    return (ci[0] * bM[1] + ci[1]) * bM[2] + ci[2];
-}
 __global__ void __launch_bounds__(1024) k_bin_count(Dev D) {
     const int r = blockIdx.y;
     if (!D.rs[r].need_rebuild) return;
@@ -390,8 +384,10 @@ __global__ void __launch_bounds__(1024) k_bin_count(Dev D) {
     int cell = 0, lrank = 0;
     if (i < n) {
         const double xs[3] = {x.x, x.y, x.z};
-        cell = cell_index(xs, bL, blc, binv, bM);
+        bool outside = false;
+        cell = cell_index(xs, bL, blc, binv, bM, &outside);
         lrank = atomicAdd(&lcnt[cell], 1);
+        if (outside) atomicOr(&D.rs[r].crop_off, 1);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < nc; c += 1024) {
@@ -463,10 +459,13 @@ __global__ void __launch_bounds__(1024) k_bin_place(Dev D) {
 // the neighbourhood is at most 5 x 5 columns x 2 pieces = 50 contiguous ranges of
 // particles, one per lane: lane l < *nseg gets its range's first cell-order index
 // and length (0 for an empty piece).  An axis with fewer than 2h + 1 cells
-// contributes each of its cells once.
+// contributes each of its cells once.  With rc2 >= 0 the ranges are cut to the
+// cells that can hold a point within sqrt(rc2) of xi, the particle's position
+// (every particle lies in the cell the sort gave it unless the replica's
+// crop_off is set; rc2 < 0 takes the whole neighbourhood).
 template <class L>
-TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, int *nseg, int *sstart,
-                             int *slen) {
+TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, const double4 &xi, double rc2,
+                             int *nseg, int *sstart, int *slen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
     const int w = 2 * h + 1;
@@ -478,12 +477,45 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, 
     const int col = lane >> 1, piece = lane & 1;
     if (col < wx * wy) {
         const int ix = col / wy, iy = col % wy;
+        // cropping (rc2 >= 0): a neighbour cell u along an axis of at least 2h + 1
+        // cells lies at least gap(u) = max(0, u lc - t, t - (u + 1) lc) from the
+        // particle (t: its coordinate wrapped once, as the cell sort wraps it; u
+        // unwrapped, so this image is the nearest); columns whose x and y gaps
+        // exceed the radius are skipped and the z range is cut to the cells
+        // within the radius left
+        const bool crop = rc2 >= 0.0;
+        double rem = rc2;
+        auto gap = [&](int a, double x, int u) {
+            double t = x;
+            if (t < 0) t += P.L[a];
+            else if (t > P.L[a]) t -= P.L[a];
+            const double lc = P.lc[a];
+            return fmax(0.0, fmax(u * lc - t, t - (u + 1) * lc));
+        };
         int n0 = ix, n1 = iy;
-        if (M0 >= w) { n0 = c0 - h + ix; n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0; }
-        if (M1 >= w) { n1 = c1 - h + iy; n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0; }
+        if (M0 >= w) {
+            n0 = c0 - h + ix;
+            if (crop) { const double g = gap(0, xi.x, n0); rem -= g * g; }
+            n0 += n0 < 0 ? M0 : 0; n0 -= n0 >= M0 ? M0 : 0;
+        }
+        if (M1 >= w) {
+            n1 = c1 - h + iy;
+            if (crop) { const double g = gap(1, xi.y, n1); rem -= g * g; }
+            n1 += n1 < 0 ? M1 : 0; n1 -= n1 >= M1 ? M1 : 0;
+        }
         int zlo = 0, zhi = M2 - 1;           // this piece's z range (zhi < zlo: empty)
-        if (M2 >= w) {
-            const int lo = c2 - h, hi = c2 + h;
+        if (crop && rem < 0.0) {
+            zhi = -1;
+        } else if (M2 >= w) {
+            int lo = c2 - h, hi = c2 + h;
+            if (crop) {
+                double t = xi.z;
+                if (t < 0) t += P.L[2];
+                else if (t > P.L[2]) t -= P.L[2];
+                const double sz = sqrt(rem), lc = P.lc[2];
+                lo = max(lo, min(c2, (int)ceil((t - sz) / lc) - 1));
+                hi = min(hi, max(c2, (int)floor((t + sz) / lc)));
+            }
             if (lo < 0) {
                 zlo = piece ? lo + M2 : 0;
                 zhi = piece ? M2 - 1 : hi;
@@ -518,10 +550,11 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, 
 // first entry and is marked by q = -1), so that the loads form a fixed sequence
 // the compiler can count in flight instead of waiting for all of them.
 template <class L, class F>
-TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, int h, F &&visit) {
+TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, int h, const double4 &xi, double rc2,
+                                      F &&visit) {
     const int lane = threadIdx.x & 63;
     int ncell, cstart, clen;
-    neighbour_cells<L>(D, P, cell, h, &ncell, &cstart, &clen);
+    neighbour_cells<L>(D, P, cell, h, xi, rc2, &ncell, &cstart, &clen);
     const int incl = wave_incl_scan(clen);
     const int total = readlane_i(incl, 63);
     if (total == 0) return;
@@ -588,6 +621,11 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
         const double rsr2 = P.rsr2, rlr2 = P.rlr2;
         const int cap_sr = P.cap_sr, cap_lr = P.cap_lr, p0 = P.p0, n = P.n, sr_h = P.sr_h;
         const long long pool0 = P.pool0;
+        // search radii of the cell cropping, 1e-3 A beyond the list radii (each
+        // particle lies within 1e-12 A of its cell); -1: no cropping
+        const bool crop = !L::ld(&S.crop_off);
+        const double rc_sr = sqrt(rsr2) + 1e-3, rc_lr = sqrt(rlr2) + 1e-3;
+        const double rc2_sr = crop ? rc_sr * rc_sr : -1.0, rc2_lr = crop ? rc_lr * rc_lr : -1.0;
         const long long stride = (long long)cap_sr + cap_lr;
         for (int t = w0; t < n; t += nw) {
             const double4 xi = L::ld(D.cpos + p0 + t);
@@ -604,7 +642,8 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
             int a = 0, b = 0;
             // an uncharged particle gets no long-range list: its short-range
             // partners lie in the cells next to its own
-            for_neighbour_cells_wave<L>(D, P, cell, chg_i ? 2 : sr_h, [&](int j, const double4 &xj, int qj) {
+            for_neighbour_cells_wave<L>(D, P, cell, chg_i ? 2 : sr_h, xi, chg_i ? rc2_lr : rc2_sr,
+                                        [&](int j, const double4 &xj, int qj) {
                 const double r2 = min_image_r2(xi, xj, bL, bhL);
                 const bool cand = j >= 0 && j != p;
                 const bool isr = cand && r2 <= rsr2;
