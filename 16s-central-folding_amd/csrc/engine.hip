@@ -732,7 +732,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
                 P.L[a] = s.box[a];
                 P.hL[a] = 0.5 * s.box[a];
                 // cells at least r_lr / 2 wide (the list build searches +- 2 cells),
-                // at most 20 per axis (the LDS counting sort holds kMaxCells)
+                // at most 20 per axis (the LDS counting sorts hold 2 kMaxCells)
                 P.M[a] = std::min(20, std::max(1, (int)(2.0 * s.box[a] / o.r_lr)));
                 P.lc[a] = s.box[a] / P.M[a];
             }
@@ -748,7 +748,7 @@ extern "C" int tis_create(const tis_system *reps, int n_rep, const tis_options *
             type_tables(P, s.type_R, s.type_eps, s.type_q, s.type_mass, s.lB);
             p0 += s.n_particles; b0 += s.n_bonds; a0 += s.n_angles; s30 += s.n_st3; s20 += s.n_st2;
             h0 += s.n_hb; si0 += s.n_hb_sites; i0 += s.n_hb_incid;
-            cell0 += P.M[0] * P.M[1] * P.M[2] + 1;
+            cell0 += 2 * P.M[0] * P.M[1] * P.M[2] + 1;   // charged and uncharged parts of the cell order
         }
         const int N = p0;
         Dev &D = c->D;

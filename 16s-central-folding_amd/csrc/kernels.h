@@ -218,12 +218,18 @@ TIS_DEV int cell_index(const double x[3], const double bL[3], const double blc[3
     }
     return (ci[0] * bM[1] + ci[1]) * bM[2] + ci[2];
 }
-// Counting sort of replica r's particles into cells by one workgroup (any
-// multiple of 64 threads up to 1024); cnt holds kMaxCells ints, scratch 20, ord
-// ord_cap 16-bit entries (the cell order, when the replica has at most ord_cap
-// particles).
+// The cell order holds each replica's charged particles first, then its
+// uncharged ones, each part sorted by cell: the sort runs over 2 nc "virtual"
+// cells, cell c of the charged part being c and of the uncharged part nc + c,
+// so the long-range search of a charged particle (charged partners only) walks
+// the charged part alone.
+TIS_DEV int virtual_cell(int cell, int nc, int ptype) { return ((ptype >> 8) & 1) ? cell : nc + cell; }
+
+// Counting sort of replica r's particles into virtual cells by one workgroup
+// (any multiple of 64 threads up to 1024); cnt holds 2 kMaxCells ints, scratch
+// 20, ord ord_cap 16-bit entries (the cell order, when the replica has at most
+// ord_cap particles).
 constexpr int kBinBatch = 8;
-constexpr int kBinOrd = 49152;   // LDS cell-order entries of k_bin
 template <class L>
 TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned short *ord, int ord_cap) {
     RepState &S = D.rs[r];
@@ -234,16 +240,18 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
     const double bL[3] = {P.L[0], P.L[1], P.L[2]}, blc[3] = {P.lc[0], P.lc[1], P.lc[2]};
     const double binv[3] = {1.0 / blc[0], 1.0 / blc[1], 1.0 / blc[2]};
     const int bM[3] = {M0, M1, M2};
-    const int p0 = P.p0, n = P.n, cell0 = P.cell0;
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) cnt[c] = 0;
+    const int p0 = P.p0, n = P.n, cell0 = P.cell0, nv = 2 * nc;
+    for (int c = threadIdx.x; c < nv; c += blockDim.x) cnt[c] = 0;
     __syncthreads();
     // kBinBatch particles per thread at a time, their loads issued together
     for (int i0 = threadIdx.x; i0 < n; i0 += kBinBatch * blockDim.x) {
         double4 xb[kBinBatch];
+        int tb[kBinBatch];
 #pragma unroll
         for (int k = 0; k < kBinBatch; ++k) {
             const int i = i0 + k * blockDim.x;
             xb[k] = i < n ? L::ld(D.pos + p0 + i) : make_double4(0, 0, 0, 0);
+            tb[k] = i < n ? D.ptype[p0 + i] : 0;
         }
 #pragma unroll
         for (int k = 0; k < kBinBatch; ++k) {
@@ -252,7 +260,7 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
             const int p = p0 + i;
             const double xs[3] = {xb[k].x, xb[k].y, xb[k].z};
             bool outside = false;
-            const int cell = cell_index(xs, bL, blc, binv, bM, &outside);
+            const int cell = virtual_cell(cell_index(xs, bL, blc, binv, bM, &outside), nc, tb[k]);
             if (outside) atomicOr(&S.crop_off, 1);
             D.cell_of[p] = cell;
             D.cell_rank[p] = atomicAdd(&cnt[cell], 1);
@@ -260,19 +268,19 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
     }
     __syncthreads();
     // exclusive scan of the cell counts: contiguous chunk per thread, then block scan
-    const int per = (nc + blockDim.x - 1) / blockDim.x;
+    const int per = (nv + blockDim.x - 1) / blockDim.x;
     const int c0 = threadIdx.x * per;
     int local = 0;
-    for (int c = c0; c < c0 + per && c < nc; ++c) local += cnt[c];
+    for (int c = c0; c < c0 + per && c < nv; ++c) local += cnt[c];
     int total;
     int run = block_excl_sum(local, scratch, &total);
-    for (int c = c0; c < c0 + per && c < nc; ++c) {
+    for (int c = c0; c < c0 + per && c < nv; ++c) {
         int v = cnt[c];
         cnt[c] = run;
         D.cell_start[cell0 + c] = run;
         run += v;
     }
-    if (threadIdx.x == 0) D.cell_start[cell0 + nc] = total;
+    if (threadIdx.x == 0) D.cell_start[cell0 + nv] = total;
     __syncthreads();
     if (n <= ord_cap) {
         // the cell order in LDS first (ord[q - p0] = p - p0), then the cell-ordered
@@ -348,15 +356,6 @@ TIS_DEV void bin_replica(const Dev &D, int r, int *cnt, int *scratch, unsigned s
         S.max_lr = 0;
     }
 }
-__global__ void __launch_bounds__(1024) k_bin(Dev D) {
-    const int r = blockIdx.x;
-    if (!D.rs[r].need_rebuild) return;
-    __shared__ int cnt[kMaxCells];
-    __shared__ int scratch[20];
-    __shared__ unsigned short ord[kBinOrd];
-    bin_replica<LdPlain>(D, r, cnt, scratch, ord, kBinOrd);
-}
-
 // The same sort in three launches over many workgroups (a replica's particles
 // need not wait on one CU).  k_bin_count, grid (ceil(n_max / 1024), replicas):
 // each workgroup ranks its 1024 particles within their cells in LDS, then
@@ -372,25 +371,29 @@ __global__ void __launch_bounds__(1024) k_bin_count(Dev D) {
     const RepParams &P = D.rp[r];
     const int n = P.n, p0 = P.p0, cell0 = P.cell0;
     if ((int)(blockIdx.x * 1024) >= n) return;
-    const int bM[3] = {P.M[0], P.M[1], P.M[2]}, nc = bM[0] * bM[1] * bM[2];
+    const int bM[3] = {P.M[0], P.M[1], P.M[2]}, nc = bM[0] * bM[1] * bM[2], nv = 2 * nc;
     const double bL[3] = {P.L[0], P.L[1], P.L[2]}, blc[3] = {P.lc[0], P.lc[1], P.lc[2]};
     const double binv[3] = {1.0 / blc[0], 1.0 / blc[1], 1.0 / blc[2]};
-    __shared__ int lcnt[kMaxCells];
+    __shared__ int lcnt[2 * kMaxCells];
     const int i = blockIdx.x * 1024 + threadIdx.x;
     double4 x = make_double4(0, 0, 0, 0);
-    if (i < n) x = D.pos[p0 + i];
-    for (int c = threadIdx.x; c < nc; c += 1024) lcnt[c] = 0;
+    int pt = 0;
+    if (i < n) {
+        x = D.pos[p0 + i];
+        pt = D.ptype[p0 + i];
+    }
+    for (int c = threadIdx.x; c < nv; c += 1024) lcnt[c] = 0;
     __syncthreads();
     int cell = 0, lrank = 0;
     if (i < n) {
         const double xs[3] = {x.x, x.y, x.z};
         bool outside = false;
-        cell = cell_index(xs, bL, blc, binv, bM, &outside);
+        cell = virtual_cell(cell_index(xs, bL, blc, binv, bM, &outside), nc, pt);
         lrank = atomicAdd(&lcnt[cell], 1);
         if (outside) atomicOr(&D.rs[r].crop_off, 1);
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < nc; c += 1024) {
+    for (int c = threadIdx.x; c < nv; c += 1024) {
         const int k = lcnt[c];
         if (k > 0) lcnt[c] = atomicAdd(&D.cell_cnt[cell0 + c], k);
     }
@@ -405,26 +408,42 @@ __global__ void __launch_bounds__(1024) k_bin_scan(Dev D) {
     RepState &S = D.rs[r];
     if (!S.need_rebuild) return;
     const RepParams &P = D.rp[r];
-    const int nc = P.M[0] * P.M[1] * P.M[2], cell0 = P.cell0;
+    const int nc = 2 * P.M[0] * P.M[1] * P.M[2], cell0 = P.cell0;   // virtual cells
     __shared__ int scratch[20];
-    const int per = (nc + 1023) / 1024, c0 = threadIdx.x * per;
-    constexpr int kPer = (kMaxCells + 1023) / 1024;
-    int v[kPer], local = 0;
+    __shared__ int sc[2 * kMaxCells];
+    // the counts staged through LDS so that global loads and stores are
+    // consecutive across lanes (a thread's contiguous run of cells would
+    // scatter them)
+    constexpr int kPer = (2 * kMaxCells + 1023) / 1024;
+    int v[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-        const int c = c0 + k;
-        v[k] = k < per && c < nc ? D.cell_cnt[cell0 + c] : 0;
-        local += v[k];
+        const int c = threadIdx.x + 1024 * k;
+        v[k] = c < nc ? D.cell_cnt[cell0 + c] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int c = threadIdx.x + 1024 * k;
+        if (c < nc) sc[c] = v[k];
+    }
+    __syncthreads();
+    const int per = (nc + 1023) / 1024, c0 = threadIdx.x * per;
+    int local = 0;
+    for (int c = c0; c < c0 + per && c < nc; ++c) local += sc[c];
     int total;
     int run = block_excl_sum(local, scratch, &total);
+    for (int c = c0; c < c0 + per && c < nc; ++c) {
+        const int x = sc[c];
+        sc[c] = run;
+        run += x;
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-        const int c = c0 + k;
-        if (k < per && c < nc) {
-            D.cell_start[cell0 + c] = run;
+        const int c = threadIdx.x + 1024 * k;
+        if (c < nc) {
+            D.cell_start[cell0 + c] = sc[c];
             D.cell_cnt[cell0 + c] = 0;   // cleared for the next rebuild
-            run += v[k];
         }
     }
     if (threadIdx.x == 0) {
@@ -462,20 +481,22 @@ __global__ void __launch_bounds__(1024) k_bin_place(Dev D) {
 // contributes each of its cells once.  With rc2 >= 0 the ranges are cut to the
 // cells that can hold a point within sqrt(rc2) of xi, the particle's position
 // (every particle lies in the cell the sort gave it unless the replica's
-// crop_off is set; rc2 < 0 takes the whole neighbourhood).
+// crop_off is set; rc2 < 0 takes the whole neighbourhood).  seg: the charged
+// (0) or uncharged (1) part of the cell order.  The ranges go to lanes lane0,
+// lane0 + 1, ... (the other lanes keep theirs, so two calls can fill one
+// wavefront); returns the lane after the last.
 template <class L>
-TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, const double4 &xi, double rc2,
-                             int *nseg, int *sstart, int *slen) {
+TIS_DEV int neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, int seg, const double4 &xi,
+                            double rc2, int lane0, int *sstart, int *slen) {
     const int M0 = P.M[0], M1 = P.M[1], M2 = P.M[2];
     const int c2 = cell % M2, c1 = (cell / M2) % M1, c0 = cell / (M1 * M2);
     const int w = 2 * h + 1;
     const int wx = M0 < w ? M0 : w, wy = M1 < w ? M1 : w;
-    *nseg = 2 * wx * wy;
     const int lane = threadIdx.x & 63;
-    *sstart = 0;
-    *slen = 0;
-    const int col = lane >> 1, piece = lane & 1;
-    if (col < wx * wy) {
+    const int col = (lane - lane0) >> 1, piece = (lane - lane0) & 1;
+    if (lane >= lane0 && col < wx * wy) {
+        *sstart = 0;
+        *slen = 0;
         const int ix = col / wy, iy = col % wy;
         // cropping (rc2 >= 0): a neighbour cell u along an axis of at least 2h + 1
         // cells lies at least gap(u) = max(0, u lc - t, t - (u + 1) lc) from the
@@ -530,16 +551,18 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, 
             zhi = -1;
         }
         if (zhi >= zlo) {
-            const int base = P.cell0 + (n0 * M1 + n1) * M2;
+            const int base = P.cell0 + seg * M0 * M1 * M2 + (n0 * M1 + n1) * M2;
             *sstart = L::ld(D.cell_start + base + zlo);
             *slen = L::ld(D.cell_start + base + zhi + 1) - *sstart;
         }
     }
+    return lane0 + 2 * wx * wy;
 }
 
-// Visits every particle of the neighbour cells, 64 per call of visit(j, x, q)
-// (particle j, its cell-ordered position x at global cell-order index q), lanes
-// past the last candidate getting j = -1.  The cells' particles are taken
+// Visits every particle of the ranges that lanes 0 .. ncell - 1 hold (from
+// neighbour_cells), 64 per call of visit(j, x, q) (particle j, its cell-ordered
+// position x at global cell-order index q), lanes past the last candidate
+// getting j = -1.  The cells' particles are taken
 // as one sequence (cell c covers positions [excl_c, incl_c) of it, prefix sums
 // over the lanes that hold the cells), so every chunk of 64 lanes is full
 // whatever the cell occupancies; a lane finds its cell by a scalar walk over the
@@ -550,11 +573,8 @@ TIS_DEV void neighbour_cells(const Dev &D, const RepParams &P, int cell, int h, 
 // first entry and is marked by q = -1), so that the loads form a fixed sequence
 // the compiler can count in flight instead of waiting for all of them.
 template <class L, class F>
-TIS_DEV void for_neighbour_cells_wave(const Dev &D, const RepParams &P, int cell, int h, const double4 &xi, double rc2,
-                                      F &&visit) {
+TIS_DEV void for_cell_ranges_wave(const Dev &D, const RepParams &P, int ncell, int cstart, int clen, F &&visit) {
     const int lane = threadIdx.x & 63;
-    int ncell, cstart, clen;
-    neighbour_cells<L>(D, P, cell, h, xi, rc2, &ncell, &cstart, &clen);
     const int incl = wave_incl_scan(clen);
     const int total = readlane_i(incl, 63);
     if (total == 0) return;
@@ -620,6 +640,7 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
         const double bL[3] = {P.L[0], P.L[1], P.L[2]}, bhL[3] = {P.hL[0], P.hL[1], P.hL[2]};
         const double rsr2 = P.rsr2, rlr2 = P.rlr2;
         const int cap_sr = P.cap_sr, cap_lr = P.cap_lr, p0 = P.p0, n = P.n, sr_h = P.sr_h;
+        const int nc = P.M[0] * P.M[1] * P.M[2];
         const long long pool0 = P.pool0;
         // search radii of the cell cropping, 1e-3 A beyond the list radii (each
         // particle lies within 1e-12 A of its cell); -1: no cropping
@@ -631,7 +652,7 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
             const double4 xi = L::ld(D.cpos + p0 + t);
             const int p = cp_index(xi.w);
             const bool chg_i = (cp_type(xi.w) >> 8) & 1;
-            const int cell = L::ld(D.cell_of + p);
+            const int vcell = L::ld(D.cell_of + p), cell = vcell >= nc ? vcell - nc : vcell;
             // entries are cell-order indices q of the partners; bonded / 1-3
             // partners of p are stored as ~q, so the step path leaves out their WCA term
             int ex[kExcl];
@@ -640,10 +661,7 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
             int *out_sr = D.pool + pool0 + (long long)(p - p0) * stride;
             int *out_lr = out_sr + cap_sr;
             int a = 0, b = 0;
-            // an uncharged particle gets no long-range list: its short-range
-            // partners lie in the cells next to its own
-            for_neighbour_cells_wave<L>(D, P, cell, chg_i ? 2 : sr_h, xi, chg_i ? rc2_lr : rc2_sr,
-                                        [&](int j, const double4 &xj, int qj) {
+            auto visit = [&](int j, const double4 &xj, int qj) {
                 const double r2 = min_image_r2(xi, xj, bL, bhL);
                 const bool cand = j >= 0 && j != p;
                 const bool isr = cand && r2 <= rsr2;
@@ -665,7 +683,31 @@ TIS_DEV void list_replica(const Dev &D, int r, int w0, int nw) {
                 }
                 a += __popcll(ms);
                 b += __popcll(ml);
-            });
+            };
+            // a charged particle: the charged part within r_lr (its long-range
+            // partners and its charged short-range ones), then the uncharged part
+            // within r_sr; an uncharged particle: both parts within r_sr (one
+            // walk when both fit a wavefront's lanes)
+            int cs = 0, cl = 0;
+            if (chg_i) {
+                int nr = neighbour_cells<L>(D, P, cell, 2, 0, xi, rc2_lr, 0, &cs, &cl);
+                for_cell_ranges_wave<L>(D, P, nr, cs, cl, visit);
+                cs = 0;
+                cl = 0;
+                nr = neighbour_cells<L>(D, P, cell, sr_h, 1, xi, rc2_sr, 0, &cs, &cl);
+                for_cell_ranges_wave<L>(D, P, nr, cs, cl, visit);
+            } else if (sr_h == 1) {
+                int nr = neighbour_cells<L>(D, P, cell, 1, 0, xi, rc2_sr, 0, &cs, &cl);
+                nr = neighbour_cells<L>(D, P, cell, 1, 1, xi, rc2_sr, nr, &cs, &cl);
+                for_cell_ranges_wave<L>(D, P, nr, cs, cl, visit);
+            } else {
+                for (int seg = 0; seg < 2; ++seg) {
+                    cs = 0;
+                    cl = 0;
+                    const int nr = neighbour_cells<L>(D, P, cell, sr_h, seg, xi, rc2_sr, 0, &cs, &cl);
+                    for_cell_ranges_wave<L>(D, P, nr, cs, cl, visit);
+                }
+            }
             if (lane == 0) {
                 if (MODE == 0) {
                     atomicMax(&S.max_sr, a);

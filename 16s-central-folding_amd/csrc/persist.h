@@ -94,8 +94,9 @@ TIS_DEV bool persist_step(const Dev &D, const PArgs &A, int r, int m, unsigned c
     const bool rebuild = LdCoh::ld(&S.need_rebuild) != 0;
     // ---- list rebuild (the trigger fired at the end of the last step)
     if (rebuild) {
-        if (!HB && m == G - 1) bin_replica<LdCoh>(D, r, (int *)lds, (int *)lds + kMaxCells, (unsigned short *)((int *)lds + kMaxCells + 32),
-                                                   (int)((kHbLds - (kMaxCells + 32) * sizeof(int)) / 2));
+        if (!HB && m == G - 1) bin_replica<LdCoh>(D, r, (int *)lds, (int *)lds + 2 * kMaxCells,
+                                                   (unsigned short *)((int *)lds + 2 * kMaxCells + 32),
+                                                   (int)((kHbLds - (2 * kMaxCells + 32) * sizeof(int)) / 2));
         if (!bar_all()) return false;
         list_replica<LdCoh, 1>(D, r, m * (kPBlock / 64) + wv, G * (kPBlock / 64));
         if (!bar_all()) return false;
