@@ -619,7 +619,7 @@ TIS_DEV void for_cell_ranges_wave(const Dev &D, const RepParams &P, int ncell, i
 }
 
 // ------------------------------------------------------------ k_list
-// One wavefront per particle: the lanes test 64 candidates at a time, two
+// One wavefront per particle: the lanes test 64 candidates at a time, four
 // chunks in flight.  Every particle owns a fixed slot of the pool (cap_sr SR
 // then cap_lr LR entries), so there is no counting pass and no shared cursor.
 // The grid is fixed (kListWaves wavefronts) and strides over the particles of
