@@ -1,18 +1,25 @@
 // Kernels of one Langevin step (fast.cpp:6380-6386) for a batch of replicas.
 //
-//   k_hb          one workgroup per replica: HB energies, valence resolution,
-//                 HB forces; reference-stream noise table (hydrogen_bonds_interactions
-//                 fast.cpp:2145-2350, full_forces draws :2598-2604)
-//   k_force       stacking (one workgroup per replica), backbone (thread per term),
-//                 short/long-range pair gather (thread per particle)
-//                 (stacking_interactions :1911-2141, polymer_constraints :2354-2520,
-//                 SR_/LR_interactions_lists :1783-1907)
-//   k_integrate   noise, leap-frog with speed cap, recentring, ion wrap, displacement
-//                 reduction (full_forces :2590-2625, move_rigid_units :2660-2732)
-//   (k_integrate  closes each replica's step: rebuild decision, check_shifts :1741-1776)
-//   k_bin         LDS counting sort of particles into cells (separate_in_cells :1461-1542)
-//   k_list        full neighbour lists, SR and LR, from the 27 neighbouring cells
-//                 (populate_lists :1546-1700)
+//   k_hbenergy    gate, energy and six bead forces of every live HB candidate
+//                 (hydrogen_bonds_interactions fast.cpp:2145-2237)
+//   k_hbres       valence resolution, grid (hb_groups, replicas): components of the
+//                 over-subscribed sites on wavefronts, forces of the live candidates
+//                 added and those switched off taken back, energy sums; the
+//                 reference-stream noise table (:2239-2350, full_forces draws :2598-2604)
+//   k_multibody   stacks, bonds and angles in slices of each replica's beads, forces
+//                 summed in LDS (stacking_interactions :1911-2141, polymer_constraints
+//                 :2354-2520)
+//   k_noise       the step's Philox random forces (full_forces :2590-2625)
+//   k_force       pair gather, SR every step and LR when due (SR_/LR_interactions_lists
+//                 :1783-1907); evaluations also use it for stacks and backbone terms
+//   k_integrate   leap-frog with speed cap, recentring, ion wrap, displacement
+//                 reduction (move_rigid_units :2660-2732), then each replica's step
+//                 close: rebuild decision (check_shifts :1741-1776)
+//   k_bin_count,  cell sort of a rebuild in two parts (charged, uncharged):
+//   k_bin_scan,   counts, cell starts, cell-ordered copy (separate_in_cells
+//   k_bin_place   :1461-1542)
+//   k_list        full SR and LR neighbour lists from the cells within each
+//                 particle's list radius (populate_lists :1546-1700)
 //   k_hbstatus,   live HB candidates and the per-site tables of the resolution
 //   k_hbsites     (populate_lists :1702-1736)
 #pragma once
@@ -195,7 +202,7 @@ TIS_DEV double wca_energy(double r2, int ab, const RepParams &P) {
     return P.elj[ab] * (u12 - 2.0 * u6 + 1.0);
 }
 
-// ------------------------------------------------------------ k_bin
+// ------------------------------------------------------------ cell sort
 // The cell of position x (separate_in_cells, fast.cpp:1510-1530): each
 // coordinate wrapped once into the box, cell ceil(t / lc) - 1 clamped to the
 // grid, the quotient from the reciprocal, and from the division itself when it
