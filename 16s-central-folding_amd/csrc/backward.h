@@ -1,4 +1,4 @@
-// Force-matching backward pass (no counterpart in the reference; SURVEY row a14).
+// Force-matching backward pass (no counterpart in the reference).
 //
 // Loss L = (1/M) sum_i |F_i(x; theta) - F_i^target|^2 over every force
 // component of every replica (M components), with F the step's forces

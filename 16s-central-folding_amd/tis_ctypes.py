@@ -143,7 +143,7 @@ def _length(expr, d):
 
 def load_library(path=LIB_PATH):
     if not os.path.exists(path):
-        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() first")
+        raise RuntimeError(f"{path} is not built: run make -C 16s-central-folding_amd first")
     lib = C.CDLL(path)
     lib.tis_last_error.restype = C.c_char_p
     lib.tis_build_16scd.argtypes = [C.c_char_p, C.POINTER(BuildArgs), C.POINTER(C.POINTER(TisSystem)),

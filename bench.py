@@ -127,10 +127,10 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline_reference(steps_one, steps_all, configs=None):
+def cpu_baseline_reference(steps_one, steps_all, cases=None):
     """The reference's own step sequence (its translation unit compiled by
     oracle/Makefile into oracle/_ref/ref_harness, one MPI rank per process) on
-    the replica configurations `configs` ((c_Mg, box, seed, c_K); default the 16
+    the replica configurations `cases` ((c_Mg, box, seed, c_K); default the 16
     of the batch).  Returns None when that binary was not built, else a dict:
       one_core   replica-steps/s with the configurations run one after another
                  on one core (`steps_one` steps each);
@@ -146,17 +146,17 @@ def cpu_baseline_reference(steps_one, steps_all, configs=None):
                  (mpiexec of the installed MPICH), or None without mpiexec."""
     if not os.access(REF_EXE, os.X_OK):
         return None
-    configs = configs or [(c, b, s, 0.050) for c, b, s in batched_params()]
+    cases = cases or [(c, b, s, 0.050) for c, b, s in batched_params()]
     run = lambda cmd: subprocess.run(cmd, cwd=REF_DATA, capture_output=True, text=True, timeout=900,
                                      check=True).stdout
     rates_one = []
-    for c_mg, box, seed, c_k in configs:
+    for c_mg, box, seed, c_k in cases:
         rates_one.append(_ref_rate(run(_ref_cmd(c_mg, box, seed, steps_one, c_k))))
         log(f"reference CPU, one core: c_Mg {c_mg * 1e3:.2f} mM box {box:.1f} A: {rates_one[-1]:.0f} steps/s")
-    one_core = len(configs) / sum(1.0 / r for r in rates_one)
+    one_core = len(cases) / sum(1.0 / r for r in rates_one)
     cores = host_cores()
     rates_all = []
-    pending = list(configs)
+    pending = list(cases)
     while pending:
         batch, pending = pending[:cores], pending[cores:]
         procs = [subprocess.Popen(_ref_cmd(c_mg, box, seed, steps_all, c_k), cwd=REF_DATA, stdout=subprocess.PIPE,
@@ -166,17 +166,17 @@ def cpu_baseline_reference(steps_one, steps_all, configs=None):
             if p.returncode != 0:
                 raise RuntimeError("reference harness failed")
             rates_all.append(_ref_rate(out))
-    if cores >= len(configs):
+    if cores >= len(cases):
         all_cores = sum(rates_all)
     else:
         all_cores = cores * len(rates_all) / sum(1.0 / r for r in rates_all)
-    log(f"reference CPU, {min(cores, len(configs))} concurrent processes: {all_cores:.0f} replica-steps/s")
+    log(f"reference CPU, {min(cores, len(cases))} concurrent processes: {all_cores:.0f} replica-steps/s")
     mpi = None
     mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.access("/opt/conda/bin/mpiexec", os.X_OK)
                                            else None)
     if mpiexec:
         mpi = {}
-        c_mg, box, seed, c_k = configs[0]
+        c_mg, box, seed, c_k = cases[0]
         n = 1
         while n <= cores:
             try:
@@ -187,7 +187,7 @@ def cpu_baseline_reference(steps_one, steps_all, configs=None):
                 break
             log(f"reference CPU, mpiexec -n {n}: {mpi[str(n)]:.0f} steps/s")
             n *= 2
-    return {"one_core": one_core, "all_cores": all_cores, "cores": min(cores, len(configs)), "mpi": mpi,
+    return {"one_core": one_core, "all_cores": all_cores, "cores": min(cores, len(cases)), "mpi": mpi,
             "steps_one": steps_one, "steps_all": steps_all}
 
 
