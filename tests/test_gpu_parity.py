@@ -67,6 +67,37 @@ def test_pair_sets_and_hb_status(lib, sys1, oracle1):
     eng.close()
 
 
+def test_pair_sets_with_particles_one_wrap_leaves_outside_the_box(lib, sys1):
+    """The cell sort wraps each coordinate once (separate_in_cells,
+    fast.cpp:1510-1530), so a particle two box lengths away stays outside the
+    box and its cell is clamped to the grid: it does not lie inside its cell,
+    and the list search cut to the cells within the list radius of each
+    particle must fall back to whole cells for the replica.  Two ions placed so
+    (+2 L in x, -1.5 L in y): SR and LR pair sets bit-exact and pair forces
+    within 1e-10 (FP64) of the oracle at the same state."""
+    d = sys1.d
+    L = np.asarray(d["box"], dtype=np.float64)
+    pos = np.array(d["pos"], dtype=np.float64).reshape(-1, 3)
+    old = np.array(d["pos_old"], dtype=np.float64).reshape(-1, 3)
+    vel = np.array(d["vel"], dtype=np.float64).reshape(-1, 3)
+    k1, k2 = d["n_beads"], d["n_particles"] - 1
+    for k, ax, s in ((k1, 0, 2.0), (k2, 1, -1.5)):
+        pos[k, ax] += s * L[ax]
+        old[k, ax] += s * L[ax]
+    o = O.Oracle(sys1)
+    o.set_state(pos, old, vel)
+    eng = T.Engine(lib, [sys1], precision=T.PREC_FP64)
+    eng.set_state(pos, old, vel)
+    assert np.array_equal(eng.pairs(0), o.pairs(0))
+    assert np.array_equal(eng.pairs(1), o.pairs(1))
+    for name, mask in (("sr", T.TERM_SR), ("lr", T.TERM_LR)):
+        f, _ = eng.eval(mask)
+        _, fo, _ = o.eval(mask)
+        print(name, "rel", rel(f, fo))
+        assert rel(f, fo) < 1e-10
+    eng.close()
+
+
 @pytest.mark.parametrize("precision,tol", [(T.PREC_FP64, 1e-10), (T.PREC_MIXED, 1e-5)])
 def test_hb_with_reference_stream(lib, sys1, precision, tol):
     raw = O.glibc_stream(12345, 4000)
