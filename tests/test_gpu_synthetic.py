@@ -97,3 +97,47 @@ def test_chain_philox_run_is_stable(lib, chain):
     assert e["klok"] == 1001
     assert e["n_rebuilds"] > 0
     eng.close()
+
+
+def test_chain_without_hb_candidates_or_stacks(lib):
+    """A 60-nt chain with no HB candidates and no stacks (586 particles, 150 A
+    box), its beads displaced by seeded N(0, 0.05 A) noise as in the chain
+    fixture (the built chain has zero backbone force): the empty HB tables,
+    resolution and stack slices must leave every term equal to the oracle's
+    (FP64, 1e-10 relative; the HB and stack terms exactly zero), the pair sets
+    bit-exact, and a Philox run of 500 steps in either precision finite with
+    zero HB and stack energies."""
+    d = T.build_synthetic(lib, n_nt=60, stack_frac=0.0, n_hb=0, radius=20.0, box=150.0)
+    assert d["n_hb"] == 0 and d["n_st3"] == 0 and d["n_st2"] == 0
+    rng = np.random.default_rng(12)
+    pos = d["pos"].reshape(-1, 3).copy()
+    pos[:d["n_beads"]] += rng.normal(0.0, 0.05, (d["n_beads"], 3))
+    vel = d["vel"].reshape(-1, 3)
+    state = (pos, pos - 0.05 * vel, vel)
+    h = T.SystemHandle(d)
+    o = O.Oracle(h)
+    o.set_state(*state)
+    print("case: no-HB chain, FP64 terms", flush=True)
+    eng = engine_at(lib, h, state, precision=T.PREC_FP64)
+    assert np.array_equal(eng.pairs(0), o.pairs(0))
+    assert np.array_equal(eng.pairs(1), o.pairs(1))
+    for name, mask in TERMS + [("hb", T.TERM_HB)]:
+        f, e = eng.eval(mask)
+        _, fo, eo = o.eval(mask)
+        if name in ("hb", "stack"):
+            assert not np.any(f) and not np.any(fo), name
+        else:
+            print(name, "rel", rel(f, fo))
+            assert rel(f, fo) < 1e-10, name
+    eng.close()
+    for prec in (T.PREC_FP64, T.PREC_MIXED):
+        print("case: no-HB chain, Philox run, precision", prec, flush=True)
+        eng = T.Engine(lib, [h], precision=prec)
+        eng.step(500)
+        p, _, v = eng.get_state()
+        e = eng.energies()
+        assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+        assert e["klok"] == 501
+        for k in ("e2_hb", "e3_hb", "e2_stack", "e3_stack"):
+            assert e[k] == 0.0, k
+        eng.close()
