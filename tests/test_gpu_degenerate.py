@@ -113,3 +113,20 @@ def test_non_finite_input_is_refused(lib, sys1):
         eng.set_state(pos, pos, np.zeros_like(pos))
     assert ei.value.code == -1 and "non-finite" in str(ei.value)
     eng.close()
+
+
+EOVERFLOW = -5
+
+
+def test_exhausted_reference_stream_stops_the_run(lib, sys1):
+    """A reference-stream run fed 6,000 glibc values, fewer than two steps draw
+    (the HB resolution's draws, then 3,000 for the step's noise table): draws
+    at or past the end read 0.5 inside the kernels (every read stays in the
+    stream's region) and the call stops with TIS_EOVERFLOW, naming the
+    exhausted stream, instead of running on."""
+    eng = T.Engine(lib, [sys1], precision=T.PREC_FP64, rng=T.RNG_STREAM)
+    eng.set_stream(T.glibc_stream(lib, 31, 6000))
+    with pytest.raises(T.TisError) as ei:
+        eng.step(10)
+    assert ei.value.code == EOVERFLOW and "stream exhausted" in str(ei.value)
+    eng.close()
